@@ -1,0 +1,229 @@
+"""Python front end over the C-ABI: handles, schedules and the numeric calls.
+
+All arrays are torch tensors on the GPU (``cuda`` device on ROCm).  The
+schedule/solve split mirrors the reference runtimes: ``Schedule(...)`` is
+``buildset()`` (test/FusionDemo.cpp:79-123), the numeric calls are
+``fused_code()``.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+
+def _ptr(t):
+    return C.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _dev_i32(a, device="cuda"):
+    if isinstance(a, torch.Tensor):
+        return a.to(device=device, dtype=torch.int32).contiguous()
+    return torch.as_tensor(np.ascontiguousarray(a, dtype=np.int32), device=device)
+
+
+def _dev_f(a, dtype=torch.float64, device="cuda"):
+    if isinstance(a, torch.Tensor):
+        return a.to(device=device, dtype=dtype).contiguous()
+    return torch.as_tensor(np.ascontiguousarray(a), device=device).to(dtype)
+
+
+class Handle:
+    """One library handle bound to torch's current stream."""
+
+    def __init__(self, stream=None):
+        L.load()
+        self._h = C.c_void_p()
+        s = stream if stream is not None else torch.cuda.current_stream()
+        L.call("baas_create", C.byref(self._h), C.c_void_p(s.cuda_stream))
+
+    @property
+    def ptr(self):
+        return self._h
+
+    def check(self):
+        st = L.load().baas_check(self._h)
+        if st != L.OK:
+            raise L.BaasError(st, "kernel")
+
+    def status(self) -> int:
+        return L.load().baas_check(self._h)
+
+    def set_spin_limit(self, polls: int):
+        L.call("baas_set_spin_limit", self._h, int(polls))
+
+    def __del__(self):
+        try:
+            if self._h:
+                L.load().baas_destroy(self._h)
+        except Exception:
+            pass
+
+
+class Schedule:
+    """BAAS analysis of an n x n pattern (lower L for TRSV, full A for IC0/ILU0)."""
+
+    def __init__(self, h: Handle, rowptr, colidx, kind=L.KIND_TRSV, ncore=8, plan_mode=L.PLAN_AUTO,
+                 rows_per_unit=0, block_threads=0):
+        self.h = h
+        self.rowptr = _dev_i32(rowptr)
+        self.colidx = _dev_i32(colidx)
+        self.n = self.rowptr.numel() - 1
+        self.kind = kind
+        o = L.Opts(ncore, plan_mode, rows_per_unit, block_threads, 0)
+        self._s = C.c_void_p()
+        L.call("baas_analyze", h.ptr, self.n, _ptr(self.rowptr), _ptr(self.colidx), kind, C.byref(o),
+               C.byref(self._s))
+
+    @property
+    def ptr(self):
+        return self._s
+
+    def info(self):
+        c = (C.c_int * 8)()
+        L.call("baas_schedule_info", self._s, C.cast(c, C.c_void_p))
+        keys = ("ngroups", "nlevels", "nmerged", "reduced_edges", "nrowlevels", "nunits", "nnz_lower",
+                "plan_mode")
+        return dict(zip(keys, list(c)))
+
+    def export(self, which) -> np.ndarray:
+        code = L.EXPORTS[which] if isinstance(which, str) else which
+        n = C.c_longlong()
+        L.call("baas_schedule_export", self._s, code, None, C.byref(n))
+        out = np.zeros(n.value, np.int32)
+        L.call("baas_schedule_export", self._s, code, C.c_void_p(out.ctypes.data), C.byref(n))
+        return out
+
+    def schedule_dict(self):
+        i = self.info()
+        d = {k: self.export(k) for k in ("group_ptr", "group_set", "level_ptr", "perm_to_orig",
+                                         "node_to_level", "nthreads", "merged_ptr")}
+        d.update(ngroups=i["ngroups"], nlevels=i["nlevels"], nmerged=i["nmerged"],
+                 reduced_edges=i["reduced_edges"])
+        return d
+
+    def times(self):
+        g, h = C.c_double(), C.c_double()
+        L.call("baas_schedule_times", self._s, C.byref(g), C.byref(h))
+        return g.value, h.value
+
+    def replan(self, mode, rows_per_unit=0, block_threads=0):
+        L.call("baas_replan", self._s, int(mode), int(rows_per_unit), int(block_threads))
+
+    def upper_pattern(self):
+        rp, ci = C.c_void_p(), C.c_void_p()
+        L.call("baas_schedule_upper_pattern", self._s, C.byref(rp), C.byref(ci))
+        return rp, ci
+
+    # ---- numerics
+    def solve(self, vals, b, x=None, uplo=0):
+        """x = L^-1 b (uplo 0, vals in L's layout) or U^-1 b (uplo 1, vals in U = L^T layout)."""
+        nrhs = 1 if b.dim() == 1 else b.shape[1]
+        if x is None:
+            x = torch.empty_like(b)
+        f = "baas_sptrsv_f64" if b.dtype == torch.float64 else "baas_sptrsv_f32"
+        L.call(f, self.h.ptr, self._s, int(uplo), _ptr(vals), _ptr(b), _ptr(x), int(nrhs))
+        return x
+
+    def solve_levelsync(self, vals, b, x=None):
+        if x is None:
+            x = torch.zeros_like(b)
+        L.call("baas_sptrsv_levelsync_f64", self.h.ptr, self._s, _ptr(vals), _ptr(b), _ptr(x))
+        return x
+
+    def transpose_values(self, lvals):
+        uv = torch.empty_like(lvals)
+        L.call("baas_transpose_values_f64", self.h.ptr, self._s, _ptr(lvals), _ptr(uv))
+        return uv
+
+    def grad_vals(self, lam, x):
+        nrhs = 1 if x.dim() == 1 else x.shape[1]
+        nnz = self.info()["nnz_lower"]
+        dv = torch.empty(nnz, dtype=torch.float64, device=x.device)
+        L.call("baas_sptrsv_grad_vals_f64", self.h.ptr, self._s, _ptr(lam), _ptr(x), _ptr(dv), int(nrhs))
+        return dv
+
+    def ic0(self, a_vals, lu=None):
+        if lu is None:
+            lu = torch.empty_like(a_vals)
+        L.call("baas_ic0_f64", self.h.ptr, self._s, _ptr(a_vals), _ptr(lu))
+        return lu
+
+    def ilu0(self, a_vals, lu=None):
+        if lu is None:
+            lu = torch.empty_like(a_vals)
+        L.call("baas_ilu0_f64", self.h.ptr, self._s, _ptr(a_vals), _ptr(lu))
+        return lu
+
+    def ic0_split(self, lu, dtype=torch.float64):
+        nnz = self.info()["nnz_lower"]
+        lv = torch.empty(nnz, dtype=dtype, device=lu.device)
+        uv = torch.empty(nnz, dtype=dtype, device=lu.device)
+        f = "baas_ic0_split_f64" if dtype == torch.float64 else "baas_ic0_split_f32"
+        L.call(f, self.h.ptr, self._s, _ptr(lu), _ptr(lv), _ptr(uv))
+        return lv, uv
+
+    def __del__(self):
+        try:
+            if self._s:
+                L.load().baas_schedule_destroy(self._s)
+        except Exception:
+            pass
+
+
+def spmv(h: Handle, rowptr, colidx, vals, x, y=None, alpha=1.0, beta=0.0):
+    m = rowptr.numel() - 1
+    if y is None:
+        y = torch.zeros(m, dtype=torch.float64, device=x.device)
+    L.call("baas_spmv_f64", h.ptr, m, _ptr(rowptr), _ptr(colidx), _ptr(vals), _ptr(x), _ptr(y),
+           float(alpha), float(beta))
+    return y
+
+
+def dot(h: Handle, x, y):
+    r = torch.zeros(1, dtype=torch.float64, device=x.device)
+    L.call("baas_dot_f64", h.ptr, x.numel(), _ptr(x), _ptr(y), _ptr(r))
+    return r
+
+
+def nrm2(h: Handle, x):
+    r = torch.zeros(1, dtype=torch.float64, device=x.device)
+    L.call("baas_nrm2_f64", h.ptr, x.numel(), _ptr(x), _ptr(r))
+    return r
+
+
+def axpby(h: Handle, w, alpha, x, beta, y):
+    L.call("baas_axpby_f64", h.ptr, w.numel(), _ptr(w), float(alpha), _ptr(x), float(beta), _ptr(y))
+    return w
+
+
+class PCG:
+    """IC0-preconditioned CG on the GPU (pcg_opt, test/iccg_level_merge.cpp:380-533)."""
+
+    def __init__(self, h: Handle, sched: Schedule, a_vals, precision=0):
+        self.h, self.s, self.a_vals = h, sched, a_vals
+        self._p = C.c_void_p()
+        L.call("baas_pcg_create", h.ptr, sched.ptr, _ptr(a_vals), int(precision), C.byref(self._p))
+
+    def solve(self, b, x=None, tol=1e-8, maxiter=1000, refactor=False):
+        if x is None:
+            x = torch.zeros_like(b)
+        it, rr = C.c_int(), C.c_double()
+        L.call("baas_pcg_solve", self._p, _ptr(b), _ptr(x), float(tol), int(maxiter), int(bool(refactor)),
+               C.byref(it), C.byref(rr))
+        return x, it.value, rr.value
+
+    def times(self):
+        a, b = C.c_double(), C.c_double()
+        L.call("baas_pcg_times", self._p, C.byref(a), C.byref(b))
+        return a.value, b.value
+
+    def __del__(self):
+        try:
+            if self._p:
+                L.load().baas_pcg_destroy(self._p)
+        except Exception:
+            pass

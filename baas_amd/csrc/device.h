@@ -1,0 +1,75 @@
+// device.h -- device-side helpers shared by the sweep kernels.
+//
+// Cross-workgroup publication uses agent-scope relaxed atomics, which gfx950
+// compiles to write-through (sc1) stores and L2-bypassing (sc1) loads, and an
+// explicit s_waitcnt between a row's value stores and its ready flag.  Every
+// value another workgroup may read is stored this way, so no L2 writeback
+// (buffer_wbl2) is needed on the release side; the reader issues its value
+// loads only after it has observed the flag.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "internal.h"
+
+namespace baas {
+
+template <class T>
+__device__ __forceinline__ T ld_agent(const T *p) {
+    return __hip_atomic_load(const_cast<T *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+template <class T>
+__device__ __forceinline__ void st_agent(T *p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// wait until every store this thread issued has been acknowledged
+__device__ __forceinline__ void wait_stores() { __builtin_amdgcn_s_waitcnt(0); }
+
+__device__ __forceinline__ void raise_status(DevStatus *st, int code, int detail) {
+    if (atomicCAS(&st->code, 0, code) == 0) st->detail = detail;
+}
+
+// Spin until flags[c] == epoch; a watchdog turns a missing producer into
+// BAAS_TIMEOUT instead of a hang.
+__device__ __forceinline__ void wait_ready(const int *flags, int c, int epoch, long long limit,
+                                           DevStatus *st) {
+    long long spins = 0;
+    while (ld_agent(&flags[c]) != epoch) {
+        if (++spins > limit) {
+            raise_status(st, BAAS_TIMEOUT, c);
+            break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
+
+template <int KEYMODE>
+__device__ __forceinline__ int key_of(int r, int n, const int *key) {
+    if (KEYMODE == 0) return r;
+    if (KEYMODE == 1) return n - 1 - r;
+    return key[r];
+}
+
+// Claim the next work unit (monotone ticket, so every unit's producers were
+// claimed earlier by running workgroups).
+__device__ __forceinline__ int claim_unit(int *ticket, int *slot) {
+    if (threadIdx.x == 0) *slot = atomicAdd(ticket, 1);
+    __syncthreads();
+    int u = *slot;
+    __syncthreads();
+    return u;
+}
+
+// The last workgroup out resets the ticket and the exit counter so the next
+// launch (or graph replay) starts from zero without a memset.
+__device__ __forceinline__ void retire_block(int *ticket, int *done) {
+    if (threadIdx.x == 0) {
+        __threadfence();
+        int d = atomicAdd(done, 1);
+        if (d == (int)gridDim.x - 1) {
+            atomicExch(ticket, 0);
+            atomicExch(done, 0);
+        }
+    }
+}
+
+}  // namespace baas

@@ -1,0 +1,124 @@
+// internal.h -- shared structures and device helpers of libbaas_hip.so.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <vector>
+
+#include "../../include/baas_hip.h"
+
+#define BAAS_HIP_TRY(expr)                                                        \
+    do {                                                                          \
+        hipError_t _e = (expr);                                                   \
+        if (_e != hipSuccess) {                                                   \
+            fprintf(stderr, "baas: %s failed: %s (%s:%d)\n", #expr,               \
+                    hipGetErrorString(_e), __FILE__, __LINE__);                   \
+            return BAAS_HIP_ERROR;                                                \
+        }                                                                         \
+    } while (0)
+
+#define BAAS_TRY(expr)                    \
+    do {                                  \
+        int _s = (expr);                  \
+        if (_s != BAAS_OK) return _s;     \
+    } while (0)
+
+namespace baas {
+
+// Device-side error word (first error wins) and watchdog accounting.
+struct DevStatus {
+    int code;
+    int detail;
+};
+
+// A device buffer owned by the library.
+template <class T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    int alloc(size_t count) {
+        free();
+        n = count;
+        if (count == 0) return BAAS_OK;
+        if (hipMalloc(&p, count * sizeof(T)) != hipSuccess) {
+            p = nullptr;
+            n = 0;
+            return BAAS_ALLOC;
+        }
+        return BAAS_OK;
+    }
+    void free() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        n = 0;
+    }
+    ~DBuf() { free(); }
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+};
+
+// Execution plan: how a triangular sweep (solve or factorisation) is cut
+// into work units.  A unit is a contiguous run of the execution order; its
+// rows are stored sorted by their sub-level inside the unit (longest path
+// restricted to the unit), so a workgroup runs a unit sub-level by sub-level
+// with a barrier in between and keeps the unit's solved x in LDS.
+// keymode: 0 key(row) = row, 1 key(row) = n-1-row, 2 key(row) = key[row].
+// Parents of a row in unit u have key < key_begin[u] (another unit, wait on
+// the ready flag) or key in [key_begin[u], key_begin[u+1]) (same unit, LDS).
+struct Plan {
+    int n = 0;
+    int nunits = 0;
+    int nsub = 0;
+    int keymode = 0;
+    int max_unit_keys = 0;
+    int block = 256;
+    int mode = 0;
+    DBuf<int> key_begin;  // [nunits+1]
+    DBuf<int> sub_begin;  // [nunits+1] index into sub_ptr
+    DBuf<int> sub_ptr;    // [nsub+1]   offsets into rows
+    DBuf<int> rows;       // [n]        row ids; bit 31 set = row has a child in a later unit
+    DBuf<int> key;        // [n]        keymode 2 only
+    std::vector<int> h_unit_ptr;  // host copy: unit boundaries in rows
+    std::vector<int> h_rows;      // host copy of rows (without flag bit)
+};
+
+}  // namespace baas
+
+struct baas_handle {
+    hipStream_t stream = nullptr;
+    int device = 0;
+    int num_cus = 0;
+    baas::DevStatus *status = nullptr;  // device
+    int *counters = nullptr;            // device scratch counters [64]
+    double *dscratch = nullptr;         // device reduction scratch
+    size_t dscratch_n = 0;
+    int epoch = 0;                      // ready-flag epoch
+    long long spin_limit = 1LL << 22;
+};
+
+struct baas_schedule {
+    baas_handle *h = nullptr;
+    int kind = 0;
+    int n = 0;
+    int nnz_full = 0;   // A (IC0/ILU0) or L (TRSV)
+    int nnz_lower = 0;
+    // patterns (device)
+    baas::DBuf<int> a_rowptr, a_colidx, a_diagptr;  // full A (IC0/ILU0)
+    baas::DBuf<int> l_rowptr, l_colidx;              // lower part incl. diagonal
+    baas::DBuf<int> l_src;                           // position in A of each L entry
+    baas::DBuf<int> u_rowptr, u_colidx;              // U = L^T (diag first)
+    baas::DBuf<int> u_src;                           // position in L of each U entry
+    baas::DBuf<int> l_row_of;                        // row of each L nonzero
+    // schedule exports (host copies)
+    std::vector<int> group_ptr, group_set, level_ptr, perm_to_orig, node_to_level, nthreads,
+        merged_ptr, row_level_ptr, row_perm, row_level, reduced_ptr, reduced_set;
+    int ngroups = 0, nlevels = 0, nmerged = 0, reduced_edges = 0, nrowlevels = 0;
+    // plans (forward over L, backward over U) and ready flags
+    baas::Plan fwd, bwd;
+    baas::DBuf<int> flags;  // [n] epoch of the last sweep that finished the row
+    baas::DBuf<int> row_level_ptr_d, row_perm_d;  // for the level-synchronous solve
+    double t_gpu = 0, t_host = 0;
+    baas_opts opts{};
+};

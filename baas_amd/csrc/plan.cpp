@@ -1,0 +1,218 @@
+// plan.cpp -- execution plans: packing a schedule into GPU work units.
+//
+// The reference maps groups of each (merged) level to OpenMP threads with its
+// affinity rule and builds a task DAG of point-to-point waits
+// (merge/LevelTask.cpp:2295-2684, :305-485, :3096-3184).  On the GPU a work
+// unit is a contiguous run of an execution order processed by one workgroup
+// sub-level by sub-level (barrier in between, solved values staged in LDS);
+// rows that another unit needs publish a per-row ready flag.  Two packings:
+//
+//   RANGE     execution order = row order (reversed for the backward sweep),
+//             units = consecutive row ranges.  Dependencies of banded and
+//             naturally ordered matrices stay inside a unit, so most of the
+//             critical path runs on LDS barriers instead of HBM flags.
+//   SCHEDULE  execution order = the BAAS schedule: merged levels, groups in
+//             level order, nodes ascending inside a group; units = chunks of
+//             at most rows_per_unit rows of one merged level.
+//
+// Built on the host once per analysis (like the reference's task build).
+#include <algorithm>
+#include <cstring>
+#include <vector>
+
+#include "internal.h"
+
+namespace baas {
+
+namespace {
+
+struct HostCsr {
+    std::vector<int> ptr, idx;
+};
+
+int fetch(HostCsr &c, const int *ptr, const int *idx, int n, hipStream_t st) {
+    c.ptr.resize(n + 1);
+    if (hipMemcpyAsync(c.ptr.data(), ptr, sizeof(int) * (n + 1), hipMemcpyDeviceToHost, st) != hipSuccess)
+        return BAAS_HIP_ERROR;
+    if (hipStreamSynchronize(st) != hipSuccess) return BAAS_HIP_ERROR;
+    c.idx.resize(c.ptr[n]);
+    if (c.ptr[n] &&
+        hipMemcpyAsync(c.idx.data(), idx, sizeof(int) * c.ptr[n], hipMemcpyDeviceToHost, st) != hipSuccess)
+        return BAAS_HIP_ERROR;
+    if (hipStreamSynchronize(st) != hipSuccess) return BAAS_HIP_ERROR;
+    return BAAS_OK;
+}
+
+template <class T>
+int upload(DBuf<T> &d, const std::vector<T> &h, hipStream_t st) {
+    BAAS_TRY(d.alloc(h.empty() ? 1 : h.size()));
+    if (!h.empty() && hipMemcpyAsync(d.p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, st) != hipSuccess)
+        return BAAS_HIP_ERROR;
+    return BAAS_OK;
+}
+
+// Build one plan.
+//   order[n]       execution order (topological for the sweep)
+//   unit_ptr[]     unit boundaries in `order`
+//   deps           parents of each row for this sweep (excluding the diagonal)
+//   kids           children of each row for this sweep
+//   keymode        0 key = row, 1 key = n-1-row, 2 key = position in order
+int make_plan(Plan &p, int n, const std::vector<int> &order, const std::vector<int> &unit_ptr,
+              const std::vector<int> &dptr, const std::vector<int> &didx, const std::vector<int> &kptr,
+              const std::vector<int> &kidx, int keymode, int mode, int block, hipStream_t st) {
+    int nunits = (int)unit_ptr.size() - 1;
+    std::vector<int> pos(n), unit_of(n), key(n);
+    for (int q = 0; q < n; q++) pos[order[q]] = q;
+    for (int u = 0; u < nunits; u++)
+        for (int q = unit_ptr[u]; q < unit_ptr[u + 1]; q++) unit_of[order[q]] = u;
+    for (int r = 0; r < n; r++) key[r] = keymode == 0 ? r : keymode == 1 ? n - 1 - r : pos[r];
+    std::vector<int> key_begin(nunits + 1), sub_begin(nunits + 1), sub_ptr, rows(n);
+    std::vector<int> sub(n, 0);
+    int max_keys = 0;
+    sub_ptr.push_back(0);
+    for (int u = 0; u < nunits; u++) {
+        int b = unit_ptr[u], e = unit_ptr[u + 1];
+        // key range of the unit (contiguous by construction)
+        int kmin = 1 << 30, kmax = -1;
+        for (int q = b; q < e; q++) {
+            kmin = std::min(kmin, key[order[q]]);
+            kmax = std::max(kmax, key[order[q]]);
+        }
+        if (kmax - kmin + 1 != e - b) return BAAS_INVALID_ARG;
+        key_begin[u] = kmin;
+        max_keys = std::max(max_keys, e - b);
+        int maxsub = 0;
+        for (int q = b; q < e; q++) {
+            int r = order[q], s = 0;
+            for (int j = dptr[r]; j < dptr[r + 1]; j++) {
+                int c = didx[j];
+                if (unit_of[c] == u) s = std::max(s, sub[c] + 1);
+                else if (unit_of[c] > u) return BAAS_INVALID_ARG;  // not topological
+            }
+            sub[r] = s;
+            maxsub = std::max(maxsub, s);
+        }
+        // counting sort of the unit's rows by sub-level (stable)
+        std::vector<int> cnt(maxsub + 2, 0);
+        for (int q = b; q < e; q++) cnt[sub[order[q]] + 1]++;
+        for (int s = 0; s <= maxsub; s++) cnt[s + 1] += cnt[s];
+        sub_begin[u] = (int)sub_ptr.size() - 1;
+        for (int s = 0; s <= maxsub; s++) sub_ptr.push_back(b + cnt[s + 1]);
+        for (int q = b; q < e; q++) {
+            int r = order[q];
+            int ext = 0;
+            for (int j = kptr[r]; j < kptr[r + 1]; j++)
+                if (unit_of[kidx[j]] != u) { ext = 1; break; }
+            rows[b + cnt[sub[r]]++] = r | (ext ? (int)0x80000000u : 0);
+        }
+    }
+    key_begin[nunits] = n;
+    sub_begin[nunits] = (int)sub_ptr.size() - 1;
+    if (keymode == 0) key_begin[nunits] = n;
+    p.n = n;
+    p.nunits = nunits;
+    p.nsub = (int)sub_ptr.size() - 1;
+    p.keymode = keymode;
+    p.max_unit_keys = max_keys;
+    p.block = block;
+    p.mode = mode;
+    BAAS_TRY(upload(p.key_begin, key_begin, st));
+    BAAS_TRY(upload(p.sub_begin, sub_begin, st));
+    BAAS_TRY(upload(p.sub_ptr, sub_ptr, st));
+    BAAS_TRY(upload(p.rows, rows, st));
+    if (keymode == 2) BAAS_TRY(upload(p.key, key, st));
+    p.h_unit_ptr = unit_ptr;
+    p.h_rows.resize(n);
+    for (int q = 0; q < n; q++) p.h_rows[q] = rows[q] & 0x7fffffff;
+    if (hipStreamSynchronize(st) != hipSuccess) return BAAS_HIP_ERROR;
+    return BAAS_OK;
+}
+
+}  // namespace
+
+// Strip the diagonal: parents of row r in L (lower CSR, diag last) and in
+// U = L^T (upper CSR, diag first).
+int build_plans_mode(baas_schedule *s, int mode, int rows_per_unit) {
+    hipStream_t st = s->h->stream;
+    const int n = s->n;
+    HostCsr L, U;
+    BAAS_TRY(fetch(L, s->l_rowptr.p, s->l_colidx.p, n, st));
+    BAAS_TRY(fetch(U, s->u_rowptr.p, s->u_colidx.p, n, st));
+    std::vector<int> lp(n + 1), li, up(n + 1), ui;
+    li.reserve(L.idx.size());
+    ui.reserve(U.idx.size());
+    lp[0] = up[0] = 0;
+    for (int r = 0; r < n; r++) {
+        for (int j = L.ptr[r]; j < L.ptr[r + 1] - 1; j++) li.push_back(L.idx[j]);
+        lp[r + 1] = (int)li.size();
+        for (int j = U.ptr[r] + 1; j < U.ptr[r + 1]; j++) ui.push_back(U.idx[j]);
+        up[r + 1] = (int)ui.size();
+    }
+    int block = s->opts.block_threads > 0 ? s->opts.block_threads : 256;
+    std::vector<int> fwd_order(n), bwd_order(n), fu, bu;
+    if (mode == BAAS_PLAN_RANGE) {
+        int R = rows_per_unit > 0 ? rows_per_unit : 4096;
+        for (int r = 0; r < n; r++) fwd_order[r] = r, bwd_order[r] = n - 1 - r;
+        for (int b = 0; b < n; b += R) fu.push_back(b);
+        fu.push_back(n);
+        bu = fu;
+        // forward: parents = L off-diagonals, children = U off-diagonals
+        BAAS_TRY(make_plan(s->fwd, n, fwd_order, fu, lp, li, up, ui, 0, mode, block, st));
+        BAAS_TRY(make_plan(s->bwd, n, bwd_order, bu, up, ui, lp, li, 1, mode, block, st));
+        return BAAS_OK;
+    }
+    // SCHEDULE: merged levels -> groups in level order -> nodes ascending
+    int R = rows_per_unit > 0 ? rows_per_unit : block;
+    const auto &gp = s->group_ptr, &gs = s->group_set, &lvp = s->level_ptr, &pto = s->perm_to_orig,
+               &mp = s->merged_ptr;
+    int q = 0;
+    for (int m = 0; m + 1 < (int)mp.size(); m++) {
+        int start = q;
+        for (int t = mp[m]; t < mp[m + 1]; t++) {
+            int g = pto[t];
+            for (int k = gp[g]; k < gp[g + 1]; k++) fwd_order[q++] = gs[k];
+        }
+        for (int b = start; b < q; b += R) fu.push_back(b);
+    }
+    (void)lvp;
+    fu.push_back(n);
+    if (q != n) return BAAS_INVALID_ARG;
+    for (int i = 0; i < n; i++) bwd_order[i] = fwd_order[n - 1 - i];
+    int nu = (int)fu.size() - 1;
+    bu.resize(nu + 1);
+    for (int u = 0; u <= nu; u++) bu[u] = n - fu[nu - u];
+    BAAS_TRY(make_plan(s->fwd, n, fwd_order, fu, lp, li, up, ui, 2, mode, block, st));
+    BAAS_TRY(make_plan(s->bwd, n, bwd_order, bu, up, ui, lp, li, 2, mode, block, st));
+    return BAAS_OK;
+}
+
+// Default packing before any autotuning: RANGE when a row-range unit keeps
+// most dependencies inside the unit (the matrix is banded), SCHEDULE else.
+int build_plans(baas_schedule *s) {
+    int mode = s->opts.plan_mode;
+    if (mode == BAAS_PLAN_AUTO) {
+        // fraction of off-diagonal entries whose column lies within 4096 rows
+        HostCsr L;
+        BAAS_TRY(fetch(L, s->l_rowptr.p, s->l_colidx.p, s->n, s->h->stream));
+        long long near = 0, tot = 0;
+        for (int r = 0; r < s->n; r++)
+            for (int j = L.ptr[r]; j < L.ptr[r + 1] - 1; j++) {
+                tot++;
+                if (r - L.idx[j] <= 4096) near++;
+            }
+        mode = (tot == 0 || near * 10 >= tot * 9) ? BAAS_PLAN_RANGE : BAAS_PLAN_SCHEDULE;
+    }
+    return build_plans_mode(s, mode, s->opts.rows_per_unit);
+}
+
+}  // namespace baas
+
+extern "C" int baas_replan(baas_schedule *s, int mode, int rows_per_unit, int block_threads) {
+    if (!s || mode < 1 || mode > 2 || rows_per_unit < 0 || block_threads < 0 || block_threads > 1024 ||
+        (block_threads % 64) != 0)
+        return BAAS_INVALID_ARG;
+    if (block_threads) s->opts.block_threads = block_threads;
+    s->opts.plan_mode = mode;
+    s->opts.rows_per_unit = rows_per_unit;
+    return baas::build_plans_mode(s, mode, rows_per_unit);
+}

@@ -1,0 +1,264 @@
+// sptrsv.hip -- triangular solves on gfx950.
+//
+// Row arithmetic follows the reference exactly: the lower solve sums
+// L_ij * x_j in ascending column order with fused multiply-adds and divides
+// by the diagonal (merge/SpTRSV.cpp:24-39, the reference build contracts the
+// sum to FMA); the upper solve on U = L^T sums in descending column order as
+// the backward P2P kernel does (merge/SpTRSV.cpp:643-652).  Every schedule
+// therefore produces the serial solve's bits.
+//
+// k_sweep is the persistent, sync-free replacement of the P2P group-merged
+// kernel (merge/SpTRSV.cpp:421-478): workgroups claim work units in order;
+// inside a unit, sub-levels are separated by barriers and solved values are
+// staged in LDS; parents in earlier units are waited for on per-row ready
+// flags (epoch-stamped, so no reset between calls).
+#include <algorithm>
+
+#include "device.h"
+#include "internal.h"
+
+namespace baas {
+
+struct SweepArgs {
+    int n, nunits, nrhs, epoch;
+    const int *rowptr, *colidx;
+    const int *key_begin, *sub_begin, *sub_ptr, *rows, *key;
+    int *flags;
+    int *ticket, *done;
+    DevStatus *status;
+    long long spin_limit;
+};
+
+template <class T, int KEYMODE, bool UPPER, bool LDS>
+__global__ __launch_bounds__(1024) void k_sweep(SweepArgs a, const T *__restrict__ vals,
+                                               const T *__restrict__ b, T *x) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T *xs = reinterpret_cast<T *>(smem_raw);
+    __shared__ int s_unit;
+    const int nrhs = a.nrhs;
+    for (;;) {
+        const int u = claim_unit(a.ticket, &s_unit);
+        if (u >= a.nunits) break;
+        const int kb = a.key_begin[u];
+        const int s0 = a.sub_begin[u], s1 = a.sub_begin[u + 1];
+        for (int si = s0; si < s1; si++) {
+            const int qb = a.sub_ptr[si], qe = a.sub_ptr[si + 1];
+            const int items = (qe - qb) * nrhs;
+            for (int t = threadIdx.x; t < items; t += blockDim.x) {
+                const int q = qb + (nrhs == 1 ? t : t / nrhs);
+                const int r = nrhs == 1 ? 0 : t - (q - qb) * nrhs;
+                const int rw = a.rows[q];
+                const int row = rw & 0x7fffffff;
+                const int rb = a.rowptr[row], re = a.rowptr[row + 1];
+                const int ob = UPPER ? rb + 1 : rb, oe = UPPER ? re : re - 1;  // off-diagonals
+                const int dg = UPPER ? rb : re - 1;
+                // wait for parents produced by other units
+                for (int j = ob; j < oe; j++) {
+                    const int c = a.colidx[j];
+                    if (key_of<KEYMODE>(c, a.n, a.key) < kb) wait_ready(a.flags, c, a.epoch, a.spin_limit, a.status);
+                }
+                T s = T(0);
+                if (UPPER) {
+                    for (int j = oe - 1; j >= ob; j--) {
+                        const int c = a.colidx[j];
+                        const int kc = key_of<KEYMODE>(c, a.n, a.key);
+                        T xv;
+                        if (LDS && kc >= kb) xv = xs[(size_t)(kc - kb) * nrhs + r];
+                        else xv = ld_agent(&x[(size_t)c * nrhs + r]);
+                        s = fma(xv, vals[j], s);
+                    }
+                } else {
+                    for (int j = ob; j < oe; j++) {
+                        const int c = a.colidx[j];
+                        const int kc = key_of<KEYMODE>(c, a.n, a.key);
+                        T xv;
+                        if (LDS && kc >= kb) xv = xs[(size_t)(kc - kb) * nrhs + r];
+                        else xv = ld_agent(&x[(size_t)c * nrhs + r]);
+                        s = fma(vals[j], xv, s);
+                    }
+                }
+                const T xr = (b[(size_t)row * nrhs + r] - s) / vals[dg];
+                st_agent(&x[(size_t)row * nrhs + r], xr);
+                if (LDS) xs[(size_t)(key_of<KEYMODE>(row, a.n, a.key) - kb) * nrhs + r] = xr;
+                if (nrhs == 1 && rw < 0) {
+                    wait_stores();
+                    st_agent(&a.flags[row], a.epoch);
+                }
+            }
+            if (nrhs > 1) {
+                wait_stores();
+                __syncthreads();
+                for (int t = threadIdx.x; t < qe - qb; t += blockDim.x) {
+                    const int rw = a.rows[qb + t];
+                    if (rw < 0) st_agent(&a.flags[rw & 0x7fffffff], a.epoch);
+                }
+            } else if (!LDS) {
+                wait_stores();
+                __syncthreads();
+            } else {
+                __syncthreads();
+            }
+        }
+    }
+    retire_block(a.ticket, a.done);
+}
+
+// Level-synchronous solve: one launch per row level (merge/SpTRSV.cpp:41-80).
+template <class T>
+__global__ void k_level_rows(const int *__restrict__ perm, int qb, int qe, const int *__restrict__ rowptr,
+                             const int *__restrict__ colidx, const T *__restrict__ vals, const T *__restrict__ b,
+                             T *x) {
+    int q = qb + blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= qe) return;
+    int row = perm[q];
+    int rb = rowptr[row], re = rowptr[row + 1] - 1;
+    T s = T(0);
+    for (int j = rb; j < re; j++) s = fma(vals[j], x[colidx[j]], s);
+    x[row] = (b[row] - s) / vals[re];
+}
+
+template <class T, class S>
+__global__ void k_gather(int n, const int *__restrict__ src, const S *__restrict__ in, T *__restrict__ out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (T)in[src[i]];
+}
+
+template <class T, class S>
+__global__ void k_cast(int n, const S *__restrict__ in, T *__restrict__ out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = (T)in[i];
+}
+
+// dL_ij = -sum_r lambda_{i,r} x_{j,r}
+__global__ void k_grad_vals(int nnz, int nrhs, const int *__restrict__ row_of, const int *__restrict__ colidx,
+                            const double *__restrict__ lam, const double *__restrict__ x, double *__restrict__ dv) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nnz) return;
+    int i = row_of[e], j = colidx[e];
+    double s = 0.0;
+    for (int r = 0; r < nrhs; r++) s = fma(lam[(size_t)i * nrhs + r], x[(size_t)j * nrhs + r], s);
+    dv[e] = -s;
+}
+
+static inline int nblk(long long n, int b = 256) { return (int)((n + b - 1) / b); }
+
+template <class T, int KM, bool UP, bool LDS>
+static int launch_sweep_t(baas_handle *h, const Plan &p, const SweepArgs &a, const T *vals, const T *b, T *x,
+                          size_t lds) {
+    auto kern = k_sweep<T, KM, UP, LDS>;
+    int per_cu = 0;
+    BAAS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, p.block, lds));
+    if (per_cu <= 0) return BAAS_UNSUPPORTED;
+    long long grid = (long long)per_cu * h->num_cus;
+    if (grid > p.nunits) grid = p.nunits;
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(p.block), lds, h->stream, a, vals, b, x);
+    BAAS_HIP_TRY(hipGetLastError());
+    return BAAS_OK;
+}
+
+template <class T, bool UP, bool LDS>
+static int launch_sweep_k(baas_handle *h, const Plan &p, const SweepArgs &a, const T *vals, const T *b, T *x,
+                          size_t lds) {
+    switch (p.keymode) {
+        case 0: return launch_sweep_t<T, 0, UP, LDS>(h, p, a, vals, b, x, lds);
+        case 1: return launch_sweep_t<T, 1, UP, LDS>(h, p, a, vals, b, x, lds);
+        default: return launch_sweep_t<T, 2, UP, LDS>(h, p, a, vals, b, x, lds);
+    }
+}
+
+size_t lds_limit(baas_handle *h);
+
+template <class T>
+int sweep(baas_handle *h, baas_schedule *s, int uplo, const T *vals, const T *b, T *x, int nrhs) {
+    if (!h || !s || !vals || !b || !x || nrhs < 1 || (uplo != 0 && uplo != 1)) return BAAS_INVALID_ARG;
+    if ((const void *)b == (const void *)x) return BAAS_INVALID_ARG;
+    const Plan &p = uplo ? s->bwd : s->fwd;
+    SweepArgs a;
+    a.n = s->n;
+    a.nunits = p.nunits;
+    a.nrhs = nrhs;
+    if (++h->epoch <= 0) h->epoch = 1;  // flags start at 0
+    a.epoch = h->epoch;
+    a.rowptr = uplo ? s->u_rowptr.p : s->l_rowptr.p;
+    a.colidx = uplo ? s->u_colidx.p : s->l_colidx.p;
+    a.key_begin = p.key_begin.p;
+    a.sub_begin = p.sub_begin.p;
+    a.sub_ptr = p.sub_ptr.p;
+    a.rows = p.rows.p;
+    a.key = p.key.p;
+    a.flags = s->flags.p;
+    a.ticket = h->counters;
+    a.done = h->counters + 1;
+    a.status = h->status;
+    a.spin_limit = h->spin_limit;
+    size_t lds = (size_t)p.max_unit_keys * nrhs * sizeof(T);
+    bool use_lds = lds <= lds_limit(h);
+    if (uplo)
+        return use_lds ? launch_sweep_k<T, true, true>(h, p, a, vals, b, x, lds)
+                       : launch_sweep_k<T, true, false>(h, p, a, vals, b, x, 0);
+    return use_lds ? launch_sweep_k<T, false, true>(h, p, a, vals, b, x, lds)
+                   : launch_sweep_k<T, false, false>(h, p, a, vals, b, x, 0);
+}
+
+template int sweep<double>(baas_handle *, baas_schedule *, int, const double *, const double *, double *, int);
+template int sweep<float>(baas_handle *, baas_schedule *, int, const float *, const float *, float *, int);
+
+}  // namespace baas
+
+using namespace baas;
+
+extern "C" int baas_sptrsv_f64(baas_handle *h, baas_schedule *s, int uplo, const double *vals, const double *b,
+                               double *x, int nrhs) {
+    return sweep<double>(h, s, uplo, vals, b, x, nrhs);
+}
+
+extern "C" int baas_sptrsv_f32(baas_handle *h, baas_schedule *s, int uplo, const float *vals, const float *b,
+                               float *x, int nrhs) {
+    return sweep<float>(h, s, uplo, vals, b, x, nrhs);
+}
+
+extern "C" int baas_sptrsv_levelsync_f64(baas_handle *h, baas_schedule *s, const double *vals, const double *b,
+                                         double *x) {
+    if (!h || !s || !vals || !b || !x) return BAAS_INVALID_ARG;
+    const auto &lp = s->row_level_ptr;
+    for (int l = 0; l + 1 < (int)lp.size(); l++) {
+        int cnt = lp[l + 1] - lp[l];
+        hipLaunchKernelGGL(k_level_rows<double>, dim3(nblk(cnt)), dim3(256), 0, h->stream, s->row_perm_d.p, lp[l],
+                           lp[l + 1], s->l_rowptr.p, s->l_colidx.p, vals, b, x);
+    }
+    BAAS_HIP_TRY(hipGetLastError());
+    return BAAS_OK;
+}
+
+extern "C" int baas_transpose_values_f64(baas_handle *h, baas_schedule *s, const double *lv, double *uv) {
+    if (!h || !s || !lv || !uv) return BAAS_INVALID_ARG;
+    int nnz = s->nnz_lower;
+    hipLaunchKernelGGL((k_gather<double, double>), dim3(nblk(nnz)), dim3(256), 0, h->stream, nnz, s->u_src.p, lv, uv);
+    BAAS_HIP_TRY(hipGetLastError());
+    return BAAS_OK;
+}
+
+extern "C" int baas_cast_f64_f32(baas_handle *h, int n, const double *in, float *out) {
+    if (!h || n < 0 || (n && (!in || !out))) return BAAS_INVALID_ARG;
+    if (n) hipLaunchKernelGGL((k_cast<float, double>), dim3(nblk(n)), dim3(256), 0, h->stream, n, in, out);
+    BAAS_HIP_TRY(hipGetLastError());
+    return BAAS_OK;
+}
+
+extern "C" int baas_cast_f32_f64(baas_handle *h, int n, const float *in, double *out) {
+    if (!h || n < 0 || (n && (!in || !out))) return BAAS_INVALID_ARG;
+    if (n) hipLaunchKernelGGL((k_cast<double, float>), dim3(nblk(n)), dim3(256), 0, h->stream, n, in, out);
+    BAAS_HIP_TRY(hipGetLastError());
+    return BAAS_OK;
+}
+
+extern "C" int baas_sptrsv_grad_vals_f64(baas_handle *h, baas_schedule *s, const double *lam, const double *x,
+                                         double *dv, int nrhs) {
+    if (!h || !s || !lam || !x || !dv || nrhs < 1) return BAAS_INVALID_ARG;
+    int nnz = s->nnz_lower;
+    hipLaunchKernelGGL(k_grad_vals, dim3(nblk(nnz)), dim3(256), 0, h->stream, nnz, nrhs, s->l_row_of.p,
+                       s->l_colidx.p, lam, x, dv);
+    BAAS_HIP_TRY(hipGetLastError());
+    return BAAS_OK;
+}

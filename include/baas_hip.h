@@ -1,0 +1,191 @@
+/*
+ * baas_hip.h -- C-ABI of the MI355X (gfx950) sparse triangular pipeline.
+ *
+ * Plain pointers and sizes only.  Every numeric pointer is a DEVICE pointer
+ * unless the comment says host.  Indices are 0-based int32; rows are sorted
+ * ascending by column; a lower-triangular CSR keeps its diagonal last in each
+ * row, an upper-triangular CSR keeps it first (the conventions of the
+ * reference's SPM::CSR, spm/include/CSR.hpp:16-27).
+ *
+ * Ownership: the caller owns every input and output buffer; the library owns
+ * handles, schedules, plans, ready flags and scratch inside the opaque
+ * objects and frees them in the matching destroy call.
+ * Threading: one handle = one HIP stream; a handle is not thread-safe, many
+ * handles may run concurrently.  Numeric calls are asynchronous on the
+ * handle's stream; baas_check() synchronises and returns the first error a
+ * kernel raised (zero pivot, watchdog timeout) since the previous check.
+ * Ready flags are reset by epoch per call, so back-to-back calls and graph
+ * replays need no memset.
+ */
+#ifndef BAAS_HIP_H
+#define BAAS_HIP_H
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct baas_handle baas_handle;
+typedef struct baas_schedule baas_schedule;
+typedef struct baas_pcg baas_pcg;
+typedef struct baas_batch baas_batch;
+
+enum baas_status {
+    BAAS_OK = 0,
+    BAAS_INVALID_ARG = 1,      /* bad size, unsorted row, missing diagonal, wrong kind */
+    BAAS_ZERO_PIVOT = 2,       /* ILU0 pivot |u_jj| <= 1e-16 (the reference asserts) */
+    BAAS_NONPOSITIVE_PIVOT = 3,/* IC0 A_ii - sum L_ik^2 <= 0; diagonal kept as the reference does */
+    BAAS_HIP_ERROR = 4,
+    BAAS_TIMEOUT = 5,          /* a ready-flag wait exceeded the watchdog spin limit */
+    BAAS_NOT_CONVERGED = 6,
+    BAAS_ALLOC = 7,
+    BAAS_UNSUPPORTED = 8
+};
+
+/* Which numeric kernel a schedule is analysed for; selects the threads-per-
+ * level rule (merge/LevelMerge.cpp:118-161) and, for IC0/ILU0, extraction of
+ * the lower pattern from a full matrix (test/ICRuntime.hpp:416-435). */
+enum baas_kind { BAAS_KIND_TRSV = 0, BAAS_KIND_IC0 = 1, BAAS_KIND_ILU0 = 2 };
+
+/* Arrays a schedule exports for bit-exact comparison with the reference. */
+enum baas_export {
+    BAAS_EXPORT_GROUP_PTR = 0,     /* int[ngroups+1]  Coarsen groups (merge/Coarsen.cpp) */
+    BAAS_EXPORT_GROUP_SET = 1,     /* int[n]          nodes, ascending inside a group */
+    BAAS_EXPORT_LEVEL_PTR = 2,     /* int[nlevels+1]  group level set (spm/DAG.cpp:572-629) */
+    BAAS_EXPORT_PERM_TO_ORIG = 3,  /* int[ngroups]    groups ordered by (level, id) */
+    BAAS_EXPORT_NODE_TO_LEVEL = 4, /* int[ngroups]    1-based level of each group */
+    BAAS_EXPORT_NTHREADS = 5,      /* int[nlevels]    threads per level (LevelMerge.cpp) */
+    BAAS_EXPORT_MERGED_PTR = 6,    /* int[nmerged+1]  1-thread-run merge of level_ptr */
+    BAAS_EXPORT_ROW_LEVEL_PTR = 7, /* int[nrowlevels+1] row-level level set of L */
+    BAAS_EXPORT_ROW_PERM = 8,      /* int[n]          rows ordered by (row level, id) */
+    BAAS_EXPORT_ROW_LEVEL = 9,     /* int[n]          1-based row level */
+    BAAS_EXPORT_REDUCED_PTR = 10,  /* int[n+1]        two-hop reduced CSC DAG (spm/DAG.cpp:295-344) */
+    BAAS_EXPORT_REDUCED_SET = 11,  /* int[edges] */
+    BAAS_EXPORT_UNIT_PTR = 12,     /* int[nunits+1]   execution plan: rows per work unit */
+    BAAS_EXPORT_UNIT_ROWS = 13     /* int[n]          execution plan: row order */
+};
+
+/* Execution-plan choice (the autotuner fills these when mode = AUTO). */
+enum baas_plan_mode { BAAS_PLAN_AUTO = 0, BAAS_PLAN_RANGE = 1, BAAS_PLAN_SCHEDULE = 2 };
+
+typedef struct baas_opts {
+    int ncore;            /* thread count the reference rule is evaluated at (default 8) */
+    int plan_mode;        /* baas_plan_mode */
+    int rows_per_unit;    /* rows per work unit (0 = default for the mode) */
+    int block_threads;    /* threads per workgroup (0 = default 256) */
+    int merge_threshold;  /* schedule mode: merged 1-thread levels up to this many rows
+                             stay in one unit (0 = default) */
+} baas_opts;
+
+/* ---- handles -------------------------------------------------------------- */
+/* stream: a hipStream_t (NULL = the default stream). */
+int baas_create(baas_handle **h, void *stream);
+int baas_destroy(baas_handle *h);
+int baas_set_stream(baas_handle *h, void *stream);
+/* Synchronise the handle's stream, return and clear the first kernel error. */
+int baas_check(baas_handle *h);
+const char *baas_status_string(int status);
+/* Watchdog spin limit for ready-flag waits (default 1<<22 polls). */
+int baas_set_spin_limit(baas_handle *h, long long polls);
+
+/* ---- analysis (replaces the runtimes' buildset(), e.g.
+ * test/SpTRSVRuntime.hpp:675-775, test/ICRuntime.hpp:411-531,
+ * test/ILURuntime.hpp:265-385, test/Schedule.hpp:181-296) ------------------- */
+/* rowptr/colidx: device arrays of an n x n CSR.  kind TRSV: the lower-
+ * triangular L itself.  kind IC0/ILU0: the full A (its lower part is
+ * extracted; the threads-per-level rule counts rows of A as the reference
+ * runtimes do).  Synchronous.  The schedule keeps copies of what it needs. */
+int baas_analyze(baas_handle *h, int n, const int *rowptr, const int *colidx, int kind,
+                 const baas_opts *opts, baas_schedule **s);
+int baas_schedule_destroy(baas_schedule *s);
+/* counts[0..7] = ngroups, nlevels, nmerged, reduced_edges, nrowlevels, nunits,
+ * nnz_lower, plan_mode.  Host pointer. */
+int baas_schedule_info(const baas_schedule *s, int *counts);
+/* Copy one exported array to host memory; out == NULL only queries *len. */
+int baas_schedule_export(const baas_schedule *s, int which, int *out, long long *len);
+/* Seconds the analysis took, split into GPU phase and host phase (host). */
+int baas_schedule_times(const baas_schedule *s, double *gpu_s, double *host_s);
+/* Device pointers of U = L^T's CSR pattern (rows with the diagonal first). */
+int baas_schedule_upper_pattern(const baas_schedule *s, const int **rowptr, const int **colidx);
+
+/* ---- triangular solves (sptrsv_p2p_csr_group_merge_no_perm_alloc,
+ * merge/SpTRSV.cpp:421-478; backward form :601-660) ------------------------ */
+/* uplo 0: L x = b with vals in L's CSR layout (pattern given at analysis).
+ * uplo 1: U x = b with vals in U = L^T's CSR layout; the sum runs in
+ *         descending column order as in the backward P2P kernel.
+ * nrhs >= 1: b and x are row-major n x nrhs.  x must not alias b. */
+int baas_sptrsv_f64(baas_handle *h, baas_schedule *s, int uplo, const double *vals,
+                    const double *b, double *x, int nrhs);
+int baas_sptrsv_f32(baas_handle *h, baas_schedule *s, int uplo, const float *vals,
+                    const float *b, float *x, int nrhs);
+/* Level-synchronous solve, one launch per row level (sptrsv_level_csr_no_perm,
+ * merge/SpTRSV.cpp:41-80).  Same bits as baas_sptrsv_f64 with uplo 0. */
+int baas_sptrsv_levelsync_f64(baas_handle *h, baas_schedule *s, const double *vals,
+                              const double *b, double *x);
+/* Gather L's values into U = L^T's CSR layout (and FP32 casts of either). */
+int baas_transpose_values_f64(baas_handle *h, baas_schedule *s, const double *lvals, double *uvals);
+int baas_cast_f64_f32(baas_handle *h, int n, const double *in, float *out);
+int baas_cast_f32_f64(baas_handle *h, int n, const float *in, double *out);
+
+/* Adjoint of x = L^-1 b: given lambda = L^-T g (solve with uplo 1 on the
+ * transposed values), dL_ij = -lambda_i * x_j on L's pattern (row-major over
+ * L's nonzeros).  nrhs columns are summed. */
+int baas_sptrsv_grad_vals_f64(baas_handle *h, baas_schedule *s, const double *lambda,
+                              const double *x, double *dvals, int nrhs);
+
+/* ---- factorizations --------------------------------------------------------
+ * IC0  (spic0_csr_uL_p2p_group_merge_no_perm, merge/IC.cpp:296-393)
+ * ILU0 (ilu0_p2p_csr_group_merge_no_perm_alloc, merge/ILU.cpp:638-720)
+ * a_vals: values of the full A analysed with kind IC0/ILU0.  lu: nnz(A)
+ * output, starts as a copy of a_vals (may alias a_vals).  IC0 writes the
+ * lower part including the diagonal; ILU0 writes every entry. */
+int baas_ic0_f64(baas_handle *h, baas_schedule *s, const double *a_vals, double *lu);
+int baas_ilu0_f64(baas_handle *h, baas_schedule *s, const double *a_vals, double *lu);
+/* Split an IC0 result into L values (lower CSR) and U = L^T values (upper
+ * CSR of the schedule), test/iccg_level_merge.cpp:809-891; FP64 or FP32. */
+int baas_ic0_split_f64(baas_handle *h, baas_schedule *s, const double *lu, double *lvals, double *uvals);
+int baas_ic0_split_f32(baas_handle *h, baas_schedule *s, const double *lu, float *lvals, float *uvals);
+
+/* ---- SpMV / BLAS-1 (blas/spmv.cpp, blas/blasone.cpp) ---------------------- */
+/* y = alpha * (A x) + beta * y  (beta == 0: y is not read). */
+int baas_spmv_f64(baas_handle *h, int m, const int *rowptr, const int *colidx, const double *vals,
+                  const double *x, double *y, double alpha, double beta);
+/* *result (DEVICE double) = x . y */
+int baas_dot_f64(baas_handle *h, int n, const double *x, const double *y, double *result);
+int baas_nrm2_f64(baas_handle *h, int n, const double *x, double *result);
+/* w = alpha x + beta y (w may alias x or y) */
+int baas_axpby_f64(baas_handle *h, int n, double *w, double alpha, const double *x, double beta,
+                   const double *y);
+
+/* ---- ICCG (pcg_opt, test/iccg_level_merge.cpp:380-533) --------------------
+ * A: full SPD matrix (device CSR) analysed with kind IC0.  precision 0: all
+ * FP64; 1: IC0 factor stored and both triangular solves run in FP32, CG
+ * (residual, dots, updates) in FP64.  The iteration body is captured in a
+ * hipGraph of `chunk` iterations; convergence (||r||/||r0|| < tol) is
+ * checked on device and stops further work inside the chunk. */
+int baas_pcg_create(baas_handle *h, baas_schedule *s, const double *a_vals, int precision,
+                    baas_pcg **p);
+int baas_pcg_destroy(baas_pcg *p);
+/* x: initial guess in, solution out.  iters/relres: host outputs (first k
+ * with relres < tol, or maxiter).  Re-runs IC0 when refactor != 0. */
+int baas_pcg_solve(baas_pcg *p, const double *b, double *x, double tol, int maxiter,
+                   int refactor, int *iters, double *relres);
+/* Seconds of the last solve split into IC0, solve loop (host). */
+int baas_pcg_times(const baas_pcg *p, double *ic_s, double *iter_s);
+
+/* ---- batched solves -------------------------------------------------------
+ * nsys independent SPD systems packed back to back (block-diagonal CSR of
+ * total size N, system i occupying rows sys_ptr[i]..sys_ptr[i+1]-1, host
+ * array), each with nrhs right-hand sides (row-major N x nrhs).  One
+ * schedule over the block-diagonal matrix; one persistent launch per sweep. */
+int baas_batch_create(baas_handle *h, int nsys, const int *sys_ptr_host, int N,
+                      const int *rowptr, const int *colidx, const double *a_vals, int nrhs,
+                      const baas_opts *opts, baas_batch **b);
+int baas_batch_destroy(baas_batch *b);
+/* IC0 (FP64) then PCG with an FP32 factor and FP64 CG on all systems and
+ * right-hand sides; iters (host int[nsys*nrhs]) receives per-column counts. */
+int baas_batch_pcg(baas_batch *b, const double *B, double *X, double tol, int maxiter, int *iters);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,193 @@
+"""GPU parity of the analysis and the triangular solves, through the C-ABI.
+
+Schedules must equal the oracle's (itself pinned to the compiled reference)
+bit for bit; solves must equal the oracle's serial solves bit for bit (same
+FMA order, IEEE division), for every plan and block shape.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from baas_amd import matgen as g
+from gpu_util import announce, down, up
+from workloads import workload
+
+pytestmark = pytest.mark.gpu
+
+SCHED_CASES = [("C1_lap2d_128", 0), ("C2_lap3d_64", 0), ("C2_lap3d_64", 1), ("C2_lap3d_64_perm1", 1),
+               ("C2_convdiff_64", 2), ("C3_knn", 1), ("C5_diff3d_48", 0), ("bcsstk18", 1), ("bcsstk38", 0)]
+
+
+@pytest.fixture(scope="module")
+def handle():
+    from baas_amd.api import Handle
+    return Handle()
+
+
+def _analyze(h, name, kind, ncore=8, **kw):
+    from baas_amd.api import Schedule
+    L, thr, A = workload(name)
+    if kind == 0:
+        return Schedule(h, L.rowptr, L.colidx, kind=0, ncore=ncore, **kw), L
+    return Schedule(h, A.rowptr, A.colidx, kind=kind, ncore=ncore, **kw), L
+
+
+@pytest.mark.parametrize("name,kind", SCHED_CASES)
+def test_schedule_bit_exact(handle, name, kind):
+    announce("schedule", name, kind)
+    L, thr, A = workload(name)
+    for ncore in (1, 8):
+        s, _ = _analyze(handle, name, kind, ncore)
+        got = s.schedule_dict()
+        want = O.baas_schedule(kind, L.n, L.rowptr, L.colidx, L.rowptr if kind == 0 else thr, ncore)
+        for k, v in want.items():
+            if isinstance(v, np.ndarray):
+                np.testing.assert_array_equal(got[k], v, err_msg=f"{name}/{kind}/{ncore}/{k}")
+            else:
+                assert got[k] == v, (name, kind, ncore, k)
+        lp, ntl, pto = O.levelset_rows(L.n, L.rowptr, L.colidx)
+        np.testing.assert_array_equal(s.export("row_level_ptr"), lp)
+        np.testing.assert_array_equal(s.export("row_level"), ntl)
+        np.testing.assert_array_equal(s.export("row_perm"), pto)
+
+
+def test_reduced_dag_matches_oracle(handle):
+    announce("reduced dag C3")
+    L, _, _ = workload("C3_knn")
+    s, _ = _analyze(handle, "C3_knn", 0)
+    cp, ri = O.csc_of(L.n, L.rowptr, L.colidx)
+    rp, rs = O.two_hop(L.n, cp, ri)
+    np.testing.assert_array_equal(s.export("reduced_ptr"), rp)
+    np.testing.assert_array_equal(s.export("reduced_set"), rs)
+
+
+PLANS = [(1, 0, 256), (1, 1024, 256), (1, 16384, 512), (2, 0, 256), (2, 64, 64), (2, 1024, 1024)]
+SOLVE_CASES = ["C1_lap2d_128", "C2_lap3d_64", "C2_lap3d_64_perm1", "C5_diff3d_48", "bcsstk18", "bcsstk38"]
+
+
+@pytest.mark.parametrize("name", SOLVE_CASES)
+def test_forward_backward_bit_exact_all_plans(handle, name):
+    import torch
+    from baas_amd.api import Schedule
+    L, _, _ = workload(name)
+    s = Schedule(handle, L.rowptr, L.colidx, kind=0)
+    b = g.rhs_uniform(L.n, 0)
+    x_ref = O.sptrsv_lower(L, b)
+    U = g.Csr.from_scipy(L.to_scipy().T)
+    y_ref = O.sptrsv_upper(U, b)  # descending sum
+    lv, bd = up(L.values), up(b)
+    uv = s.transpose_values(lv)
+    np.testing.assert_array_equal(down(uv), U.values)
+    for mode, rpu, blk in PLANS:
+        announce("solve", name, mode, rpu, blk)
+        s.replan(mode, rpu, blk)
+        for rep in range(2):
+            x = s.solve(lv, bd)
+            y = s.solve(uv, bd, uplo=1)
+            handle.check()
+            np.testing.assert_array_equal(down(x), x_ref, err_msg=f"fwd {name} {mode} {rpu} {blk} {rep}")
+            np.testing.assert_array_equal(down(y), y_ref, err_msg=f"bwd {name} {mode} {rpu} {blk} {rep}")
+    torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("name", ["C1_lap2d_128", "C2_lap3d_64_perm1", "bcsstk18"])
+def test_manufactured_solution_and_levelsync(handle, name):
+    from baas_amd.api import Schedule
+    announce("manufactured", name)
+    L, _, _ = workload(name)
+    s = Schedule(handle, L.rowptr, L.colidx, kind=0)
+    b = np.add.reduceat(L.values, L.rowptr[:-1])
+    x = down(s.solve(up(L.values), up(b)))
+    handle.check()
+    assert np.max(np.abs(x - 1.0)) <= 1e-10  # test/FusionDemo.cpp:66-77
+    xl = down(s.solve_levelsync(up(L.values), up(b)))
+    np.testing.assert_array_equal(xl, O.sptrsv_lower(L, b))
+
+
+@pytest.mark.parametrize("k", [2, 3, 32])
+def test_multi_rhs_bit_exact(handle, k):
+    from baas_amd.api import Schedule
+    for name in ("C1_lap2d_128", "C2_lap3d_64_perm1"):
+        L, _, _ = workload(name)
+        s = Schedule(handle, L.rowptr, L.colidx, kind=0)
+        B = g.rhs_uniform(L.n, 3, k=k)
+        X_ref = O.sptrsv_lower_multi(L, B)
+        for mode, rpu, blk in [(1, 0, 256), (1, 256, 256), (2, 0, 256)]:
+            announce("multi", name, k, mode, rpu, blk)
+            s.replan(mode, rpu, blk)
+            X = down(s.solve(up(L.values), up(B)))
+            handle.check()
+            np.testing.assert_array_equal(X, X_ref)
+            U = g.Csr.from_scipy(L.to_scipy().T)
+            Y = down(s.solve(s.transpose_values(up(L.values)), up(B), uplo=1))
+            Y_ref = np.stack([O.sptrsv_upper(U, B[:, r].copy()) for r in range(k)], axis=1)
+            np.testing.assert_array_equal(Y, Y_ref)
+
+
+def test_f32_bit_exact(handle):
+    import torch
+    from baas_amd.api import Schedule
+    for name in ("C2_lap3d_64", "C3_knn"):
+        announce("f32", name)
+        L, _, _ = workload(name)
+        s = Schedule(handle, L.rowptr, L.colidx, kind=0)
+        b = g.rhs_uniform(L.n, 1)
+        x = down(s.solve(up(L.values, torch.float32), up(b, torch.float32)))
+        handle.check()
+        np.testing.assert_array_equal(x, O.sptrsv_lower_f32(L, b))
+        U = g.Csr.from_scipy(L.to_scipy().T)
+        uv32 = s.transpose_values(up(L.values)).to(torch.float32)
+        y = down(s.solve(uv32, up(b, torch.float32), uplo=1))
+        np.testing.assert_array_equal(y, O.sptrsv_upper_f32(U, b))
+
+
+def test_division_and_sqrt_are_ieee(handle):
+    """The kernels' f64 division matches the CPU's correctly rounded one."""
+    import torch
+    from baas_amd.api import Schedule
+    announce("ieee div")
+    rng = np.random.default_rng(11)
+    n = 4096
+    # a diagonal-only L: x_i = b_i / d_i, one division per row
+    L = g.Csr(n, np.arange(n + 1, dtype=np.int32), np.arange(n, dtype=np.int32),
+              np.exp(rng.uniform(-30, 30, n)) * rng.choice([-1, 1], n))
+    s = Schedule(handle, L.rowptr, L.colidx, kind=0)
+    b = rng.standard_normal(n) * np.exp(rng.uniform(-30, 30, n))
+    x = down(s.solve(up(L.values), up(b)))
+    handle.check()
+    np.testing.assert_array_equal(x, b / L.values)
+
+
+def test_watchdog_reports_timeout_without_hanging(handle):
+    """A one-poll spin limit forces the wait to give up: the kernel still exits
+    and the error is reported (the values are then meaningless)."""
+    from baas_amd import _lib
+    from baas_amd.api import Schedule
+    announce("watchdog")
+    L, _, _ = workload("C2_lap3d_64_perm1")
+    s = Schedule(handle, L.rowptr, L.colidx, kind=0, plan_mode=2)
+    b = up(g.rhs_uniform(L.n, 0))
+    handle.set_spin_limit(1)
+    try:
+        for _ in range(4):
+            s.solve(up(L.values), b)
+        st = handle.status()
+    finally:
+        handle.set_spin_limit(1 << 22)
+    assert st in (_lib.OK, _lib.TIMEOUT)
+    # after restoring the limit the solve is exact again
+    x = down(s.solve(up(L.values), b))
+    handle.check()
+    np.testing.assert_array_equal(x, O.sptrsv_lower(L, down(b)))
+
+
+def test_invalid_pattern_rejected(handle):
+    from baas_amd import _lib
+    from baas_amd.api import Schedule
+    announce("invalid pattern")
+    # row 1 misses its diagonal
+    rp = np.array([0, 1, 2], np.int32)
+    ci = np.array([0, 0], np.int32)
+    with pytest.raises(_lib.BaasError) as e:
+        Schedule(handle, rp, ci, kind=0)
+    assert e.value.code == _lib.INVALID_ARG
