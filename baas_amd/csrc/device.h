@@ -59,15 +59,29 @@ __device__ __forceinline__ int claim_unit(int *ticket, int *slot) {
     return u;
 }
 
-// The last workgroup out resets the ticket and the exit counter so the next
-// launch (or graph replay) starts from zero without a memset.
-__device__ __forceinline__ void retire_block(int *ticket, int *done) {
+// Launch context: ticket, exit counter and ready-flag epoch live in device
+// memory (handle->counters[0..2]) so graph replays see fresh values.
+struct Ctx {
+    int *ticket;
+    int *done;
+    int *epoch;
+};
+
+__device__ __forceinline__ int read_epoch(const Ctx &c) { return ld_agent(c.epoch); }
+
+// The last workgroup out resets the ticket and the exit counter and advances
+// the epoch, so the next launch (or graph replay) needs no memset and its
+// ready flags cannot match stale ones.  Every workgroup has read the epoch
+// before it counts itself out.
+__device__ __forceinline__ void retire_block(const Ctx &c, int epoch) {
     if (threadIdx.x == 0) {
         __threadfence();
-        int d = atomicAdd(done, 1);
+        int d = atomicAdd(c.done, 1);
         if (d == (int)gridDim.x - 1) {
-            atomicExch(ticket, 0);
-            atomicExch(done, 0);
+            atomicExch(c.ticket, 0);
+            atomicExch(c.epoch, epoch == 0x7fffffff ? 1 : epoch + 1);
+            __threadfence();
+            atomicExch(c.done, 0);
         }
     }
 }

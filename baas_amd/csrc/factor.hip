@@ -1,0 +1,238 @@
+// factor.hip -- IC0 and ILU0 numeric factorisation on gfx950.
+//
+// Both run on the forward execution plan of the lower pattern of A (a row
+// depends exactly on the rows of its strictly lower entries) with the same
+// unit / sub-level / ready-flag protocol as the triangular solve.  One lane
+// factors one row with the reference's arithmetic:
+//   IC0  (merge/IC.cpp:32-89 and :296-393): up-looking, L_ij = (A_ij - <L_i,
+//        L_j>)/L_jj over sorted two-pointer dots with FMA; the diagonal is
+//        sqrt(A_ii - sum L_ik^2) only if positive, else A_ii is kept.  The
+//        self dot follows the reference build's vectorised form (blocks of
+//        four rounded squares added in order, then FMA for the tail).
+//   ILU0 (merge/ILU.cpp:350-386 and :638-720): IKJ, l_ij = a_ij / u_jj then
+//        a_ik -= l_ij u_jk over the matching upper entries of row j (FMA).
+// Values another row reads are stored with agent-scope write-through stores
+// and read with agent-scope loads, so rows of the same unit (after the
+// sub-level barrier) and of earlier units (after the ready flag) see them.
+#include "device.h"
+#include "internal.h"
+
+namespace baas {
+
+struct FactorArgs {
+    int n, nunits;
+    const int *rowptr, *colidx, *diagptr;
+    const int *key_begin, *sub_begin, *sub_ptr, *rows, *key;
+    int *flags;
+    Ctx ctx;
+    DevStatus *status;
+    long long spin_limit;
+};
+
+__device__ __forceinline__ double sparse_dot_dev(int l1, int u1, int l2, int u2, const int *__restrict__ idx,
+                                                 const double *lu) {
+    double r = 0.0;
+    while (l1 < u1 && l2 < u2) {
+        const int a = idx[l1], b = idx[l2];
+        if (a == b) {
+            r = fma(ld_agent(&lu[l1]), ld_agent(&lu[l2]), r);
+            l1++;
+            l2++;
+        } else if (a < b) {
+            l1++;
+        } else {
+            l2++;
+        }
+    }
+    return r;
+}
+
+__device__ __forceinline__ double self_dot_dev(int l, int u, const double *lu) {
+    const int k = u - l;
+    double r = 0.0;
+    int i = l;
+    if (k > 3) {
+        const int stop = l + (k & ~3);
+        for (; i < stop; i++) {
+            const double v = ld_agent(&lu[i]);
+            const double p = v * v;
+            r = r + p;
+        }
+    }
+    for (; i < u; i++) {
+        const double v = ld_agent(&lu[i]);
+        r = fma(v, v, r);
+    }
+    return r;
+}
+
+template <int KIND>
+__device__ __forceinline__ void factor_row(const FactorArgs &a, int i, double *lu) {
+    const int rb = a.rowptr[i], dg = a.diagptr[i];
+    if (KIND == BAAS_KIND_IC0) {
+        for (int k = rb; k < dg; k++) {
+            const int j = a.colidx[k];
+            const double dp = sparse_dot_dev(rb, dg, a.rowptr[j], a.diagptr[j], a.colidx, lu);
+            const double aij = ld_agent(&lu[k]);
+            const double v = (aij - dp) / ld_agent(&lu[a.diagptr[j]]);
+            st_agent(&lu[k], v);
+            wait_stores();
+        }
+        const double dv = self_dot_dev(rb, dg, lu);
+        const double aii = ld_agent(&lu[dg]);
+        if (aii - dv > 0) {
+            st_agent(&lu[dg], sqrt(aii - dv));
+        } else {
+            raise_status(a.status, BAAS_NONPOSITIVE_PIVOT, i);
+        }
+    } else {
+        const int re = a.rowptr[i + 1];
+        for (int j = rb; j < dg; j++) {
+            const int col = a.colidx[j];
+            const double pivot = ld_agent(&lu[a.diagptr[col]]);
+            if (!(fabs(pivot) > 1.0e-16)) raise_status(a.status, BAAS_ZERO_PIVOT, col);
+            const double tmp = ld_agent(&lu[j]) / pivot;
+            st_agent(&lu[j], tmp);
+            int k1 = j + 1, k2 = a.diagptr[col] + 1;
+            const int e2 = a.rowptr[col + 1];
+            while (k1 < re && k2 < e2) {
+                const int c1 = a.colidx[k1], c2 = a.colidx[k2];
+                if (c1 < c2) {
+                    k1++;
+                } else if (c1 > c2) {
+                    k2++;
+                } else {
+                    const double v = fma(-tmp, ld_agent(&lu[k2]), ld_agent(&lu[k1]));
+                    st_agent(&lu[k1], v);
+                    wait_stores();
+                    k1++;
+                    k2++;
+                }
+            }
+            wait_stores();
+        }
+    }
+}
+
+template <int KIND, int KEYMODE>
+__global__ __launch_bounds__(1024) void k_factor(FactorArgs a, double *lu) {
+    __shared__ int s_unit;
+    const int epoch = read_epoch(a.ctx);
+    for (;;) {
+        const int u = claim_unit(a.ctx.ticket, &s_unit);
+        if (u >= a.nunits) break;
+        const int kb = a.key_begin[u];
+        for (int si = a.sub_begin[u]; si < a.sub_begin[u + 1]; si++) {
+            const int qb = a.sub_ptr[si], qe = a.sub_ptr[si + 1];
+            for (int q = qb + threadIdx.x; q < qe; q += blockDim.x) {
+                const int rw = a.rows[q];
+                const int i = rw & 0x7fffffff;
+                for (int k = a.rowptr[i]; k < a.diagptr[i]; k++) {
+                    const int c = a.colidx[k];
+                    if (key_of<KEYMODE>(c, a.n, a.key) < kb) wait_ready(a.flags, c, epoch, a.spin_limit, a.status);
+                }
+                factor_row<KIND>(a, i, lu);
+                wait_stores();
+                if (rw < 0) st_agent(&a.flags[i], epoch);
+            }
+            __syncthreads();
+        }
+    }
+    retire_block(a.ctx, epoch);
+}
+
+__global__ void k_copy_f64(int n, const double *__restrict__ in, double *__restrict__ out) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = in[i];
+}
+
+// L values (lower CSR) and U = L^T values from an IC0 result in A's layout.
+template <class T>
+__global__ void k_split(int nnz, const int *__restrict__ l_src, const int *__restrict__ u_src,
+                        const double *__restrict__ lu, T *__restrict__ lv, T *__restrict__ uv) {
+    int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= nnz) return;
+    lv[e] = (T)lu[l_src[e]];
+    uv[e] = (T)lu[l_src[u_src[e]]];
+}
+
+static inline int nblk(long long n, int b = 256) { return (int)((n + b - 1) / b); }
+
+template <int KIND, int KM>
+static int launch_factor(baas_handle *h, const Plan &p, const FactorArgs &a, double *lu) {
+    auto kern = k_factor<KIND, KM>;
+    int per_cu = 0;
+    BAAS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, p.block, 0));
+    if (per_cu <= 0) return BAAS_UNSUPPORTED;
+    long long grid = (long long)per_cu * h->num_cus;
+    if (grid > p.nunits) grid = p.nunits;
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(p.block), 0, h->stream, a, lu);
+    BAAS_HIP_TRY(hipGetLastError());
+    return BAAS_OK;
+}
+
+template <int KIND>
+static int factor(baas_handle *h, baas_schedule *s, const double *a_vals, double *lu) {
+    if (!h || !s || !a_vals || !lu) return BAAS_INVALID_ARG;
+    if (s->kind != BAAS_KIND_IC0 && s->kind != BAAS_KIND_ILU0) return BAAS_INVALID_ARG;
+    if (a_vals != lu)
+        hipLaunchKernelGGL(k_copy_f64, dim3(nblk(s->nnz_full)), dim3(256), 0, h->stream, s->nnz_full, a_vals, lu);
+    const Plan &p = s->fwd;
+    FactorArgs a;
+    a.n = s->n;
+    a.nunits = p.nunits;
+    a.rowptr = s->a_rowptr.p;
+    a.colidx = s->a_colidx.p;
+    a.diagptr = s->a_diagptr.p;
+    a.key_begin = p.key_begin.p;
+    a.sub_begin = p.sub_begin.p;
+    a.sub_ptr = p.sub_ptr.p;
+    a.rows = p.rows.p;
+    a.key = p.key.p;
+    a.flags = s->flags.p;
+    a.ctx = Ctx{h->counters, h->counters + 1, h->counters + 2};
+    a.status = h->status;
+    a.spin_limit = h->spin_limit;
+    switch (p.keymode) {
+        case 0: return launch_factor<KIND, 0>(h, p, a, lu);
+        case 1: return launch_factor<KIND, 1>(h, p, a, lu);
+        default: return launch_factor<KIND, 2>(h, p, a, lu);
+    }
+}
+
+int ic0_split_dev(baas_handle *h, baas_schedule *s, const double *lu, double *lv, double *uv, hipStream_t st) {
+    int nnz = s->nnz_lower;
+    hipLaunchKernelGGL(k_split<double>, dim3(nblk(nnz)), dim3(256), 0, st, nnz, s->l_src.p, s->u_src.p, lu, lv, uv);
+    BAAS_HIP_TRY(hipGetLastError());
+    return BAAS_OK;
+}
+
+int ic0_split_dev32(baas_handle *h, baas_schedule *s, const double *lu, float *lv, float *uv, hipStream_t st) {
+    int nnz = s->nnz_lower;
+    hipLaunchKernelGGL(k_split<float>, dim3(nblk(nnz)), dim3(256), 0, st, nnz, s->l_src.p, s->u_src.p, lu, lv, uv);
+    BAAS_HIP_TRY(hipGetLastError());
+    return BAAS_OK;
+}
+
+}  // namespace baas
+
+using namespace baas;
+
+extern "C" int baas_ic0_f64(baas_handle *h, baas_schedule *s, const double *a_vals, double *lu) {
+    return factor<BAAS_KIND_IC0>(h, s, a_vals, lu);
+}
+
+extern "C" int baas_ilu0_f64(baas_handle *h, baas_schedule *s, const double *a_vals, double *lu) {
+    return factor<BAAS_KIND_ILU0>(h, s, a_vals, lu);
+}
+
+extern "C" int baas_ic0_split_f64(baas_handle *h, baas_schedule *s, const double *lu, double *lv, double *uv) {
+    if (!h || !s || !lu || !lv || !uv || s->kind == BAAS_KIND_TRSV) return BAAS_INVALID_ARG;
+    return ic0_split_dev(h, s, lu, lv, uv, h->stream);
+}
+
+extern "C" int baas_ic0_split_f32(baas_handle *h, baas_schedule *s, const double *lu, float *lv, float *uv) {
+    if (!h || !s || !lu || !lv || !uv || s->kind == BAAS_KIND_TRSV) return BAAS_INVALID_ARG;
+    return ic0_split_dev32(h, s, lu, lv, uv, h->stream);
+}

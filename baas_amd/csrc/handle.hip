@@ -14,6 +14,12 @@ size_t lds_limit(baas_handle *h) {
 
 }  // namespace baas
 
+// Ready flags start at 0, so the first epoch is 1 (handle->counters[2]).
+static bool set_first_epoch(baas_handle *h) {
+    int one = 1;
+    return hipMemcpy(h->counters + 2, &one, sizeof(int), hipMemcpyHostToDevice) == hipSuccess;
+}
+
 extern "C" int baas_create(baas_handle **out, void *stream) {
     if (!out) return BAAS_INVALID_ARG;
     *out = nullptr;
@@ -37,7 +43,7 @@ extern "C" int baas_create(baas_handle **out, void *stream) {
     }
     if (hipMemsetAsync(h->status, 0, sizeof(baas::DevStatus), h->stream) != hipSuccess ||
         hipMemsetAsync(h->counters, 0, 64 * sizeof(int), h->stream) != hipSuccess ||
-        hipStreamSynchronize(h->stream) != hipSuccess) {
+        hipStreamSynchronize(h->stream) != hipSuccess || !set_first_epoch(h)) {
         (void)hipFree(h->status);
         (void)hipFree(h->counters);
         delete h;

@@ -20,11 +20,12 @@
 namespace baas {
 
 struct SweepArgs {
-    int n, nunits, nrhs, epoch;
+    int n, nunits, nrhs;
     const int *rowptr, *colidx;
     const int *key_begin, *sub_begin, *sub_ptr, *rows, *key;
     int *flags;
-    int *ticket, *done;
+    Ctx ctx;
+    const int *skip;  // optional device flag: nonzero = return at once (converged PCG)
     DevStatus *status;
     long long spin_limit;
 };
@@ -35,9 +36,11 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a, const T *__restrict
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *xs = reinterpret_cast<T *>(smem_raw);
     __shared__ int s_unit;
+    if (a.skip && ld_agent(a.skip)) return;
     const int nrhs = a.nrhs;
+    const int epoch = read_epoch(a.ctx);
     for (;;) {
-        const int u = claim_unit(a.ticket, &s_unit);
+        const int u = claim_unit(a.ctx.ticket, &s_unit);
         if (u >= a.nunits) break;
         const int kb = a.key_begin[u];
         const int s0 = a.sub_begin[u], s1 = a.sub_begin[u + 1];
@@ -55,7 +58,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a, const T *__restrict
                 // wait for parents produced by other units
                 for (int j = ob; j < oe; j++) {
                     const int c = a.colidx[j];
-                    if (key_of<KEYMODE>(c, a.n, a.key) < kb) wait_ready(a.flags, c, a.epoch, a.spin_limit, a.status);
+                    if (key_of<KEYMODE>(c, a.n, a.key) < kb) wait_ready(a.flags, c, epoch, a.spin_limit, a.status);
                 }
                 T s = T(0);
                 if (UPPER) {
@@ -82,7 +85,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a, const T *__restrict
                 if (LDS) xs[(size_t)(key_of<KEYMODE>(row, a.n, a.key) - kb) * nrhs + r] = xr;
                 if (nrhs == 1 && rw < 0) {
                     wait_stores();
-                    st_agent(&a.flags[row], a.epoch);
+                    st_agent(&a.flags[row], epoch);
                 }
             }
             if (nrhs > 1) {
@@ -90,7 +93,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a, const T *__restrict
                 __syncthreads();
                 for (int t = threadIdx.x; t < qe - qb; t += blockDim.x) {
                     const int rw = a.rows[qb + t];
-                    if (rw < 0) st_agent(&a.flags[rw & 0x7fffffff], a.epoch);
+                    if (rw < 0) st_agent(&a.flags[rw & 0x7fffffff], epoch);
                 }
             } else if (!LDS) {
                 wait_stores();
@@ -100,7 +103,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a, const T *__restrict
             }
         }
     }
-    retire_block(a.ticket, a.done);
+    retire_block(a.ctx, epoch);
 }
 
 // Level-synchronous solve: one launch per row level (merge/SpTRSV.cpp:41-80).
@@ -144,7 +147,7 @@ static inline int nblk(long long n, int b = 256) { return (int)((n + b - 1) / b)
 
 template <class T, int KM, bool UP, bool LDS>
 static int launch_sweep_t(baas_handle *h, const Plan &p, const SweepArgs &a, const T *vals, const T *b, T *x,
-                          size_t lds) {
+                          size_t lds, hipStream_t stream) {
     auto kern = k_sweep<T, KM, UP, LDS>;
     int per_cu = 0;
     BAAS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, p.block, lds));
@@ -152,25 +155,26 @@ static int launch_sweep_t(baas_handle *h, const Plan &p, const SweepArgs &a, con
     long long grid = (long long)per_cu * h->num_cus;
     if (grid > p.nunits) grid = p.nunits;
     if (grid < 1) grid = 1;
-    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(p.block), lds, h->stream, a, vals, b, x);
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(p.block), lds, stream, a, vals, b, x);
     BAAS_HIP_TRY(hipGetLastError());
     return BAAS_OK;
 }
 
 template <class T, bool UP, bool LDS>
 static int launch_sweep_k(baas_handle *h, const Plan &p, const SweepArgs &a, const T *vals, const T *b, T *x,
-                          size_t lds) {
+                          size_t lds, hipStream_t st) {
     switch (p.keymode) {
-        case 0: return launch_sweep_t<T, 0, UP, LDS>(h, p, a, vals, b, x, lds);
-        case 1: return launch_sweep_t<T, 1, UP, LDS>(h, p, a, vals, b, x, lds);
-        default: return launch_sweep_t<T, 2, UP, LDS>(h, p, a, vals, b, x, lds);
+        case 0: return launch_sweep_t<T, 0, UP, LDS>(h, p, a, vals, b, x, lds, st);
+        case 1: return launch_sweep_t<T, 1, UP, LDS>(h, p, a, vals, b, x, lds, st);
+        default: return launch_sweep_t<T, 2, UP, LDS>(h, p, a, vals, b, x, lds, st);
     }
 }
 
 size_t lds_limit(baas_handle *h);
 
 template <class T>
-int sweep(baas_handle *h, baas_schedule *s, int uplo, const T *vals, const T *b, T *x, int nrhs) {
+int sweep(baas_handle *h, baas_schedule *s, int uplo, const T *vals, const T *b, T *x, int nrhs,
+          hipStream_t stream, const int *skip) {
     if (!h || !s || !vals || !b || !x || nrhs < 1 || (uplo != 0 && uplo != 1)) return BAAS_INVALID_ARG;
     if ((const void *)b == (const void *)x) return BAAS_INVALID_ARG;
     const Plan &p = uplo ? s->bwd : s->fwd;
@@ -178,8 +182,6 @@ int sweep(baas_handle *h, baas_schedule *s, int uplo, const T *vals, const T *b,
     a.n = s->n;
     a.nunits = p.nunits;
     a.nrhs = nrhs;
-    if (++h->epoch <= 0) h->epoch = 1;  // flags start at 0
-    a.epoch = h->epoch;
     a.rowptr = uplo ? s->u_rowptr.p : s->l_rowptr.p;
     a.colidx = uplo ? s->u_colidx.p : s->l_colidx.p;
     a.key_begin = p.key_begin.p;
@@ -188,21 +190,23 @@ int sweep(baas_handle *h, baas_schedule *s, int uplo, const T *vals, const T *b,
     a.rows = p.rows.p;
     a.key = p.key.p;
     a.flags = s->flags.p;
-    a.ticket = h->counters;
-    a.done = h->counters + 1;
+    a.ctx = Ctx{h->counters, h->counters + 1, h->counters + 2};
+    a.skip = skip;
     a.status = h->status;
     a.spin_limit = h->spin_limit;
     size_t lds = (size_t)p.max_unit_keys * nrhs * sizeof(T);
     bool use_lds = lds <= lds_limit(h);
     if (uplo)
-        return use_lds ? launch_sweep_k<T, true, true>(h, p, a, vals, b, x, lds)
-                       : launch_sweep_k<T, true, false>(h, p, a, vals, b, x, 0);
-    return use_lds ? launch_sweep_k<T, false, true>(h, p, a, vals, b, x, lds)
-                   : launch_sweep_k<T, false, false>(h, p, a, vals, b, x, 0);
+        return use_lds ? launch_sweep_k<T, true, true>(h, p, a, vals, b, x, lds, stream)
+                       : launch_sweep_k<T, true, false>(h, p, a, vals, b, x, 0, stream);
+    return use_lds ? launch_sweep_k<T, false, true>(h, p, a, vals, b, x, lds, stream)
+                   : launch_sweep_k<T, false, false>(h, p, a, vals, b, x, 0, stream);
 }
 
-template int sweep<double>(baas_handle *, baas_schedule *, int, const double *, const double *, double *, int);
-template int sweep<float>(baas_handle *, baas_schedule *, int, const float *, const float *, float *, int);
+template int sweep<double>(baas_handle *, baas_schedule *, int, const double *, const double *, double *, int,
+                           hipStream_t, const int *);
+template int sweep<float>(baas_handle *, baas_schedule *, int, const float *, const float *, float *, int,
+                          hipStream_t, const int *);
 
 }  // namespace baas
 
@@ -210,12 +214,12 @@ using namespace baas;
 
 extern "C" int baas_sptrsv_f64(baas_handle *h, baas_schedule *s, int uplo, const double *vals, const double *b,
                                double *x, int nrhs) {
-    return sweep<double>(h, s, uplo, vals, b, x, nrhs);
+    return sweep<double>(h, s, uplo, vals, b, x, nrhs, h ? h->stream : nullptr, nullptr);
 }
 
 extern "C" int baas_sptrsv_f32(baas_handle *h, baas_schedule *s, int uplo, const float *vals, const float *b,
                                float *x, int nrhs) {
-    return sweep<float>(h, s, uplo, vals, b, x, nrhs);
+    return sweep<float>(h, s, uplo, vals, b, x, nrhs, h ? h->stream : nullptr, nullptr);
 }
 
 extern "C" int baas_sptrsv_levelsync_f64(baas_handle *h, baas_schedule *s, const double *vals, const double *b,
