@@ -60,6 +60,7 @@ _SIGS = {
     "baas_cast_f64_f32": (_i, [_P, _i, _P, _P]),
     "baas_cast_f32_f64": (_i, [_P, _i, _P, _P]),
     "baas_sptrsv_grad_vals_f64": (_i, [_P, _P, _P, _P, _P, _i]),
+    "baas_csr_outer_grad_f64": (_i, [_P, _i, _P, _P, _P, _P, _P, _i]),
     "baas_ic0_f64": (_i, [_P, _P, _P, _P]),
     "baas_ilu0_f64": (_i, [_P, _P, _P, _P]),
     "baas_ic0_split_f64": (_i, [_P, _P, _P, _P, _P]),
@@ -75,6 +76,7 @@ _SIGS = {
     "baas_batch_create": (_i, [_P, _i, _P, _i, _P, _P, _P, _i, C.POINTER(Opts), C.POINTER(_P)]),
     "baas_batch_destroy": (_i, [_P]),
     "baas_batch_pcg": (_i, [_P, _P, _P, _d, _i, _P]),
+    "baas_batch_info": (_i, [_P, _P, _P]),
 }
 
 

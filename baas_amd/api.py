@@ -227,3 +227,56 @@ class PCG:
                 L.load().baas_pcg_destroy(self._p)
         except Exception:
             pass
+
+
+class Batch:
+    """Many SPD systems x many right-hand sides: IC0 + PCG (FP32 factor, FP64 CG)."""
+
+    def __init__(self, h: Handle, mats, nrhs, block_threads=0):
+        import numpy as _np
+        self.h = h
+        self.sizes = [m.n for m in mats]
+        self.sys_ptr = _np.zeros(len(mats) + 1, _np.int32)
+        _np.cumsum(self.sizes, out=self.sys_ptr[1:])
+        N = int(self.sys_ptr[-1])
+        rp, ci, va = [_np.zeros(1, _np.int64)], [], []
+        off, nz = 0, 0
+        for m in mats:
+            rp.append(m.rowptr[1:].astype(_np.int64) + nz)
+            ci.append(m.colidx.astype(_np.int64) + off)
+            va.append(m.values)
+            off += m.n
+            nz += m.nnz
+        self.rowptr = _dev_i32(_np.concatenate(rp))
+        self.colidx = _dev_i32(_np.concatenate(ci))
+        self.vals = _dev_f(_np.concatenate(va))
+        self.N, self.k = N, int(nrhs)
+        o = L.Opts(8, L.PLAN_RANGE, 0, block_threads, 0)
+        self._b = C.c_void_p()
+        L.call("baas_batch_create", h.ptr, len(mats), C.c_void_p(self.sys_ptr.ctypes.data), N, _ptr(self.rowptr),
+               _ptr(self.colidx), _ptr(self.vals), self.k, C.byref(o), C.byref(self._b))
+
+    def solve(self, B, X=None, tol=1e-8, maxiter=1000):
+        """B, X: (N, nrhs) float64 device tensors.  Returns X and per-(system, rhs) iterations."""
+        import numpy as _np
+        if X is None:
+            X = torch.zeros_like(B)
+        it = _np.zeros(len(self.sizes) * self.k, _np.int32)
+        st = L.load().baas_batch_pcg(self._b, _ptr(B), _ptr(X), float(tol), int(maxiter),
+                                      C.c_void_p(it.ctypes.data))
+        if st not in (L.OK, L.NOT_CONVERGED):
+            raise L.BaasError(st, "baas_batch_pcg")
+        return X, it.reshape(len(self.sizes), self.k)
+
+    def info(self):
+        t = (C.c_double * 3)()
+        r = (C.c_int * 3)()
+        L.call("baas_batch_info", self._b, C.cast(t, C.c_void_p), C.cast(r, C.c_void_p))
+        return {"ic0_setup_s": t[0], "cg_s": t[1], "tune_s": t[2], "rows_per_unit": list(r)}
+
+    def __del__(self):
+        try:
+            if self._b:
+                L.load().baas_batch_destroy(self._b)
+        except Exception:
+            pass

@@ -186,6 +186,34 @@ int build_plans_mode(baas_schedule *s, int mode, int rows_per_unit) {
     return BAAS_OK;
 }
 
+// RANGE plans over caller-chosen unit boundaries (row space, ascending);
+// the backward plan uses the same units in reverse order.
+int make_plan_custom(baas_schedule *s, const std::vector<int> &fu, int block) {
+    hipStream_t st = s->h->stream;
+    const int n = s->n;
+    if (fu.empty() || fu.front() != 0 || fu.back() != n) return BAAS_INVALID_ARG;
+    HostCsr L, U;
+    BAAS_TRY(fetch(L, s->l_rowptr.p, s->l_colidx.p, n, st));
+    BAAS_TRY(fetch(U, s->u_rowptr.p, s->u_colidx.p, n, st));
+    std::vector<int> lp(n + 1), li, up(n + 1), ui;
+    lp[0] = up[0] = 0;
+    for (int r = 0; r < n; r++) {
+        for (int j = L.ptr[r]; j < L.ptr[r + 1] - 1; j++) li.push_back(L.idx[j]);
+        lp[r + 1] = (int)li.size();
+        for (int j = U.ptr[r] + 1; j < U.ptr[r + 1]; j++) ui.push_back(U.idx[j]);
+        up[r + 1] = (int)ui.size();
+    }
+    std::vector<int> fo(n), bo(n);
+    for (int r = 0; r < n; r++) fo[r] = r, bo[r] = n - 1 - r;
+    int nu = (int)fu.size() - 1;
+    std::vector<int> bu(nu + 1);
+    for (int u = 0; u <= nu; u++) bu[u] = n - fu[nu - u];
+    if (block <= 0) block = 256;
+    BAAS_TRY(make_plan(s->fwd, n, fo, fu, lp, li, up, ui, 0, BAAS_PLAN_RANGE, block, st));
+    BAAS_TRY(make_plan(s->bwd, n, bo, bu, up, ui, lp, li, 1, BAAS_PLAN_RANGE, block, st));
+    return BAAS_OK;
+}
+
 // Default packing before any autotuning: RANGE when a row-range unit keeps
 // most dependencies inside the unit (the matrix is banded), SCHEDULE else.
 int build_plans(baas_schedule *s) {

@@ -143,6 +143,19 @@ __global__ void k_grad_vals(int nnz, int nrhs, const int *__restrict__ row_of, c
     dv[e] = -s;
 }
 
+// dA_ij = -sum_r lambda_{i,r} x_{j,r} over any CSR pattern, one lane per row
+__global__ void k_csr_outer_grad(int m, int nrhs, const int *__restrict__ rp, const int *__restrict__ ci,
+                                 const double *__restrict__ lam, const double *__restrict__ x, double *__restrict__ dv) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m) return;
+    for (int e = rp[i]; e < rp[i + 1]; e++) {
+        const int j = ci[e];
+        double s = 0.0;
+        for (int r = 0; r < nrhs; r++) s = fma(lam[(size_t)i * nrhs + r], x[(size_t)j * nrhs + r], s);
+        dv[e] = -s;
+    }
+}
+
 static inline int nblk(long long n, int b = 256) { return (int)((n + b - 1) / b); }
 
 template <class T, int KM, bool UP, bool LDS>
@@ -263,6 +276,14 @@ extern "C" int baas_sptrsv_grad_vals_f64(baas_handle *h, baas_schedule *s, const
     int nnz = s->nnz_lower;
     hipLaunchKernelGGL(k_grad_vals, dim3(nblk(nnz)), dim3(256), 0, h->stream, nnz, nrhs, s->l_row_of.p,
                        s->l_colidx.p, lam, x, dv);
+    BAAS_HIP_TRY(hipGetLastError());
+    return BAAS_OK;
+}
+
+extern "C" int baas_csr_outer_grad_f64(baas_handle *h, int m, const int *rp, const int *ci, const double *lam,
+                                       const double *x, double *dv, int nrhs) {
+    if (!h || m <= 0 || !rp || !ci || !lam || !x || !dv || nrhs < 1) return BAAS_INVALID_ARG;
+    hipLaunchKernelGGL(k_csr_outer_grad, dim3(nblk(m)), dim3(256), 0, h->stream, m, nrhs, rp, ci, lam, x, dv);
     BAAS_HIP_TRY(hipGetLastError());
     return BAAS_OK;
 }

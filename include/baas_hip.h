@@ -132,6 +132,11 @@ int baas_cast_f32_f64(baas_handle *h, int n, const float *in, double *out);
 int baas_sptrsv_grad_vals_f64(baas_handle *h, baas_schedule *s, const double *lambda,
                               const double *x, double *dvals, int nrhs);
 
+/* Adjoint of x = A^-1 b for any CSR A: dA_ij = -sum_r lambda_{i,r} x_{j,r}
+ * with lambda = A^-T g (for ICCG: a second PCG solve on the symmetric A). */
+int baas_csr_outer_grad_f64(baas_handle *h, int m, const int *rowptr, const int *colidx,
+                            const double *lambda, const double *x, double *dvals, int nrhs);
+
 /* ---- factorizations --------------------------------------------------------
  * IC0  (spic0_csr_uL_p2p_group_merge_no_perm, merge/IC.cpp:296-393)
  * ILU0 (ilu0_p2p_csr_group_merge_no_perm_alloc, merge/ILU.cpp:638-720)
@@ -184,6 +189,14 @@ int baas_batch_destroy(baas_batch *b);
 /* IC0 (FP64) then PCG with an FP32 factor and FP64 CG on all systems and
  * right-hand sides; iters (host int[nsys*nrhs]) receives per-column counts. */
 int baas_batch_pcg(baas_batch *b, const double *B, double *X, double tol, int maxiter, int *iters);
+/* times[0..2] (host) = IC0 + setup, CG loop, autotuning seconds of the last
+ * call; rpu[0..2] (host) = tuned rows per unit for systems of <= 4096,
+ * <= 16384 and more rows. */
+int baas_batch_info(const baas_batch *b, double *times, int *rpu);
+
+/* Re-pack a schedule's execution plan (RANGE or SCHEDULE, rows per unit,
+ * threads per workgroup: a multiple of 64 up to 1024). */
+int baas_replan(baas_schedule *s, int mode, int rows_per_unit, int block_threads);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,153 @@
+"""GPU parity of IC0 / ILU0, SpMV / BLAS-1 and ICCG through the C-ABI.
+
+IC0 and ILU0 must equal the oracle (itself bit-exact with the compiled
+reference) bit for bit; SpMV and waxpby likewise; dots within 1e-13
+relative (the reduction order differs from the reference's OpenMP one);
+PCG must reach ||r||/||r0|| < 1e-8 within +-2 iterations of the oracle's
+restatement of pcg_ref, in FP64 and with the FP32 factor.
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+from baas_amd import _lib
+from baas_amd import matgen as g
+from gpu_util import announce, down, up
+from workloads import workload
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def handle():
+    from baas_amd.api import Handle
+    return Handle()
+
+
+def _status_ok(h, allowed=()):
+    st = h.status()
+    assert st == _lib.OK or st in allowed, _lib.BaasError(st, "kernel")
+    return st
+
+
+@pytest.mark.parametrize("name", ["C2_lap3d_64", "C2_lap3d_64_perm1", "C3_knn", "C5_diff3d_48", "bcsstk18",
+                                  "bcsstk38"])
+def test_ic0_bit_exact(handle, name):
+    from baas_amd.api import Schedule
+    _, _, A = workload(name)
+    want = O.ic0(A)
+    s = Schedule(handle, A.rowptr, A.colidx, kind=_lib.KIND_IC0)
+    av = up(A.values)
+    for mode, rpu, blk in [(1, 0, 256), (2, 0, 256), (2, 64, 64), (1, 1024, 512)]:
+        announce("ic0", name, mode, rpu, blk)
+        s.replan(mode, rpu, blk)
+        lu = s.ic0(av)
+        _status_ok(handle, (_lib.NONPOSITIVE_PIVOT,))
+        np.testing.assert_array_equal(down(lu), want, err_msg=f"{name} {mode} {rpu} {blk}")
+    # split into L and U = L^T, FP64 and FP32
+    Lf, Uf = O.ic0_split(A, want)
+    lv, uv = s.ic0_split(lu)
+    np.testing.assert_array_equal(down(lv), Lf.values)
+    np.testing.assert_array_equal(down(uv), Uf.values)
+    import torch
+    lv32, uv32 = s.ic0_split(lu, torch.float32)
+    np.testing.assert_array_equal(down(lv32), Lf.values.astype(np.float32))
+
+
+def test_ic0_nonpositive_pivot_semantics(handle):
+    from baas_amd.api import Schedule
+    announce("ic0 nonpositive pivot")
+    A = g.Csr(2, np.array([0, 2, 4], np.int32), np.array([0, 1, 0, 1], np.int32), np.array([1.0, 2.0, 2.0, 1.0]))
+    s = Schedule(handle, A.rowptr, A.colidx, kind=_lib.KIND_IC0)
+    lu = s.ic0(up(A.values))
+    assert handle.status() == _lib.NONPOSITIVE_PIVOT
+    np.testing.assert_array_equal(down(lu), O.ic0(A))
+
+
+@pytest.mark.parametrize("name", ["C2_convdiff_64", "C2_lap3d_64", "bcsstk18"])
+def test_ilu0_bit_exact(handle, name):
+    from baas_amd.api import Schedule
+    _, _, A = workload(name)
+    want = O.ilu0(A)
+    s = Schedule(handle, A.rowptr, A.colidx, kind=_lib.KIND_ILU0)
+    for mode in (1, 2):
+        announce("ilu0", name, mode)
+        s.replan(mode, 0, 256)
+        lu = s.ilu0(up(A.values))
+        _status_ok(handle)
+        np.testing.assert_array_equal(down(lu), want)
+
+
+def test_ilu0_zero_pivot_reported(handle):
+    from baas_amd.api import Schedule
+    announce("ilu0 zero pivot")
+    A = g.Csr(2, np.array([0, 2, 4], np.int32), np.array([0, 1, 0, 1], np.int32), np.array([0.0, 1.0, 1.0, 1.0]))
+    s = Schedule(handle, A.rowptr, A.colidx, kind=_lib.KIND_ILU0)
+    s.ilu0(up(A.values))
+    assert handle.status() == _lib.ZERO_PIVOT
+
+
+def test_spmv_and_blas(handle):
+    from baas_amd import api
+    announce("blas")
+    _, _, A = workload("bcsstk18")
+    rng = np.random.default_rng(5)
+    x, y = rng.standard_normal(A.n), rng.standard_normal(A.n)
+    rp, ci, va = up(A.rowptr), up(A.colidx), up(A.values)
+    got = down(api.spmv(handle, rp, ci, va, up(x)))
+    np.testing.assert_array_equal(got, O.spmv(A, x))
+    yy = up(y.copy())
+    api.spmv(handle, rp, ci, va, up(x), yy, -1.0, 1.0)
+    ref = y.copy()
+    O.lib.orc_dcsrmv(A.n, A.rowptr, A.colidx, A.values, x, ref, -1.0, 1.0)
+    np.testing.assert_array_equal(down(yy), ref)
+    for a, b in [(1.0, 1.0), (1.0, -1.0), (1.0, 0.375), (0.625, 1.0), (-1.0, 0.0), (-1.0, 0.5), (0.25, 0.0),
+                 (0.3, 0.9)]:
+        w = up(np.zeros(A.n))
+        api.axpby(handle, w, a, up(x), b, up(y))
+        np.testing.assert_array_equal(down(w), O.waxpby(a, x, b, y), err_msg=f"{a} {b}")
+    d = float(down(api.dot(handle, up(x), up(y)))[0])
+    assert abs(d - float(np.dot(x, y))) <= 1e-13 * np.abs(x * y).sum()
+    nr = float(down(api.nrm2(handle, up(x)))[0])
+    assert abs(nr - np.linalg.norm(x)) <= 1e-13 * np.linalg.norm(x)
+    handle.check()
+
+
+@pytest.mark.parametrize("name,prec", [("C3_knn", 0), ("C3_knn", 1), ("bcsstk18", 0), ("bcsstk18", 1)])
+def test_pcg_iterations_match_oracle(handle, name, prec):
+    from baas_amd.api import PCG, Schedule
+    announce("pcg", name, prec)
+    _, _, A = workload(name)
+    b = g.knn_stiffness()[1] if name == "C3_knn" else np.random.default_rng(9).standard_normal(A.n)
+    _, k_ref, _ = O.pcg(A, b, maxiter=5000, tol=1e-8, precision=prec)
+    s = Schedule(handle, A.rowptr, A.colidx, kind=_lib.KIND_IC0)
+    av = up(A.values)
+    solver = PCG(handle, s, av, precision=prec)
+    x, k, rel = solver.solve(up(b), tol=1e-8, maxiter=5000)
+    # bcsstk18 is not an M-matrix: IC0 meets non-positive pivots and keeps the
+    # diagonal there, exactly as the reference does (merge/IC.cpp:80-87)
+    _status_ok(handle, (_lib.NONPOSITIVE_PIVOT,))
+    assert k > 0 and rel < 1e-8
+    assert abs(k - k_ref) <= 2, (k, k_ref)
+    xs = down(x)
+    res = np.linalg.norm(b - A.to_scipy() @ xs) / np.linalg.norm(b)
+    assert res <= 1.1e-8
+    # a second solve replays the captured graph and must give the same answer
+    x2, k2, _ = solver.solve(up(b), tol=1e-8, maxiter=5000)
+    _status_ok(handle, (_lib.NONPOSITIVE_PIVOT,))
+    assert k2 == k
+    np.testing.assert_array_equal(down(x2), xs)
+
+
+def test_pcg_mixed_error_vs_fp64(handle):
+    from baas_amd.api import PCG, Schedule
+    announce("pcg mixed vs fp64")
+    A, b = g.knn_stiffness()
+    s = Schedule(handle, A.rowptr, A.colidx, kind=_lib.KIND_IC0)
+    av = up(A.values)
+    x64, k64, _ = PCG(handle, s, av, 0).solve(up(b), tol=1e-8, maxiter=5000)
+    x32, k32, _ = PCG(handle, s, av, 1).solve(up(b), tol=1e-8, maxiter=5000)
+    handle.check()
+    err = np.linalg.norm(down(x32) - down(x64)) / np.linalg.norm(down(x64))
+    print(f"[pcg] fp64 iters {k64}, fp32-factor iters {k32}, rel. difference of solutions {err:.3e}")
+    assert err < 1e-6
