@@ -110,8 +110,10 @@ class Schedule:
         L.call("baas_schedule_times", self._s, C.byref(g), C.byref(h))
         return g.value, h.value
 
-    def replan(self, mode, rows_per_unit=0, block_threads=0):
+    def replan(self, mode, rows_per_unit=0, block_threads=0, dataflow=None):
         L.call("baas_replan", self._s, int(mode), int(rows_per_unit), int(block_threads))
+        if dataflow is not None:
+            L.call("baas_plan_dataflow", self._s, int(bool(dataflow)))
 
     def upper_pattern(self):
         rp, ci = C.c_void_p(), C.c_void_p()

@@ -57,9 +57,26 @@ def build(verbose: bool = False) -> str:
     if r.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
     os.replace(tmp, LIB)
+    build_demo()
     if verbose:
-        print("built", LIB)
+        print("built", LIB, "and", DEMO)
     return LIB
+
+
+DEMO = os.path.join(HERE, "bin", "baas_demo")
+
+
+def build_demo():
+    """The C++ runtime classes (host/) and their driver, linked to the library."""
+    os.makedirs(os.path.dirname(DEMO), exist_ok=True)
+    host = os.path.join(HERE, "host")
+    cmd = [_hipcc(), "-O2", "-std=c++17", "-Wno-unused-result", os.path.join(host, "baas_runtime.cpp"),
+           os.path.join(host, "baas_demo.cpp"), "-L" + HERE, "-lbaas_hip", "-Wl,-rpath,$ORIGIN/..",
+           "-o", DEMO + ".tmp"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"demo build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    os.replace(DEMO + ".tmp", DEMO)
 
 
 if __name__ == "__main__":

@@ -80,6 +80,13 @@ struct Plan {
     DBuf<int> sub_ptr;    // [nsub+1]   offsets into rows
     DBuf<int> rows;       // [n]        row ids; bit 31 set = row has a child in a later unit
     DBuf<int> key;        // [n]        keymode 2 only
+    // dataflow form (one lane per row, LDS slots instead of sub-level barriers)
+    DBuf<int> unit_ptr;   // [nunits+1] unit boundaries in rows
+    DBuf<int> scol;       // [nnz]      LDS slot of each entry's parent (-1 on the diagonal)
+    DBuf<int> ext_ptr;    // [nunits+1] external parents of each unit ...
+    DBuf<int> ext_cols;   // [...]      ... in order of first use
+    int max_slots = 0;    // max over units of keys + external parents
+    int dataflow = 0;     // use the dataflow kernel for single-RHS sweeps
     std::vector<int> h_unit_ptr;  // host copy: unit boundaries in rows
     std::vector<int> h_rows;      // host copy of rows (without flag bit)
 };

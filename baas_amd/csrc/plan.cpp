@@ -59,7 +59,8 @@ int upload(DBuf<T> &d, const std::vector<T> &h, hipStream_t st) {
 //   keymode        0 key = row, 1 key = n-1-row, 2 key = position in order
 int make_plan(Plan &p, int n, const std::vector<int> &order, const std::vector<int> &unit_ptr,
               const std::vector<int> &dptr, const std::vector<int> &didx, const std::vector<int> &kptr,
-              const std::vector<int> &kidx, int keymode, int mode, int block, hipStream_t st) {
+              const std::vector<int> &kidx, int keymode, int mode, int block, hipStream_t st,
+              const HostCsr &full, bool upper) {
     int nunits = (int)unit_ptr.size() - 1;
     std::vector<int> pos(n), unit_of(n), key(n);
     for (int q = 0; q < n; q++) pos[order[q]] = q;
@@ -108,7 +109,43 @@ int make_plan(Plan &p, int n, const std::vector<int> &order, const std::vector<i
     }
     key_begin[nunits] = n;
     sub_begin[nunits] = (int)sub_ptr.size() - 1;
-    if (keymode == 0) key_begin[nunits] = n;
+    // dataflow slots: intra-unit parents live at key - key_begin, external
+    // parents after the unit's keys in order of first use
+    std::vector<int> scol(full.idx.size(), -1), ext_ptr(nunits + 1, 0), ext_cols;
+    std::vector<int> ext_slot(n, -1), ext_touched;
+    int max_slots = 0;
+    for (int u = 0; u < nunits; u++) {
+        const int kb = key_begin[u], nk = unit_ptr[u + 1] - unit_ptr[u];
+        ext_ptr[u] = (int)ext_cols.size();
+        for (int q = unit_ptr[u]; q < unit_ptr[u + 1]; q++) {
+            const int r = rows[q] & 0x7fffffff;
+            const int rb = full.ptr[r], re = full.ptr[r + 1];
+            const int dg = upper ? rb : re - 1;
+            for (int j = rb; j < re; j++) {
+                if (j == dg) continue;
+                const int c = full.idx[j];
+                if (unit_of[c] == u) {
+                    scol[j] = key[c] - kb;
+                } else {
+                    if (ext_slot[c] < 0) {
+                        ext_slot[c] = nk + (int)(ext_cols.size() - ext_ptr[u]);
+                        ext_cols.push_back(c);
+                        ext_touched.push_back(c);
+                    }
+                    scol[j] = ext_slot[c];
+                }
+            }
+        }
+        for (int c : ext_touched) ext_slot[c] = -1;
+        ext_touched.clear();
+        max_slots = std::max(max_slots, nk + (int)ext_cols.size() - ext_ptr[u]);
+    }
+    ext_ptr[nunits] = (int)ext_cols.size();
+    BAAS_TRY(upload(p.scol, scol, st));
+    BAAS_TRY(upload(p.ext_ptr, ext_ptr, st));
+    BAAS_TRY(upload(p.ext_cols, ext_cols, st));
+    BAAS_TRY(upload(p.unit_ptr, unit_ptr, st));
+    p.max_slots = max_slots;
     p.n = n;
     p.nunits = nunits;
     p.nsub = (int)sub_ptr.size() - 1;
@@ -157,8 +194,8 @@ int build_plans_mode(baas_schedule *s, int mode, int rows_per_unit) {
         fu.push_back(n);
         bu = fu;
         // forward: parents = L off-diagonals, children = U off-diagonals
-        BAAS_TRY(make_plan(s->fwd, n, fwd_order, fu, lp, li, up, ui, 0, mode, block, st));
-        BAAS_TRY(make_plan(s->bwd, n, bwd_order, bu, up, ui, lp, li, 1, mode, block, st));
+        BAAS_TRY(make_plan(s->fwd, n, fwd_order, fu, lp, li, up, ui, 0, mode, block, st, L, false));
+        BAAS_TRY(make_plan(s->bwd, n, bwd_order, bu, up, ui, lp, li, 1, mode, block, st, U, true));
         return BAAS_OK;
     }
     // SCHEDULE: merged levels -> groups in level order -> nodes ascending
@@ -181,8 +218,8 @@ int build_plans_mode(baas_schedule *s, int mode, int rows_per_unit) {
     int nu = (int)fu.size() - 1;
     bu.resize(nu + 1);
     for (int u = 0; u <= nu; u++) bu[u] = n - fu[nu - u];
-    BAAS_TRY(make_plan(s->fwd, n, fwd_order, fu, lp, li, up, ui, 2, mode, block, st));
-    BAAS_TRY(make_plan(s->bwd, n, bwd_order, bu, up, ui, lp, li, 2, mode, block, st));
+    BAAS_TRY(make_plan(s->fwd, n, fwd_order, fu, lp, li, up, ui, 2, mode, block, st, L, false));
+    BAAS_TRY(make_plan(s->bwd, n, bwd_order, bu, up, ui, lp, li, 2, mode, block, st, U, true));
     return BAAS_OK;
 }
 
@@ -209,8 +246,8 @@ int make_plan_custom(baas_schedule *s, const std::vector<int> &fu, int block) {
     std::vector<int> bu(nu + 1);
     for (int u = 0; u <= nu; u++) bu[u] = n - fu[nu - u];
     if (block <= 0) block = 256;
-    BAAS_TRY(make_plan(s->fwd, n, fo, fu, lp, li, up, ui, 0, BAAS_PLAN_RANGE, block, st));
-    BAAS_TRY(make_plan(s->bwd, n, bo, bu, up, ui, lp, li, 1, BAAS_PLAN_RANGE, block, st));
+    BAAS_TRY(make_plan(s->fwd, n, fo, fu, lp, li, up, ui, 0, BAAS_PLAN_RANGE, block, st, L, false));
+    BAAS_TRY(make_plan(s->bwd, n, bo, bu, up, ui, lp, li, 1, BAAS_PLAN_RANGE, block, st, U, true));
     return BAAS_OK;
 }
 

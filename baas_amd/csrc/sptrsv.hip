@@ -106,6 +106,143 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a, const T *__restrict
     retire_block(a.ctx, epoch);
 }
 
+// ---------------------------------------------------------------------------
+// Dataflow sweep.  One lane owns one row; the unit's solved values live in
+// LDS slots pre-filled with a signalling-NaN sentinel, so a lane reads its
+// parent's slot until the sentinel is gone -- no sub-level barriers, every
+// row waits exactly for its own parents.  Parents in earlier units are staged
+// into LDS slots (after the unit's own keys) by dedicated prefetch waves that
+// poll the per-row ready flags, so the flag round trips overlap with compute.
+// Each lane retries in one loop that consumes every ready entry in ascending
+// (descending for U) column order, so lanes of one wavefront that depend on
+// each other still make progress.
+template <class T> struct Sentinel;
+template <> struct Sentinel<double> {
+    static __device__ __forceinline__ double value() { return __longlong_as_double(0x7ff4deadbeef0001LL); }
+    static __device__ __forceinline__ bool is(double v) { return __double_as_longlong(v) == 0x7ff4deadbeef0001LL; }
+};
+template <> struct Sentinel<float> {
+    static __device__ __forceinline__ float value() { return __int_as_float(0x7fa0dead); }
+    static __device__ __forceinline__ bool is(float v) { return __float_as_int(v) == 0x7fa0dead; }
+};
+
+template <class T>
+__device__ __forceinline__ T ld_lds(const T *p) {
+    return __hip_atomic_load(const_cast<T *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+template <class T>
+__device__ __forceinline__ void st_lds(T *p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+struct DfArgs {
+    int n, nunits, npf;
+    const int *rowptr, *scol;
+    const int *key_begin, *unit_ptr, *rows, *key, *ext_ptr, *ext_cols;
+    int *flags;
+    Ctx ctx;
+    const int *skip;
+    DevStatus *status;
+    long long spin_limit;
+};
+
+constexpr int DF_CHUNK = 8;
+
+template <class T, int KEYMODE, bool UPPER>
+__global__ __launch_bounds__(1024) void k_sweep_df(DfArgs a, const T *__restrict__ vals, const T *__restrict__ b,
+                                                  T *x) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    T *sl = reinterpret_cast<T *>(smem_raw);
+    __shared__ int s_unit;
+    if (a.skip && ld_agent(a.skip)) return;
+    const int epoch = read_epoch(a.ctx);
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int npf = a.npf;
+    for (;;) {
+        const int u = claim_unit(a.ctx.ticket, &s_unit);
+        if (u >= a.nunits) break;
+        const int kb = a.key_begin[u], nk = a.key_begin[u + 1] - kb;
+        const int e0 = a.ext_ptr[u], e1 = a.ext_ptr[u + 1];
+        const int nslots = nk + (e1 - e0);
+        for (int i = threadIdx.x; i < nslots; i += blockDim.x) st_lds(&sl[i], Sentinel<T>::value());
+        __syncthreads();
+        if (wv < npf) {
+            // prefetch waves: stage external parents in order of first use
+            for (int e = e0 + wv * 64 + lane; e < e1; e += npf * 64) {
+                const int c = a.ext_cols[e];
+                wait_ready(a.flags, c, epoch, a.spin_limit, a.status);
+                st_lds(&sl[nk + (e - e0)], ld_agent(&x[c]));
+            }
+        } else {
+            const int q0 = a.unit_ptr[u], q1 = a.unit_ptr[u + 1];
+            const int C = nw - npf, cw = wv - npf;
+            for (int base = q0 + cw * 64; base < q1; base += C * 64) {
+                const int q = base + lane;
+                bool pending = q < q1;
+                int rw = 0, row = 0, dg = 0, left = 0, jn = 0;
+                T bv = T(0), s = T(0);
+                if (pending) {
+                    rw = a.rows[q];
+                    row = rw & 0x7fffffff;
+                    const int rb = a.rowptr[row], re = a.rowptr[row + 1];
+                    dg = UPPER ? rb : re - 1;
+                    left = re - rb - 1;               // off-diagonal entries still to consume
+                    jn = UPPER ? re - 1 : rb;         // next entry to load
+                    bv = b[row];
+                }
+                int cs[DF_CHUNK];
+                T cv[DF_CHUNK];
+                int nc = 0, pc = 0;
+                long long spins = 0;
+                while (pending) {
+                    if (pc == nc && left > 0) {  // load the next chunk of entries
+                        nc = left < DF_CHUNK ? left : DF_CHUNK;
+#pragma unroll
+                        for (int t = 0; t < DF_CHUNK; t++) {
+                            if (t < nc) {
+                                const int j = UPPER ? jn - t : jn + t;
+                                cs[t] = a.scol[j];
+                                cv[t] = vals[j];
+                            }
+                        }
+                        jn = UPPER ? jn - nc : jn + nc;
+                        left -= nc;
+                        pc = 0;
+                    }
+                    if (pc < nc) {
+#pragma unroll
+                        for (int t = 0; t < DF_CHUNK; t++) {
+                            if (t == pc && t < nc) {
+                                const T v = ld_lds(&sl[cs[t]]);
+                                if (!Sentinel<T>::is(v)) {
+                                    s = UPPER ? fma(v, cv[t], s) : fma(cv[t], v, s);
+                                    pc++;
+                                }
+                            }
+                        }
+                        if (pc < nc && ++spins > a.spin_limit) {
+                            raise_status(a.status, BAAS_TIMEOUT, row);
+                            pc = nc;  // give up on this row's remaining parents
+                            left = 0;
+                        }
+                    } else {
+                        const T xr = (bv - s) / vals[dg];
+                        st_lds(&sl[key_of<KEYMODE>(row, a.n, a.key) - kb], xr);
+                        st_agent(&x[row], xr);
+                        if (rw < 0) {
+                            wait_stores();
+                            st_agent(&a.flags[row], epoch);
+                        }
+                        pending = false;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+    retire_block(a.ctx, epoch);
+}
+
 // Level-synchronous solve: one launch per row level (merge/SpTRSV.cpp:41-80).
 template <class T>
 __global__ void k_level_rows(const int *__restrict__ perm, int qb, int qe, const int *__restrict__ rowptr,
@@ -185,12 +322,59 @@ static int launch_sweep_k(baas_handle *h, const Plan &p, const SweepArgs &a, con
 
 size_t lds_limit(baas_handle *h);
 
+template <class T, int KM, bool UP>
+static int launch_df_t(baas_handle *h, const Plan &p, const DfArgs &a, const T *vals, const T *b, T *x, size_t lds,
+                       hipStream_t stream) {
+    auto kern = k_sweep_df<T, KM, UP>;
+    int per_cu = 0;
+    BAAS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, p.block, lds));
+    if (per_cu <= 0) return BAAS_UNSUPPORTED;
+    long long grid = (long long)per_cu * h->num_cus;
+    if (grid > p.nunits) grid = p.nunits;
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(p.block), lds, stream, a, vals, b, x);
+    BAAS_HIP_TRY(hipGetLastError());
+    return BAAS_OK;
+}
+
+template <class T, bool UP>
+static int launch_df(baas_handle *h, const Plan &p, const DfArgs &a, const T *vals, const T *b, T *x, size_t lds,
+                     hipStream_t st) {
+    switch (p.keymode) {
+        case 0: return launch_df_t<T, 0, UP>(h, p, a, vals, b, x, lds, st);
+        case 1: return launch_df_t<T, 1, UP>(h, p, a, vals, b, x, lds, st);
+        default: return launch_df_t<T, 2, UP>(h, p, a, vals, b, x, lds, st);
+    }
+}
+
 template <class T>
 int sweep(baas_handle *h, baas_schedule *s, int uplo, const T *vals, const T *b, T *x, int nrhs,
           hipStream_t stream, const int *skip) {
     if (!h || !s || !vals || !b || !x || nrhs < 1 || (uplo != 0 && uplo != 1)) return BAAS_INVALID_ARG;
     if ((const void *)b == (const void *)x) return BAAS_INVALID_ARG;
     const Plan &p = uplo ? s->bwd : s->fwd;
+    if (nrhs == 1 && p.dataflow && p.block >= 128 && (size_t)p.max_slots * sizeof(T) <= lds_limit(h)) {
+        DfArgs d;
+        d.n = s->n;
+        d.nunits = p.nunits;
+        d.npf = p.block >= 512 ? 2 : 1;
+        d.rowptr = uplo ? s->u_rowptr.p : s->l_rowptr.p;
+        d.scol = p.scol.p;
+        d.key_begin = p.key_begin.p;
+        d.unit_ptr = p.unit_ptr.p;
+        d.rows = p.rows.p;
+        d.key = p.key.p;
+        d.ext_ptr = p.ext_ptr.p;
+        d.ext_cols = p.ext_cols.p;
+        d.flags = s->flags.p;
+        d.ctx = Ctx{h->counters, h->counters + 1, h->counters + 2};
+        d.skip = skip;
+        d.status = h->status;
+        d.spin_limit = h->spin_limit;
+        size_t lds = (size_t)p.max_slots * sizeof(T);
+        return uplo ? launch_df<T, true>(h, p, d, vals, b, x, lds, stream)
+                    : launch_df<T, false>(h, p, d, vals, b, x, lds, stream);
+    }
     SweepArgs a;
     a.n = s->n;
     a.nunits = p.nunits;
@@ -285,5 +469,11 @@ extern "C" int baas_csr_outer_grad_f64(baas_handle *h, int m, const int *rp, con
     if (!h || m <= 0 || !rp || !ci || !lam || !x || !dv || nrhs < 1) return BAAS_INVALID_ARG;
     hipLaunchKernelGGL(k_csr_outer_grad, dim3(nblk(m)), dim3(256), 0, h->stream, m, nrhs, rp, ci, lam, x, dv);
     BAAS_HIP_TRY(hipGetLastError());
+    return BAAS_OK;
+}
+
+extern "C" int baas_plan_dataflow(baas_schedule *s, int enable) {
+    if (!s) return BAAS_INVALID_ARG;
+    s->fwd.dataflow = s->bwd.dataflow = enable ? 1 : 0;
     return BAAS_OK;
 }

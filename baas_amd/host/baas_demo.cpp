@@ -1,0 +1,261 @@
+// baas_demo.cpp -- driver with the reference mains' shape
+// (test/SpTRSV_level_merge.cpp:27-310, test/ic0_level_merge.cpp:36-139,
+// test/ilu0_level_merge.cpp:27-152): load or generate a matrix, compute the
+// serial answer, run the GPU runtime's evaluate(), check it, print one CSV
+// line "Matrix_Name,row,nnz,test_turns,Algorithm,Core,Scheduling_Time,
+// Executor_Runtime,nlevel" (test/SpTRSV_level_merge.cpp:62-81).
+//
+//   baas_demo trsv|ic0|ilu0|sched <grid-side|file.mtx> [nthread] [num_test]
+//
+// A numeric argument generates the 3D 7-point Laplacian of that side (ilu0:
+// the convection-diffusion operator); a path loads a Matrix Market file
+// (symmetric files are expanded; "trsv" uses the lower loader semantics of
+// spm/COO.cpp:554-622: strictly lower part plus a unit diagonal).
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <map>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "baas_runtime.hpp"
+
+namespace {
+
+struct Coo {
+    int n = 0;
+    std::vector<int> r, c;
+    std::vector<double> v;
+};
+
+SPM::CSR *to_csr(const Coo &a) {
+    std::vector<std::map<int, double>> rows(a.n);
+    for (size_t k = 0; k < a.r.size(); k++) rows[a.r[k]][a.c[k]] += a.v[k];
+    int nnz = 0;
+    for (auto &m : rows) nnz += (int)m.size();
+    SPM::CSR *A = new SPM::CSR(a.n, a.n, nnz);
+    int p = 0;
+    for (int i = 0; i < a.n; i++) {
+        A->rowptr[i] = p;
+        for (auto &kv : rows[i]) {
+            A->colidx[p] = kv.first;
+            A->values[p++] = kv.second;
+        }
+    }
+    A->rowptr[a.n] = p;
+    return A;
+}
+
+Coo grid7(int side, bool convdiff) {
+    Coo a;
+    a.n = side * side * side;
+    const double vel[3] = {0.5, 0.3, 0.2};
+    const double diag = convdiff ? (6.0 + 1.0) * 1.05 : 6.0;
+    for (int z = 0; z < side; z++)
+        for (int y = 0; y < side; y++)
+            for (int x = 0; x < side; x++) {
+                int i = x + side * (y + side * z);
+                a.r.push_back(i), a.c.push_back(i), a.v.push_back(diag);
+                int co[3] = {x, y, z}, st[3] = {1, side, side * side};
+                for (int d = 0; d < 3; d++) {
+                    if (co[d] > 0) a.r.push_back(i), a.c.push_back(i - st[d]), a.v.push_back(convdiff ? -1.0 - vel[d] : -1.0);
+                    if (co[d] < side - 1) a.r.push_back(i), a.c.push_back(i + st[d]), a.v.push_back(-1.0);
+                }
+            }
+    return a;
+}
+
+Coo read_mtx(const char *path, bool lower_unit_diag) {
+    std::ifstream f(path);
+    if (!f) {
+        fprintf(stderr, "cannot open %s\n", path);
+        exit(2);
+    }
+    std::string line;
+    std::getline(f, line);
+    bool sym = line.find("symmetric") != std::string::npos;
+    bool pattern = line.find("pattern") != std::string::npos;
+    while (std::getline(f, line) && line[0] == '%') {
+    }
+    std::istringstream hs(line);
+    int m, n, nz;
+    hs >> m >> n >> nz;
+    Coo a;
+    a.n = m;
+    for (int k = 0; k < nz; k++) {
+        int i, j;
+        double v = 1.0;
+        f >> i >> j;
+        if (!pattern) f >> v;
+        i--, j--;
+        if (lower_unit_diag) {
+            if (j < i || (sym && i < j)) {
+                int r = std::max(i, j), c = std::min(i, j);
+                a.r.push_back(r), a.c.push_back(c), a.v.push_back(v);
+            }
+            continue;
+        }
+        a.r.push_back(i), a.c.push_back(j), a.v.push_back(v);
+        if (sym && i != j) a.r.push_back(j), a.c.push_back(i), a.v.push_back(v);
+    }
+    if (lower_unit_diag)
+        for (int i = 0; i < m; i++) a.r.push_back(i), a.c.push_back(i), a.v.push_back(1.0);
+    return a;
+}
+
+SPM::CSR *lower_of(const SPM::CSR &A) {
+    int nnz = 0;
+    for (int i = 0; i < A.m; i++)
+        for (int j = A.rowptr[i]; j < A.rowptr[i + 1]; j++) nnz += A.colidx[j] <= i;
+    SPM::CSR *L = new SPM::CSR(A.m, A.n, nnz);
+    int p = 0;
+    for (int i = 0; i < A.m; i++) {
+        L->rowptr[i] = p;
+        for (int j = A.rowptr[i]; j < A.rowptr[i + 1]; j++)
+            if (A.colidx[j] <= i) L->colidx[p] = A.colidx[j], L->values[p++] = A.values[j];
+    }
+    L->rowptr[A.m] = p;
+    return L;
+}
+
+// serial references (merge/IC.cpp:32-89, merge/ILU.cpp:350-386)
+void ic0_serial(SPM::CSR &A, std::vector<double> &lu) {
+    A.constructDiagPtr();
+    lu.assign(A.values, A.values + A.getNnz());
+    for (int i = 0; i < A.m; i++) {
+        for (int k = A.rowptr[i]; k < A.diagptr[i]; k++) {
+            int j = A.colidx[k];
+            double dp = 0.0;
+            int l1 = A.rowptr[i], l2 = A.rowptr[j];
+            while (l1 < A.diagptr[i] && l2 < A.diagptr[j]) {
+                if (A.colidx[l1] == A.colidx[l2]) dp = std::fma(lu[l1++], lu[l2++], dp);
+                else if (A.colidx[l1] < A.colidx[l2]) l1++;
+                else l2++;
+            }
+            lu[k] = (A.values[k] - dp) / lu[A.diagptr[j]];
+        }
+        double dv = 0.0;
+        int len = A.diagptr[i] - A.rowptr[i], q = A.rowptr[i];
+        if (len > 3)
+            for (int stop = q + (len & ~3); q < stop; q++) dv = dv + lu[q] * lu[q];
+        for (; q < A.diagptr[i]; q++) dv = std::fma(lu[q], lu[q], dv);
+        double a = A.values[A.diagptr[i]];
+        if (a - dv > 0) lu[A.diagptr[i]] = std::sqrt(a - dv);
+    }
+}
+
+void ilu0_serial(SPM::CSR &A, std::vector<double> &lu) {
+    A.constructDiagPtr();
+    lu.assign(A.values, A.values + A.getNnz());
+    for (int i = 0; i < A.m; i++)
+        for (int j = A.rowptr[i]; j < A.diagptr[i]; j++) {
+            int col = A.colidx[j];
+            double tmp = lu[j] = lu[j] / lu[A.diagptr[col]];
+            int k1 = j + 1, k2 = A.diagptr[col] + 1;
+            while (k1 < A.rowptr[i + 1] && k2 < A.rowptr[col + 1]) {
+                if (A.colidx[k1] < A.colidx[k2]) k1++;
+                else if (A.colidx[k1] > A.colidx[k2]) k2++;
+                else lu[k1] = std::fma(-tmp, lu[k2], lu[k1]), k1++, k2++;
+            }
+        }
+}
+
+}  // namespace
+
+int main(int argc, char **argv) {
+    if (argc < 3) {
+        fprintf(stderr, "usage: %s trsv|ic0|ilu0|sched <grid-side|file.mtx> [nthread] [num_test]\n", argv[0]);
+        return 2;
+    }
+    std::string kind = argv[1], src = argv[2];
+    int nthread = argc > 3 ? atoi(argv[3]) : 8, num_test = argc > 4 ? atoi(argv[4]) : 20;
+    bool is_file = src.find(".mtx") != std::string::npos;
+    int side = is_file ? 0 : atoi(src.c_str());
+    try {
+        bool ok = false;
+        if (kind == "trsv") {
+            SPM::CSR *full = is_file ? nullptr : to_csr(grid7(side, false));
+            SPM::CSR *L = is_file ? to_csr(read_mtx(src.c_str(), true)) : lower_of(*full);
+            delete full;
+            int n = L->m, nnz = L->getNnz();
+            std::vector<double> correct(n, 1.0);  // alpha = 1 (test/SpTRSV_level_merge.cpp:55-56)
+            SpTRSVRT::SpTRSV_BAAS rt(L, correct.data(), n, nnz, 1.0, "BAAS_GPU", nthread);
+            rt.setNumTest(num_test);
+            double ms = rt.evaluate(&ok);
+            int c[8];
+            baas_schedule_info(rt.schedule(), c);
+            printf("%s,%d,%d,%d,%s,%d,%.4f,%.5f,%d\n", src.c_str(), n, nnz, num_test, "BAAS_GPU", nthread,
+                   rt.getAnalysisTime(), ms, c[1]);
+            delete L;
+        } else if (kind == "ic0" || kind == "ilu0") {
+            SPM::CSR *A = is_file ? to_csr(read_mtx(src.c_str(), false)) : to_csr(grid7(side, kind == "ilu0"));
+            std::vector<double> correct;
+            if (kind == "ic0") ic0_serial(*A, correct);
+            else ilu0_serial(*A, correct);
+            double ms;
+            double at;
+            if (kind == "ic0") {
+                ICRT::IC_BAAS rt(A, correct.data(), nthread);
+                ms = rt.evaluate(num_test, &ok);
+                at = rt.analysis_time;
+            } else {
+                ILURT::ILU_BAAS rt(A, correct.data(), nthread);
+                ms = rt.evaluate(num_test, &ok);
+                at = rt.analysis_time;
+            }
+            printf("%s,%d,%d,%d,%s,%d,%.4f,%.5f,%d\n", src.c_str(), A->m, A->getNnz(), num_test,
+                   kind == "ic0" ? "IC0_BAAS_GPU" : "ILU0_BAAS_GPU", nthread, at, ms, 0);
+            delete A;
+        } else if (kind == "sched") {
+            // ICCG's scheduler: IC0, then a forward and a backward solve with b = 1
+            SPM::CSR *A = is_file ? to_csr(read_mtx(src.c_str(), false)) : to_csr(grid7(side, false));
+            std::vector<double> correct;
+            ic0_serial(*A, correct);
+            Schedule::Scheduler sc(A, nthread);
+            sc.preprocessing();
+            std::vector<double> lu(A->getNnz());
+            sc.runIC0(A, lu.data());
+            ok = true;
+            for (int i = 0; i < A->m && ok; i++)
+                for (int j = A->rowptr[i]; j <= A->diagptr[i]; j++)
+                    if (std::fabs(lu[j] - correct[j]) > 1e-10) ok = false;
+            // L from lu, U = L^T; y = L^-1 b, z = U^-1 y; check L (L^T z) = b
+            SPM::CSR *Lc = lower_of(*A);
+            int p = 0;
+            for (int i = 0; i < A->m; i++)
+                for (int j = A->rowptr[i]; j <= A->diagptr[i]; j++) Lc->values[p++] = lu[j];
+            SPM::CSC Ct;
+            SPM::transpositionToCSC(Lc, &Ct);
+            SPM::CSR U(A->m, A->n, Ct.colptr, Ct.rowidx, Ct.values);
+            std::vector<double> b(A->m, 1.0), y(A->m), z(A->m), t(A->m, 0.0), r(A->m, 0.0);
+            sc.runForwardTRSVNoPerm(Lc, y.data(), b.data());
+            sc.runBackwardTRSVNoPerm(&U, z.data(), y.data());
+            for (int i = 0; i < A->m; i++)  // t = U z = L^T z
+                for (int j = U.rowptr[i]; j < U.rowptr[i + 1]; j++) t[i] += U.values[j] * z[U.colidx[j]];
+            double err = 0.0;
+            for (int i = 0; i < A->m; i++) {
+                for (int j = Lc->rowptr[i]; j < Lc->rowptr[i + 1]; j++) r[i] += Lc->values[j] * t[Lc->colidx[j]];
+                err = std::max(err, std::fabs(r[i] - 1.0));
+            }
+            ok = ok && err < 1e-9;
+            printf("%s,%d,%d,1,SCHEDULER_GPU,%d,0,0,%d (max |L L^T z - b| = %.3e)\n", src.c_str(), A->m,
+                   A->getNnz(), nthread, 0, err);
+            delete Lc;
+            delete A;
+        } else {
+            fprintf(stderr, "unknown kind %s\n", kind.c_str());
+            return 2;
+        }
+        if (!ok) {
+            fprintf(stderr, "CHECK FAILED\n");
+            return 1;
+        }
+    } catch (const std::exception &e) {
+        fprintf(stderr, "error: %s\n", e.what());
+        return 1;
+    }
+    return 0;
+}

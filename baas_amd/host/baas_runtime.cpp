@@ -1,0 +1,376 @@
+// baas_runtime.cpp -- the C++ runtime classes on top of the C-ABI.
+#include "baas_runtime.hpp"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+
+namespace {
+void hip_ok(hipError_t e, const char *what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+template <class T>
+T *dalloc(size_t n) {
+    T *p = nullptr;
+    hip_ok(hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)), "hipMalloc");
+    return p;
+}
+}  // namespace
+
+namespace SPM {
+
+CSR::CSR(int m_, int n_, int nnz) : m(m_), n(n_), ownData(true) {
+    rowptr = new int[m + 1]();
+    colidx = new int[std::max(nnz, 1)]();
+    values = new double[std::max(nnz, 1)]();
+}
+
+CSR::CSR(int m_, int n_, int *rp, int *ci, double *va) : m(m_), n(n_), rowptr(rp), colidx(ci), values(va) {}
+
+CSR::~CSR() {
+    if (ownData) {
+        delete[] rowptr;
+        delete[] colidx;
+        delete[] values;
+    }
+    delete[] diagptr;
+}
+
+void CSR::constructDiagPtr() {
+    if (diagptr) return;
+    diagptr = new int[m];
+    for (int i = 0; i < m; i++) {
+        diagptr[i] = rowptr[i + 1];
+        for (int j = rowptr[i]; j < rowptr[i + 1]; j++)
+            if (colidx[j] >= i) {
+                diagptr[i] = j;
+                break;
+            }
+    }
+}
+
+CSC::~CSC() {
+    delete[] colptr;
+    delete[] rowidx;
+    delete[] values;
+}
+
+void transpositionToCSC(const CSR *A, CSC *C) {
+    const int nnz = A->getNnz();
+    C->m = A->m;
+    C->n = A->n;
+    C->colptr = new int[A->n + 1]();
+    C->rowidx = new int[std::max(nnz, 1)];
+    C->values = new double[std::max(nnz, 1)];
+    for (int j = 0; j < nnz; j++) C->colptr[A->colidx[j] + 1]++;
+    for (int c = 0; c < A->n; c++) C->colptr[c + 1] += C->colptr[c];
+    std::vector<int> fill(C->colptr, C->colptr + A->n);
+    for (int i = 0; i < A->m; i++)
+        for (int j = A->rowptr[i]; j < A->rowptr[i + 1]; j++) {
+            int p = fill[A->colidx[j]]++;
+            C->rowidx[p] = i;
+            C->values[p] = A->values[j];
+        }
+}
+
+}  // namespace SPM
+
+namespace baas_rt {
+
+void check(int st, const char *what) {
+    if (st != BAAS_OK) throw std::runtime_error(std::string(what) + ": " + baas_status_string(st));
+}
+
+void DeviceCsr::upload(const SPM::CSR &A) {
+    release();
+    n = A.m;
+    nnz = A.getNnz();
+    rowptr = dalloc<int>(n + 1);
+    colidx = dalloc<int>(nnz);
+    values = dalloc<double>(nnz);
+    hip_ok(hipMemcpy(rowptr, A.rowptr, sizeof(int) * (n + 1), hipMemcpyHostToDevice), "upload rowptr");
+    hip_ok(hipMemcpy(colidx, A.colidx, sizeof(int) * nnz, hipMemcpyHostToDevice), "upload colidx");
+    hip_ok(hipMemcpy(values, A.values, sizeof(double) * nnz, hipMemcpyHostToDevice), "upload values");
+}
+
+void DeviceCsr::release() {
+    if (rowptr) (void)hipFree(rowptr);
+    if (colidx) (void)hipFree(colidx);
+    if (values) (void)hipFree(values);
+    rowptr = colidx = nullptr;
+    values = nullptr;
+}
+
+Engine::Engine() {
+    hip_ok(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "stream");
+    hip_ok(hipEventCreate(&e0), "event");
+    hip_ok(hipEventCreate(&e1), "event");
+    check(baas_create(&h, stream), "baas_create");
+}
+
+Engine::~Engine() {
+    if (h) baas_destroy(h);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (stream) (void)hipStreamDestroy(stream);
+}
+
+void Engine::start() { hip_ok(hipEventRecord(e0, stream), "event record"); }
+
+double Engine::stop_ms() {
+    hip_ok(hipEventRecord(e1, stream), "event record");
+    hip_ok(hipEventSynchronize(e1), "event sync");
+    float ms = 0;
+    hip_ok(hipEventElapsedTime(&ms, e0, e1), "elapsed");
+    return ms;
+}
+
+}  // namespace baas_rt
+
+namespace SpTRSVRT {
+
+FusionDemo::~FusionDemo() {
+    delete[] x;
+    delete[] y;
+}
+
+// b_i = sum_j L_ij * alpha, x := 0 (test/FusionDemo.cpp:47-65)
+void FusionDemo::setup() {
+    if (!x) x = new double[n];
+    if (!y) y = new double[n];
+    for (int i = 0; i < n; i++) {
+        double s = 0.0;
+        for (int j = csrA->rowptr[i]; j < csrA->rowptr[i + 1]; j++) s += csrA->values[j] * alpha;
+        y[i] = s;
+        x[i] = 0.0;
+    }
+}
+
+// |x - correct_x| <= 1e-10, first failure reported (test/FusionDemo.cpp:66-77)
+bool FusionDemo::testing() {
+    if (!correct_x) return true;
+    for (int i = 0; i < n; i++)
+        if (std::fabs(x[i] - correct_x[i]) > 1e-10) {
+            fprintf(stderr, "%s: x[%d] = %.17g, expected %.17g\n", algName.c_str(), i, x[i], correct_x[i]);
+            return false;
+        }
+    return true;
+}
+
+double FusionDemo::evaluate(bool *ok) {
+    auto t0 = std::chrono::steady_clock::now();
+    buildset();
+    analysis_time = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    setup();
+    double total = 0.0;
+    for (int t = 0; t < num_test; t++) total += fused_code();
+    bool good = testing();
+    if (ok) *ok = good;
+    return total / std::max(num_test, 1);
+}
+
+SpTRSV_BAAS::SpTRSV_BAAS(SPM::CSR *A, double *cx, int n_, int nnz_, double alpha_, std::string name,
+                         int num_thread_, int plan_mode)
+    : FusionDemo(n_, nnz_, std::move(name)) {
+    csrA = A;
+    correct_x = cx;
+    alpha = alpha_;
+    num_thread = num_thread_;
+    opts.ncore = num_thread_;
+    opts.plan_mode = plan_mode;
+}
+
+SpTRSV_BAAS::~SpTRSV_BAAS() {
+    if (sched) baas_schedule_destroy(sched);
+    if (d_b) (void)hipFree(d_b);
+    if (d_x) (void)hipFree(d_x);
+}
+
+void SpTRSV_BAAS::buildset() {
+    dL.upload(*csrA);
+    if (sched) baas_schedule_destroy(sched);
+    baas_rt::check(baas_analyze(eng.h, n, dL.rowptr, dL.colidx, BAAS_KIND_TRSV, &opts, &sched), "baas_analyze");
+}
+
+void SpTRSV_BAAS::setup() {
+    FusionDemo::setup();
+    if (!d_b) d_b = dalloc<double>(n);
+    if (!d_x) d_x = dalloc<double>(n);
+    hip_ok(hipMemcpy(d_b, y, sizeof(double) * n, hipMemcpyHostToDevice), "upload b");
+}
+
+double SpTRSV_BAAS::fused_code() {
+    eng.start();
+    baas_rt::check(baas_sptrsv_f64(eng.h, sched, 0, dL.values, d_b, d_x, 1), "baas_sptrsv_f64");
+    return eng.stop_ms();
+}
+
+bool SpTRSV_BAAS::testing() {
+    baas_rt::check(baas_check(eng.h), "sptrsv kernel");
+    hip_ok(hipMemcpy(x, d_x, sizeof(double) * n, hipMemcpyDeviceToHost), "download x");
+    return FusionDemo::testing();
+}
+
+}  // namespace SpTRSVRT
+
+namespace ICRT {
+
+IC_BAAS::IC_BAAS(SPM::CSR *A, double *cl, int nt) : csrA(A), correct_lu(cl), num_thread(nt) {}
+
+IC_BAAS::~IC_BAAS() {
+    if (sched) baas_schedule_destroy(sched);
+    if (d_lu) (void)hipFree(d_lu);
+}
+
+void IC_BAAS::buildset() {
+    auto t0 = std::chrono::steady_clock::now();
+    dA.upload(*csrA);
+    baas_opts o{};
+    o.ncore = num_thread;
+    if (sched) baas_schedule_destroy(sched);
+    baas_rt::check(baas_analyze(eng.h, csrA->m, dA.rowptr, dA.colidx, BAAS_KIND_IC0, &o, &sched), "baas_analyze");
+    if (!d_lu) d_lu = dalloc<double>(dA.nnz);
+    analysis_time = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// lu := A.values, then IC0 (test/ICRuntime.hpp:533-553)
+double IC_BAAS::fused_code() {
+    eng.start();
+    baas_rt::check(baas_ic0_f64(eng.h, sched, dA.values, d_lu), "baas_ic0_f64");
+    return eng.stop_ms();
+}
+
+// lower part incl. diagonal within 1e-10 (test/ICRuntime.hpp:95-124)
+bool IC_BAAS::testing() {
+    int st = baas_check(eng.h);
+    if (st != BAAS_OK && st != BAAS_NONPOSITIVE_PIVOT) baas_rt::check(st, "ic0 kernel");
+    lu.resize(dA.nnz);
+    hip_ok(hipMemcpy(lu.data(), d_lu, sizeof(double) * dA.nnz, hipMemcpyDeviceToHost), "download lu");
+    if (!correct_lu) return true;
+    csrA->constructDiagPtr();
+    for (int i = 0; i < csrA->m; i++)
+        for (int j = csrA->rowptr[i]; j <= csrA->diagptr[i]; j++)
+            if (std::fabs(lu[j] - correct_lu[j]) > 1e-10) {
+                fprintf(stderr, "IC0: lu[%d] = %.17g, expected %.17g\n", j, lu[j], correct_lu[j]);
+                return false;
+            }
+    return true;
+}
+
+double IC_BAAS::evaluate(int num_test, bool *ok) {
+    buildset();
+    double total = 0.0;
+    for (int t = 0; t < num_test; t++) total += fused_code();
+    bool good = testing();
+    if (ok) *ok = good;
+    return total / std::max(num_test, 1);
+}
+
+}  // namespace ICRT
+
+namespace ILURT {
+
+ILU_BAAS::ILU_BAAS(SPM::CSR *A, double *cl, int nt) : csrA(A), correct_lu(cl), num_thread(nt) {}
+
+ILU_BAAS::~ILU_BAAS() {
+    if (sched) baas_schedule_destroy(sched);
+    if (d_lu) (void)hipFree(d_lu);
+}
+
+void ILU_BAAS::buildset() {
+    auto t0 = std::chrono::steady_clock::now();
+    dA.upload(*csrA);
+    baas_opts o{};
+    o.ncore = num_thread;
+    if (sched) baas_schedule_destroy(sched);
+    baas_rt::check(baas_analyze(eng.h, csrA->m, dA.rowptr, dA.colidx, BAAS_KIND_ILU0, &o, &sched), "baas_analyze");
+    if (!d_lu) d_lu = dalloc<double>(dA.nnz);
+    analysis_time = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+double ILU_BAAS::fused_code() {
+    eng.start();
+    baas_rt::check(baas_ilu0_f64(eng.h, sched, dA.values, d_lu), "baas_ilu0_f64");
+    return eng.stop_ms();
+}
+
+// all entries within 1e-10 (test/ILUDemo.cpp:40-55)
+bool ILU_BAAS::testing() {
+    baas_rt::check(baas_check(eng.h), "ilu0 kernel");
+    lu.resize(dA.nnz);
+    hip_ok(hipMemcpy(lu.data(), d_lu, sizeof(double) * dA.nnz, hipMemcpyDeviceToHost), "download lu");
+    if (!correct_lu) return true;
+    for (int j = 0; j < dA.nnz; j++)
+        if (std::fabs(lu[j] - correct_lu[j]) > 1e-10) {
+            fprintf(stderr, "ILU0: lu[%d] = %.17g, expected %.17g\n", j, lu[j], correct_lu[j]);
+            return false;
+        }
+    return true;
+}
+
+double ILU_BAAS::evaluate(int num_test, bool *ok) {
+    buildset();
+    double total = 0.0;
+    for (int t = 0; t < num_test; t++) total += fused_code();
+    bool good = testing();
+    if (ok) *ok = good;
+    return total / std::max(num_test, 1);
+}
+
+}  // namespace ILURT
+
+namespace Schedule {
+
+Scheduler::Scheduler(const SPM::CSR *A, int nt) : csrA(A), num_thread(nt) {}
+
+Scheduler::~Scheduler() {
+    if (sched) baas_schedule_destroy(sched);
+    for (double *p : {d_lu, d_lv, d_uv, d_x, d_b})
+        if (p) (void)hipFree(p);
+}
+
+void Scheduler::preprocessing() {
+    dA.upload(*csrA);
+    baas_opts o{};
+    o.ncore = num_thread;
+    if (sched) baas_schedule_destroy(sched);
+    baas_rt::check(baas_analyze(eng.h, csrA->m, dA.rowptr, dA.colidx, BAAS_KIND_IC0, &o, &sched), "baas_analyze");
+    int c[8];
+    baas_schedule_info(sched, c);
+    nnzL = c[6];
+    d_lu = dalloc<double>(dA.nnz);
+    d_lv = dalloc<double>(nnzL);
+    d_uv = dalloc<double>(nnzL);
+    d_x = dalloc<double>(dA.n);
+    d_b = dalloc<double>(dA.n);
+}
+
+void Scheduler::runIC0(const SPM::CSR *mat, double *lu) {
+    hip_ok(hipMemcpy(dA.values, mat->values, sizeof(double) * dA.nnz, hipMemcpyHostToDevice), "upload A");
+    baas_rt::check(baas_ic0_f64(eng.h, sched, dA.values, d_lu), "baas_ic0_f64");
+    int st = baas_check(eng.h);
+    if (st != BAAS_OK && st != BAAS_NONPOSITIVE_PIVOT) baas_rt::check(st, "ic0 kernel");
+    hip_ok(hipMemcpy(lu, d_lu, sizeof(double) * dA.nnz, hipMemcpyDeviceToHost), "download lu");
+    baas_rt::check(baas_ic0_split_f64(eng.h, sched, d_lu, d_lv, d_uv), "split");
+}
+
+void Scheduler::runForwardTRSVNoPerm(const SPM::CSR *L, double *x, const double *b) {
+    hip_ok(hipMemcpy(d_lv, L->values, sizeof(double) * nnzL, hipMemcpyHostToDevice), "upload L");
+    hip_ok(hipMemcpy(d_b, b, sizeof(double) * dA.n, hipMemcpyHostToDevice), "upload b");
+    baas_rt::check(baas_sptrsv_f64(eng.h, sched, 0, d_lv, d_b, d_x, 1), "forward");
+    baas_rt::check(baas_check(eng.h), "forward kernel");
+    hip_ok(hipMemcpy(x, d_x, sizeof(double) * dA.n, hipMemcpyDeviceToHost), "download x");
+}
+
+void Scheduler::runBackwardTRSVNoPerm(const SPM::CSR *U, double *x, const double *b) {
+    hip_ok(hipMemcpy(d_uv, U->values, sizeof(double) * nnzL, hipMemcpyHostToDevice), "upload U");
+    hip_ok(hipMemcpy(d_b, b, sizeof(double) * dA.n, hipMemcpyHostToDevice), "upload b");
+    baas_rt::check(baas_sptrsv_f64(eng.h, sched, 1, d_uv, d_b, d_x, 1), "backward");
+    baas_rt::check(baas_check(eng.h), "backward kernel");
+    hip_ok(hipMemcpy(x, d_x, sizeof(double) * dA.n, hipMemcpyDeviceToHost), "download x");
+}
+
+}  // namespace Schedule

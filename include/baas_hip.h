@@ -197,6 +197,10 @@ int baas_batch_info(const baas_batch *b, double *times, int *rpu);
 /* Re-pack a schedule's execution plan (RANGE or SCHEDULE, rows per unit,
  * threads per workgroup: a multiple of 64 up to 1024). */
 int baas_replan(baas_schedule *s, int mode, int rows_per_unit, int block_threads);
+/* Single-RHS sweeps of this schedule use the dataflow kernel (per-row LDS
+ * waits, prefetch waves for parents of other units) when enable != 0 and the
+ * unit's slots fit in LDS. */
+int baas_plan_dataflow(baas_schedule *s, int enable);
 
 #ifdef __cplusplus
 }

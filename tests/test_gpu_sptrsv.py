@@ -61,7 +61,10 @@ def test_reduced_dag_matches_oracle(handle):
     np.testing.assert_array_equal(s.export("reduced_set"), rs)
 
 
-PLANS = [(1, 0, 256), (1, 1024, 256), (1, 16384, 512), (2, 0, 256), (2, 64, 64), (2, 1024, 1024)]
+# (plan mode, rows per unit, threads per workgroup, dataflow kernel)
+PLANS = [(1, 0, 256, False), (1, 1024, 256, False), (1, 16384, 512, False), (2, 0, 256, False),
+         (2, 64, 64, False), (2, 1024, 1024, False), (1, 4096, 256, True), (1, 12288, 512, True),
+         (1, 1024, 128, True), (2, 256, 256, True), (2, 2048, 512, True), (1, 8192, 1024, True)]
 SOLVE_CASES = ["C1_lap2d_128", "C2_lap3d_64", "C2_lap3d_64_perm1", "C5_diff3d_48", "bcsstk18", "bcsstk38"]
 
 
@@ -78,15 +81,15 @@ def test_forward_backward_bit_exact_all_plans(handle, name):
     lv, bd = up(L.values), up(b)
     uv = s.transpose_values(lv)
     np.testing.assert_array_equal(down(uv), U.values)
-    for mode, rpu, blk in PLANS:
-        announce("solve", name, mode, rpu, blk)
-        s.replan(mode, rpu, blk)
+    for mode, rpu, blk, df in PLANS:
+        announce("solve", name, mode, rpu, blk, df)
+        s.replan(mode, rpu, blk, dataflow=df)
         for rep in range(2):
             x = s.solve(lv, bd)
             y = s.solve(uv, bd, uplo=1)
             handle.check()
-            np.testing.assert_array_equal(down(x), x_ref, err_msg=f"fwd {name} {mode} {rpu} {blk} {rep}")
-            np.testing.assert_array_equal(down(y), y_ref, err_msg=f"bwd {name} {mode} {rpu} {blk} {rep}")
+            np.testing.assert_array_equal(down(x), x_ref, err_msg=f"fwd {name} {mode} {rpu} {blk} {df} {rep}")
+            np.testing.assert_array_equal(down(y), y_ref, err_msg=f"bwd {name} {mode} {rpu} {blk} {df} {rep}")
     torch.cuda.synchronize()
 
 
@@ -132,9 +135,11 @@ def test_f32_bit_exact(handle):
         L, _, _ = workload(name)
         s = Schedule(handle, L.rowptr, L.colidx, kind=0)
         b = g.rhs_uniform(L.n, 1)
-        x = down(s.solve(up(L.values, torch.float32), up(b, torch.float32)))
-        handle.check()
-        np.testing.assert_array_equal(x, O.sptrsv_lower_f32(L, b))
+        for df in (False, True):
+            s.replan(1, 4096, 256, dataflow=df)
+            x = down(s.solve(up(L.values, torch.float32), up(b, torch.float32)))
+            handle.check()
+            np.testing.assert_array_equal(x, O.sptrsv_lower_f32(L, b))
         U = g.Csr.from_scipy(L.to_scipy().T)
         uv32 = s.transpose_values(up(L.values)).to(torch.float32)
         y = down(s.solve(uv32, up(b, torch.float32), uplo=1))
