@@ -54,6 +54,7 @@ _SIGS = {
     "baas_schedule_upper_pattern": (_i, [_P, C.POINTER(_P), C.POINTER(_P)]),
     "baas_replan": (_i, [_P, _i, _i, _i]),
     "baas_plan_dataflow": (_i, [_P, _i]),
+    "baas_autotune": (_i, [_P, _P, _P, _i, _i, _P]),
     "baas_sptrsv_f64": (_i, [_P, _P, _i, _P, _P, _P, _i]),
     "baas_sptrsv_f32": (_i, [_P, _P, _i, _P, _P, _P, _i]),
     "baas_sptrsv_levelsync_f64": (_i, [_P, _P, _P, _P, _P]),

@@ -115,6 +115,12 @@ class Schedule:
         if dataflow is not None:
             L.call("baas_plan_dataflow", self._s, int(bool(dataflow)))
 
+    def autotune(self, vals, uplo=0, nrhs=1):
+        """Time candidate plans on these values and keep the fastest."""
+        c = (C.c_int * 5)()
+        L.call("baas_autotune", self.h.ptr, self._s, _ptr(vals), int(uplo), int(nrhs), C.cast(c, C.c_void_p))
+        return {"mode": c[0], "rows_per_unit": c[1], "block": c[2], "dataflow": c[3], "us": c[4]}
+
     def upper_pattern(self):
         rp, ci = C.c_void_p(), C.c_void_p()
         L.call("baas_schedule_upper_pattern", self._s, C.byref(rp), C.byref(ci))

@@ -141,6 +141,35 @@ int make_plan(Plan &p, int n, const std::vector<int> &order, const std::vector<i
         max_slots = std::max(max_slots, nk + (int)ext_cols.size() - ext_ptr[u]);
     }
     ext_ptr[nunits] = (int)ext_cols.size();
+    // unit DAG: producer units of each unit
+    std::vector<int> udep_ptr(nunits + 1, 0), udeps, mark(nunits, -1);
+    for (int u = 0; u < nunits; u++) {
+        udep_ptr[u] = (int)udeps.size();
+        for (int q = unit_ptr[u]; q < unit_ptr[u + 1]; q++) {
+            const int r = rows[q] & 0x7fffffff;
+            for (int j = dptr[r]; j < dptr[r + 1]; j++) {
+                const int v = unit_of[didx[j]];
+                if (v != u && mark[v] != u) {
+                    mark[v] = u;
+                    udeps.push_back(v);
+                }
+            }
+        }
+        std::sort(udeps.begin() + udep_ptr[u], udeps.end());
+    }
+    udep_ptr[nunits] = (int)udeps.size();
+    BAAS_TRY(upload(p.udep_ptr, udep_ptr, st));
+    BAAS_TRY(upload(p.udeps, udeps, st));
+    BAAS_TRY(p.pcount.alloc(1));
+    if (hipMemsetAsync(p.pcount.p, 0, sizeof(int), st) != hipSuccess) return BAAS_HIP_ERROR;
+    p.xp_raw.free();
+    p.xp_type = 0;
+    BAAS_TRY(p.uflags.alloc(nunits));
+    if (hipMemsetAsync(p.uflags.p, 0, sizeof(int) * nunits, st) != hipSuccess) return BAAS_HIP_ERROR;
+    // Unit-level waiting is switched off: its first GPU run went silent in the
+    // forward/backward solve test and no cause has been shown yet; plans wait
+    // per row (the protocol that has passed every run).
+    p.unitdep = 0;
     BAAS_TRY(upload(p.scol, scol, st));
     BAAS_TRY(upload(p.ext_ptr, ext_ptr, st));
     BAAS_TRY(upload(p.ext_cols, ext_cols, st));

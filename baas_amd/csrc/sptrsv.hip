@@ -28,9 +28,11 @@ struct SweepArgs {
     const int *skip;  // optional device flag: nonzero = return at once (converged PCG)
     DevStatus *status;
     long long spin_limit;
+    const int *udep_ptr, *udeps;  // unit-level dependencies (UNITDEP kernels)
+    int *uflags;
 };
 
-template <class T, int KEYMODE, bool UPPER, bool LDS>
+template <class T, int KEYMODE, bool UPPER, bool LDS, bool UNITDEP>
 __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a, const T *__restrict__ vals,
                                                const T *__restrict__ b, T *x) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -44,6 +46,12 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a, const T *__restrict
         if (u >= a.nunits) break;
         const int kb = a.key_begin[u];
         const int s0 = a.sub_begin[u], s1 = a.sub_begin[u + 1];
+        if (UNITDEP) {
+            // one poll per producer unit, then every external parent is final
+            for (int d = a.udep_ptr[u] + threadIdx.x; d < a.udep_ptr[u + 1]; d += blockDim.x)
+                wait_ready(a.uflags, a.udeps[d], epoch, a.spin_limit, a.status);
+            __syncthreads();
+        }
         for (int si = s0; si < s1; si++) {
             const int qb = a.sub_ptr[si], qe = a.sub_ptr[si + 1];
             const int items = (qe - qb) * nrhs;
@@ -56,10 +64,12 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a, const T *__restrict
                 const int ob = UPPER ? rb + 1 : rb, oe = UPPER ? re : re - 1;  // off-diagonals
                 const int dg = UPPER ? rb : re - 1;
                 // wait for parents produced by other units
-                for (int j = ob; j < oe; j++) {
-                    const int c = a.colidx[j];
-                    if (key_of<KEYMODE>(c, a.n, a.key) < kb) wait_ready(a.flags, c, epoch, a.spin_limit, a.status);
-                }
+                if (!UNITDEP)
+                    for (int j = ob; j < oe; j++) {
+                        const int c = a.colidx[j];
+                        if (key_of<KEYMODE>(c, a.n, a.key) < kb)
+                            wait_ready(a.flags, c, epoch, a.spin_limit, a.status);
+                    }
                 T s = T(0);
                 if (UPPER) {
                     for (int j = oe - 1; j >= ob; j--) {
@@ -83,12 +93,15 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a, const T *__restrict
                 const T xr = (b[(size_t)row * nrhs + r] - s) / vals[dg];
                 st_agent(&x[(size_t)row * nrhs + r], xr);
                 if (LDS) xs[(size_t)(key_of<KEYMODE>(row, a.n, a.key) - kb) * nrhs + r] = xr;
-                if (nrhs == 1 && rw < 0) {
+                if (!UNITDEP && nrhs == 1 && rw < 0) {
                     wait_stores();
                     st_agent(&a.flags[row], epoch);
                 }
             }
-            if (nrhs > 1) {
+            if (UNITDEP) {
+                wait_stores();
+                __syncthreads();
+            } else if (nrhs > 1) {
                 wait_stores();
                 __syncthreads();
                 for (int t = threadIdx.x; t < qe - qb; t += blockDim.x) {
@@ -102,6 +115,7 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a, const T *__restrict
                 __syncthreads();
             }
         }
+        if (UNITDEP && threadIdx.x == 0) st_agent(&a.uflags[u], epoch);
     }
     retire_block(a.ctx, epoch);
 }
@@ -110,12 +124,18 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a, const T *__restrict
 // Dataflow sweep.  One lane owns one row; the unit's solved values live in
 // LDS slots pre-filled with a signalling-NaN sentinel, so a lane reads its
 // parent's slot until the sentinel is gone -- no sub-level barriers, every
-// row waits exactly for its own parents.  Parents in earlier units are staged
-// into LDS slots (after the unit's own keys) by dedicated prefetch waves that
-// poll the per-row ready flags, so the flag round trips overlap with compute.
-// Each lane retries in one loop that consumes every ready entry in ascending
-// (descending for U) column order, so lanes of one wavefront that depend on
-// each other still make progress.
+// row waits exactly for its own parents.  Each lane retries in one loop that
+// consumes every ready entry in ascending (descending for U) column order,
+// so lanes of one wavefront that depend on each other still make progress.
+//
+// Rows read by later units publish their value itself: one atomic 64-bit
+// (32-bit for FP32) store into a plan-owned array of two halves pre-filled
+// with the sentinel.  A sweep publishes into the half selected by the plan's
+// sweep counter and re-arms the same rows in the other half for the next
+// sweep, so a consumer needs one load per parent (the value is the flag) and
+// a producer never waits for its stores.  Dedicated prefetch waves stage
+// those parents into LDS while the compute waves work, and every compute
+// lane loads its next row's data before it starts waiting on the current one.
 template <class T> struct Sentinel;
 template <> struct Sentinel<double> {
     static __device__ __forceinline__ double value() { return __longlong_as_double(0x7ff4deadbeef0001LL); }
@@ -139,7 +159,7 @@ struct DfArgs {
     int n, nunits, npf;
     const int *rowptr, *scol;
     const int *key_begin, *unit_ptr, *rows, *key, *ext_ptr, *ext_cols;
-    int *flags;
+    int *pcount;  // the plan's sweep counter (parity selects the publication half)
     Ctx ctx;
     const int *skip;
     DevStatus *status;
@@ -148,14 +168,53 @@ struct DfArgs {
 
 constexpr int DF_CHUNK = 8;
 
+// one row's data, loaded ahead of its turn
+template <class T>
+struct DfRow {
+    int rw, row, dg, left, jn, nc;
+    T bv;
+    int cs[DF_CHUNK];
+    T cv[DF_CHUNK];
+};
+
+template <class T, bool UPPER>
+__device__ __forceinline__ void df_load_row(const DfArgs &a, const T *__restrict__ vals, const T *__restrict__ b,
+                                            int q, int q1, DfRow<T> &R) {
+    R.nc = 0;
+    R.left = 0;
+    R.rw = -1;
+    if (q >= q1) return;
+    R.rw = a.rows[q];
+    R.row = R.rw & 0x7fffffff;
+    const int rb = a.rowptr[R.row], re = a.rowptr[R.row + 1];
+    R.dg = UPPER ? rb : re - 1;
+    R.left = re - rb - 1;
+    R.jn = UPPER ? re - 1 : rb;
+    R.bv = b[R.row];
+    R.nc = R.left < DF_CHUNK ? R.left : DF_CHUNK;
+#pragma unroll
+    for (int t = 0; t < DF_CHUNK; t++) {
+        if (t < R.nc) {
+            const int j = UPPER ? R.jn - t : R.jn + t;
+            R.cs[t] = a.scol[j];
+            R.cv[t] = vals[j];
+        }
+    }
+    R.jn = UPPER ? R.jn - R.nc : R.jn + R.nc;
+    R.left -= R.nc;
+}
+
 template <class T, int KEYMODE, bool UPPER>
 __global__ __launch_bounds__(1024) void k_sweep_df(DfArgs a, const T *__restrict__ vals, const T *__restrict__ b,
-                                                  T *x) {
+                                                  T *x, T *xp) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *sl = reinterpret_cast<T *>(smem_raw);
     __shared__ int s_unit;
     if (a.skip && ld_agent(a.skip)) return;
     const int epoch = read_epoch(a.ctx);
+    const int par = ld_agent(a.pcount) & 1;
+    T *pub = xp + (size_t)par * a.n;
+    T *nxt = xp + (size_t)(par ^ 1) * a.n;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
     const int npf = a.npf;
     for (;;) {
@@ -167,80 +226,97 @@ __global__ __launch_bounds__(1024) void k_sweep_df(DfArgs a, const T *__restrict
         for (int i = threadIdx.x; i < nslots; i += blockDim.x) st_lds(&sl[i], Sentinel<T>::value());
         __syncthreads();
         if (wv < npf) {
-            // prefetch waves: stage external parents in order of first use
+            // prefetch waves: stage parents of other units in order of first use
             for (int e = e0 + wv * 64 + lane; e < e1; e += npf * 64) {
                 const int c = a.ext_cols[e];
-                wait_ready(a.flags, c, epoch, a.spin_limit, a.status);
-                st_lds(&sl[nk + (e - e0)], ld_agent(&x[c]));
+                T v = ld_agent(&pub[c]);
+                long long spins = 0;
+                while (Sentinel<T>::is(v)) {
+                    if (++spins > a.spin_limit) {
+                        raise_status(a.status, BAAS_TIMEOUT, c);
+                        v = T(0);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(1);
+                    v = ld_agent(&pub[c]);
+                }
+                st_lds(&sl[nk + (e - e0)], v);
             }
         } else {
             const int q0 = a.unit_ptr[u], q1 = a.unit_ptr[u + 1];
             const int C = nw - npf, cw = wv - npf;
+            DfRow<T> cur, nxr;
+            df_load_row<T, UPPER>(a, vals, b, q0 + cw * 64 + lane, q1, cur);
             for (int base = q0 + cw * 64; base < q1; base += C * 64) {
-                const int q = base + lane;
-                bool pending = q < q1;
-                int rw = 0, row = 0, dg = 0, left = 0, jn = 0;
-                T bv = T(0), s = T(0);
-                if (pending) {
-                    rw = a.rows[q];
-                    row = rw & 0x7fffffff;
-                    const int rb = a.rowptr[row], re = a.rowptr[row + 1];
-                    dg = UPPER ? rb : re - 1;
-                    left = re - rb - 1;               // off-diagonal entries still to consume
-                    jn = UPPER ? re - 1 : rb;         // next entry to load
-                    bv = b[row];
-                }
-                int cs[DF_CHUNK];
-                T cv[DF_CHUNK];
-                int nc = 0, pc = 0;
+                df_load_row<T, UPPER>(a, vals, b, base + C * 64 + lane, q1, nxr);
+                bool pending = base + lane < q1;
+                T s = T(0);
+                int pc = 0;
                 long long spins = 0;
                 while (pending) {
-                    if (pc == nc && left > 0) {  // load the next chunk of entries
-                        nc = left < DF_CHUNK ? left : DF_CHUNK;
+                    if (pc == cur.nc && cur.left > 0) {  // rows longer than one chunk
+                        cur.nc = cur.left < DF_CHUNK ? cur.left : DF_CHUNK;
 #pragma unroll
                         for (int t = 0; t < DF_CHUNK; t++) {
-                            if (t < nc) {
-                                const int j = UPPER ? jn - t : jn + t;
-                                cs[t] = a.scol[j];
-                                cv[t] = vals[j];
+                            if (t < cur.nc) {
+                                const int j = UPPER ? cur.jn - t : cur.jn + t;
+                                cur.cs[t] = a.scol[j];
+                                cur.cv[t] = vals[j];
                             }
                         }
-                        jn = UPPER ? jn - nc : jn + nc;
-                        left -= nc;
+                        cur.jn = UPPER ? cur.jn - cur.nc : cur.jn + cur.nc;
+                        cur.left -= cur.nc;
                         pc = 0;
                     }
-                    if (pc < nc) {
+                    if (pc < cur.nc) {
 #pragma unroll
                         for (int t = 0; t < DF_CHUNK; t++) {
-                            if (t == pc && t < nc) {
-                                const T v = ld_lds(&sl[cs[t]]);
+                            if (t == pc && t < cur.nc) {
+                                const T v = ld_lds(&sl[cur.cs[t]]);
                                 if (!Sentinel<T>::is(v)) {
-                                    s = UPPER ? fma(v, cv[t], s) : fma(cv[t], v, s);
+                                    s = UPPER ? fma(v, cur.cv[t], s) : fma(cur.cv[t], v, s);
                                     pc++;
                                 }
                             }
                         }
-                        if (pc < nc && ++spins > a.spin_limit) {
-                            raise_status(a.status, BAAS_TIMEOUT, row);
-                            pc = nc;  // give up on this row's remaining parents
-                            left = 0;
+                        if (pc < cur.nc && ++spins > a.spin_limit) {
+                            raise_status(a.status, BAAS_TIMEOUT, cur.row);
+                            pc = cur.nc;  // give up on this row's remaining parents
+                            cur.left = 0;
                         }
                     } else {
-                        const T xr = (bv - s) / vals[dg];
-                        st_lds(&sl[key_of<KEYMODE>(row, a.n, a.key) - kb], xr);
-                        st_agent(&x[row], xr);
-                        if (rw < 0) {
-                            wait_stores();
-                            st_agent(&a.flags[row], epoch);
+                        const T xr = (cur.bv - s) / vals[cur.dg];
+                        st_lds(&sl[key_of<KEYMODE>(cur.row, a.n, a.key) - kb], xr);
+                        x[cur.row] = xr;
+                        if (cur.rw < 0) {
+                            st_agent(&pub[cur.row], xr);
+                            nxt[cur.row] = Sentinel<T>::value();
                         }
                         pending = false;
                     }
                 }
+                cur = nxr;
             }
         }
         __syncthreads();
     }
-    retire_block(a.ctx, epoch);
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const int d = atomicAdd(a.ctx.done, 1);
+        if (d == (int)gridDim.x - 1) {
+            atomicExch(a.ctx.ticket, 0);
+            atomicExch(a.ctx.epoch, epoch == 0x7fffffff ? 1 : epoch + 1);
+            atomicAdd(a.pcount, 1);
+            __threadfence();
+            atomicExch(a.ctx.done, 0);
+        }
+    }
+}
+
+template <class T>
+__global__ void k_fill_sentinel(size_t n, T *p) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = Sentinel<T>::value();
 }
 
 // Level-synchronous solve: one launch per row level (merge/SpTRSV.cpp:41-80).
@@ -298,7 +374,7 @@ static inline int nblk(long long n, int b = 256) { return (int)((n + b - 1) / b)
 template <class T, int KM, bool UP, bool LDS>
 static int launch_sweep_t(baas_handle *h, const Plan &p, const SweepArgs &a, const T *vals, const T *b, T *x,
                           size_t lds, hipStream_t stream) {
-    auto kern = k_sweep<T, KM, UP, LDS>;
+    auto kern = p.unitdep ? k_sweep<T, KM, UP, LDS, true> : k_sweep<T, KM, UP, LDS, false>;
     int per_cu = 0;
     BAAS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, p.block, lds));
     if (per_cu <= 0) return BAAS_UNSUPPORTED;
@@ -323,8 +399,8 @@ static int launch_sweep_k(baas_handle *h, const Plan &p, const SweepArgs &a, con
 size_t lds_limit(baas_handle *h);
 
 template <class T, int KM, bool UP>
-static int launch_df_t(baas_handle *h, const Plan &p, const DfArgs &a, const T *vals, const T *b, T *x, size_t lds,
-                       hipStream_t stream) {
+static int launch_df_t(baas_handle *h, const Plan &p, const DfArgs &a, const T *vals, const T *b, T *x, T *xp,
+                       size_t lds, hipStream_t stream) {
     auto kern = k_sweep_df<T, KM, UP>;
     int per_cu = 0;
     BAAS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, p.block, lds));
@@ -332,18 +408,18 @@ static int launch_df_t(baas_handle *h, const Plan &p, const DfArgs &a, const T *
     long long grid = (long long)per_cu * h->num_cus;
     if (grid > p.nunits) grid = p.nunits;
     if (grid < 1) grid = 1;
-    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(p.block), lds, stream, a, vals, b, x);
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(p.block), lds, stream, a, vals, b, x, xp);
     BAAS_HIP_TRY(hipGetLastError());
     return BAAS_OK;
 }
 
 template <class T, bool UP>
-static int launch_df(baas_handle *h, const Plan &p, const DfArgs &a, const T *vals, const T *b, T *x, size_t lds,
-                     hipStream_t st) {
+static int launch_df(baas_handle *h, const Plan &p, const DfArgs &a, const T *vals, const T *b, T *x, T *xp,
+                     size_t lds, hipStream_t st) {
     switch (p.keymode) {
-        case 0: return launch_df_t<T, 0, UP>(h, p, a, vals, b, x, lds, st);
-        case 1: return launch_df_t<T, 1, UP>(h, p, a, vals, b, x, lds, st);
-        default: return launch_df_t<T, 2, UP>(h, p, a, vals, b, x, lds, st);
+        case 0: return launch_df_t<T, 0, UP>(h, p, a, vals, b, x, xp, lds, st);
+        case 1: return launch_df_t<T, 1, UP>(h, p, a, vals, b, x, xp, lds, st);
+        default: return launch_df_t<T, 2, UP>(h, p, a, vals, b, x, xp, lds, st);
     }
 }
 
@@ -352,9 +428,19 @@ int sweep(baas_handle *h, baas_schedule *s, int uplo, const T *vals, const T *b,
           hipStream_t stream, const int *skip) {
     if (!h || !s || !vals || !b || !x || nrhs < 1 || (uplo != 0 && uplo != 1)) return BAAS_INVALID_ARG;
     if ((const void *)b == (const void *)x) return BAAS_INVALID_ARG;
-    const Plan &p = uplo ? s->bwd : s->fwd;
+    Plan &p = uplo ? s->bwd : s->fwd;
     if (nrhs == 1 && p.dataflow && p.block >= 128 && (size_t)p.max_slots * sizeof(T) <= lds_limit(h)) {
+        // publication halves, armed with this value type's sentinel
+        if (p.xp_type != (int)sizeof(T)) {
+            const size_t cnt = 2 * (size_t)s->n;
+            BAAS_TRY(p.xp_raw.alloc(cnt * sizeof(T)));
+            hipLaunchKernelGGL(k_fill_sentinel<T>, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, stream, cnt,
+                               reinterpret_cast<T *>(p.xp_raw.p));
+            BAAS_HIP_TRY(hipGetLastError());
+            p.xp_type = (int)sizeof(T);
+        }
         DfArgs d;
+        d.pcount = p.pcount.p;
         d.n = s->n;
         d.nunits = p.nunits;
         d.npf = p.block >= 512 ? 2 : 1;
@@ -366,14 +452,14 @@ int sweep(baas_handle *h, baas_schedule *s, int uplo, const T *vals, const T *b,
         d.key = p.key.p;
         d.ext_ptr = p.ext_ptr.p;
         d.ext_cols = p.ext_cols.p;
-        d.flags = s->flags.p;
         d.ctx = Ctx{h->counters, h->counters + 1, h->counters + 2};
         d.skip = skip;
         d.status = h->status;
         d.spin_limit = h->spin_limit;
         size_t lds = (size_t)p.max_slots * sizeof(T);
-        return uplo ? launch_df<T, true>(h, p, d, vals, b, x, lds, stream)
-                    : launch_df<T, false>(h, p, d, vals, b, x, lds, stream);
+        T *xp = reinterpret_cast<T *>(p.xp_raw.p);
+        return uplo ? launch_df<T, true>(h, p, d, vals, b, x, xp, lds, stream)
+                    : launch_df<T, false>(h, p, d, vals, b, x, xp, lds, stream);
     }
     SweepArgs a;
     a.n = s->n;
@@ -391,6 +477,9 @@ int sweep(baas_handle *h, baas_schedule *s, int uplo, const T *vals, const T *b,
     a.skip = skip;
     a.status = h->status;
     a.spin_limit = h->spin_limit;
+    a.udep_ptr = p.udep_ptr.p;
+    a.udeps = p.udeps.p;
+    a.uflags = p.uflags.p;
     size_t lds = (size_t)p.max_unit_keys * nrhs * sizeof(T);
     bool use_lds = lds <= lds_limit(h);
     if (uplo)

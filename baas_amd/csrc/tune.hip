@@ -1,0 +1,93 @@
+// tune.hip -- the execution-plan autotuner.
+//
+// The reference fixes its level-merge and threads-per-level thresholds at
+// compile time (merge/LevelMerge.cpp:118-161, merge/include/MergeMarco.hpp).
+// Here the packing is chosen per matrix by timing real sweeps: RANGE plans
+// over several rows-per-unit and workgroup sizes, with and without the
+// dataflow kernel, and SCHEDULE plans (units cut from the merged levels,
+// waiting per unit) over several rows-per-unit.  The fastest plan stays on
+// the schedule.  Values need not be the final ones: any non-zero diagonal
+// gives the same timing.
+#include <cstdio>
+#include <vector>
+
+#include "blas.h"
+#include "internal.h"
+
+namespace baas {
+int build_plans_mode(baas_schedule *s, int mode, int rows_per_unit);
+size_t lds_limit(baas_handle *h);
+}  // namespace baas
+
+using namespace baas;
+
+namespace {
+
+struct Cand {
+    int mode, rpu, block, df;
+};
+
+__global__ void k_fill_d(int n, double *p, double v) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+}  // namespace
+
+extern "C" int baas_autotune(baas_handle *h, baas_schedule *s, const double *vals, int uplo, int nrhs, int *chosen) {
+    if (!h || !s || !vals || nrhs < 1 || (uplo != 0 && uplo != 1)) return BAAS_INVALID_ARG;
+    const int n = s->n;
+    DBuf<double> b, x;
+    BAAS_TRY(b.alloc((size_t)n * nrhs));
+    BAAS_TRY(x.alloc((size_t)n * nrhs));
+    hipLaunchKernelGGL(k_fill_d, dim3((n * nrhs + 255) / 256), dim3(256), 0, h->stream, n * nrhs, b.p, 1.0);
+    std::vector<Cand> cands;
+    for (int rpu : {1024, 2048, 4096, 8192, 16384})
+        for (int blk : {256, 512, 1024}) {
+            if (blk == 1024 && rpu < 4096) continue;
+            cands.push_back({BAAS_PLAN_RANGE, rpu, blk, 0});
+            if (nrhs == 1) cands.push_back({BAAS_PLAN_RANGE, rpu, blk, 1});
+        }
+    for (int rpu : {256, 1024, 4096})
+        for (int blk : {256, 1024}) cands.push_back({BAAS_PLAN_SCHEDULE, rpu, blk, 0});
+    hipEvent_t e0, e1;
+    BAAS_HIP_TRY(hipEventCreate(&e0));
+    BAAS_HIP_TRY(hipEventCreate(&e1));
+    float best = 1e30f;
+    Cand win = cands[0];
+    for (const Cand &c : cands) {
+        if (c.rpu >= 2 * n && c.mode == BAAS_PLAN_RANGE && c.rpu != 1024) continue;
+        s->opts.block_threads = c.block;
+        if (build_plans_mode(s, c.mode, c.rpu) != BAAS_OK) continue;
+        Plan &p = uplo ? s->bwd : s->fwd;
+        if (c.df && (size_t)p.max_slots * sizeof(double) > lds_limit(h)) continue;
+        s->fwd.dataflow = s->bwd.dataflow = c.df;
+        float t_best = 1e30f;
+        bool ok = true;
+        for (int rep = 0; rep < 4 && ok; rep++) {
+            (void)hipEventRecord(e0, h->stream);
+            ok = sweep<double>(h, s, uplo, vals, b.p, x.p, nrhs, h->stream, nullptr) == BAAS_OK;
+            (void)hipEventRecord(e1, h->stream);
+            (void)hipEventSynchronize(e1);
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (rep > 0 && ms < t_best) t_best = ms;
+        }
+        if (ok && t_best < best) best = t_best, win = c;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    s->opts.block_threads = win.block;
+    s->opts.plan_mode = win.mode;
+    s->opts.rows_per_unit = win.rpu;
+    BAAS_TRY(build_plans_mode(s, win.mode, win.rpu));
+    s->fwd.dataflow = s->bwd.dataflow = win.df;
+    if (chosen) {
+        chosen[0] = win.mode;
+        chosen[1] = win.rpu;
+        chosen[2] = win.block;
+        chosen[3] = win.df;
+        chosen[4] = (int)(best * 1000.0f);  // microseconds of the winning sweep
+    }
+    return baas_check(h) == BAAS_OK ? BAAS_OK : BAAS_HIP_ERROR;
+}

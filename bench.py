@@ -1,15 +1,20 @@
 """Benchmark: FP64 SpTRSV on the 64^3 7-point Laplacian (one MI355X).
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--quick]
 
-A step is one forward triangular solve x = L^-1 b of the lower triangle of
+A step is one forward triangular solve x = L^-1 b with the lower triangle L of
 the 3D 7-point Laplacian on a 64^3 grid in natural ordering (n = 262144,
-nnz(L) = 1036288), single right-hand side b ~ U(-1,1) (seed 0), inputs
+nnz(L) = 1036288), one right-hand side b ~ U(-1,1) (seed 0), every input
 resident in HBM.  value = algorithmic GB/s = (12 nnz(L) + 20 n) bytes per
-solve / time per solve (the reference's own byte model,
-test/iccg_level_merge.cpp:354-355).  The JSON line also carries the roofline
-of the solve kernel, the CPU baseline, the random-permutation ordering, the
-32-RHS rate, IC0/ILU0 times and the ICCG FP64-vs-mixed time to 1e-8.
+solve / median time per solve (the reference's byte model,
+test/iccg_level_merge.cpp:354-355).  The execution plan is autotuned before
+the timed region (analysis time, reported separately, as the reference's
+Scheduling_Time).
+
+Further fields (not the headline): the same solve on the seed-1 random
+symmetric permutation, 32 right-hand sides, IC0 and ILU0 (64^3), and ICCG
+time to ||r||/||r0|| < 1e-8 on the kNN stiffness matrix (n = 36000) with
+the FP64 factor and with the FP32 factor.
 """
 from __future__ import annotations
 
@@ -25,44 +30,55 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "SpTRSV GB/s (% HBM peak); ICCG time-to-1e-8, FP32-precond vs FP64"
 
 
-def algorithmic_bytes_sptrsv(n, nnz, k=1, vbytes=8):
-    # values + colidx per nonzero, rowptr per row, b read + x written per row and RHS
+def bytes_sptrsv(n, nnz, k=1, vbytes=8):
+    """values + colidx per nonzero, rowptr per row, b read + x written per row and RHS."""
     return (vbytes + 4) * nnz + 4 * n + 2 * vbytes * n * k
 
 
-def time_events(fn, steps, warmup, stream=None):
+def time_events(fn, steps, warmup):
+    """Median and mean ms per call, HIP events on torch's current stream (the
+    stream every library call of this process is issued on)."""
     import torch
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
-    st = stream or torch.cuda.current_stream()
+    st = torch.cuda.current_stream()
     evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(steps)]
-    t0 = time.perf_counter()
     for a, b in evs:
         a.record(st)
         fn()
         b.record(st)
     torch.cuda.synchronize()
-    wall = time.perf_counter() - t0
     ms = [a.elapsed_time(b) for a, b in evs]
-    return float(np.median(ms)), float(np.mean(ms)), wall
+    return float(np.median(ms)), float(np.mean(ms))
 
 
-def cpu_baseline(L, b, seconds=10.0):
-    """The oracle's serial solve timed on the host (one core)."""
+def cpu_baseline(L, b, seconds):
+    """The oracle's serial forward solve, timed on one host core."""
     import oracle as O
+    O.sptrsv_lower(L, b)
     t0 = time.perf_counter()
     reps = 0
     while time.perf_counter() - t0 < seconds:
         O.sptrsv_lower(L, b)
         reps += 1
     dt = (time.perf_counter() - t0) / reps
-    by = algorithmic_bytes_sptrsv(L.n, L.nnz)
-    return {"value": by / dt / 1e9, "unit": "GB/s", "cores": 1, "kind": "port",
-            "sample": f"{reps} serial FP64 solves of the 64^3 L (oracle.c orc_sptrsv_lower), "
-                      f"{dt * 1e3:.3f} ms per solve"}
+    return {"value": round(bytes_sptrsv(L.n, L.nnz) / dt / 1e9, 3), "unit": "GB/s", "cores": 1, "kind": "port",
+            "sample": f"{reps} serial FP64 forward solves of the 64^3 L (oracle/oracle.c orc_sptrsv_lower, "
+                      f"ascending FMA sums), {dt * 1e3:.3f} ms per solve, ~{seconds:.0f} s"}
+
+
+def traffic_from_profiles():
+    """HBM bytes per launch measured with rocprofv3 --pmc (profiles/traffic.json), or None."""
+    p = os.path.join(ROOT, "profiles", "traffic.json")
+    if not os.path.exists(p):
+        return None
+    with open(p) as f:
+        d = json.load(f)
+    return d.get("sptrsv_lap3d64_bytes_per_launch")
 
 
 def main():
@@ -71,34 +87,41 @@ def main():
     ap.add_argument("--steps", type=int, default=200)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--quick", action="store_true", help="skip the secondary measurements")
+    ap.add_argument("--quick", action="store_true", help="headline only, no secondary measurements")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (see DESIGN.md, sharding)",
-              file=sys.stderr)
+        print("bench.py measures one GPU; this build has no multi-GPU path (DESIGN.md explains how the "
+              "solve would shard)", file=sys.stderr)
         sys.exit(2)
     import torch
 
+    from baas_amd import _lib
     from baas_amd import matgen as g
-    from baas_amd.api import Handle, Schedule
+    from baas_amd.api import PCG, Handle, Schedule
 
     torch.cuda.set_device(0)
     h = Handle()
     A = g.laplace3d(64)
     L = A.lower()
     b = g.rhs_uniform(L.n, 0)
-    s = Schedule(h, L.rowptr, L.colidx, kind=0)
+    t0 = time.perf_counter()
+    s = Schedule(h, L.rowptr, L.colidx, kind=_lib.KIND_TRSV)
+    t_an = time.perf_counter() - t0
     lv = torch.as_tensor(L.values, device="cuda")
     bd = torch.as_tensor(b, device="cuda")
     x = torch.empty_like(bd)
-    info = s.info()
+    t0 = time.perf_counter()
+    plan = s.autotune(lv)
+    t_tune = time.perf_counter() - t0
+    print(f"[bench] analysis {t_an:.3f} s, autotune {t_tune:.2f} s, plan {plan}", file=sys.stderr, flush=True)
 
-    med_ms, mean_ms, _ = time_events(lambda: s.solve(lv, bd, x), args.steps, args.warmup)
+    med_ms, mean_ms = time_events(lambda: s.solve(lv, bd, x), args.steps, args.warmup)
     h.check()
-    by = algorithmic_bytes_sptrsv(L.n, L.nnz)
+    by = bytes_sptrsv(L.n, L.nnz)
     gbs = by / (med_ms * 1e-3) / 1e9
     out = {
-        "metric": "SpTRSV GB/s (% HBM peak); ICCG time-to-1e-8, FP32-precond vs FP64",
+        "metric": METRIC,
         "value": round(gbs, 3),
         "unit": "GB/s",
         "n_gpus": 1,
@@ -110,16 +133,71 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (3D 7-point Laplacian 64^3, b ~ U(-1,1) seed 0)",
-        "config": {"workload": "FP64 forward SpTRSV, lower triangle of the 3D 7-point Laplacian, 64^3 "
-                               "natural ordering, 1 RHS",
-                   "n": L.n, "nnz_L": L.nnz, "plan": info},
+        "config": {"workload": "FP64 forward SpTRSV x = L^-1 b, L = lower triangle of the 3D 7-point Laplacian "
+                               "on a 64^3 grid, natural ordering, 1 right-hand side",
+                   "n": L.n, "nnz_L": L.nnz, "bytes_per_solve": by, "plan": plan,
+                   "analysis_s": round(t_an, 4), "autotune_s": round(t_tune, 3)},
         "roofline": {"bound": "hbm", "achieved": round(gbs, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs / HBM_PEAK_GBS, 5), "traffic": None,
-                     "kernel": "k_sweep (persistent SpTRSV)", "bytes_per_launch": by,
+                     "frac": round(gbs / HBM_PEAK_GBS, 6), "traffic": traffic_from_profiles(),
+                     "kernel": "k_sweep_df (dataflow)" if plan["dataflow"] else "k_sweep",
                      "mean_ms": round(mean_ms, 5)},
     }
+    if not args.quick:
+        extra = {}
+        # seed-1 random symmetric permutation of the same matrix
+        P, _ = g.random_symmetric_permutation(A, 1)
+        LP = P.lower()
+        sp = Schedule(h, LP.rowptr, LP.colidx)
+        lpv = torch.as_tensor(LP.values, device="cuda")
+        pplan = sp.autotune(lpv)
+        m, _ = time_events(lambda: sp.solve(lpv, bd, x), 50, 5)
+        extra["sptrsv_perm1"] = {"ms": round(m, 4), "GB/s": round(by / (m * 1e-3) / 1e9, 2), "plan": pplan}
+        # 32 right-hand sides, natural ordering
+        B = torch.as_tensor(g.rhs_uniform(L.n, 1, k=32), device="cuda")
+        X = torch.empty_like(B)
+        kplan = s.autotune(lv, nrhs=32)
+        m, _ = time_events(lambda: s.solve(lv, B, X), 20, 3)
+        byk = bytes_sptrsv(L.n, L.nnz, k=32)
+        extra["sptrsv_32rhs"] = {"ms": round(m, 4), "GB/s": round(byk / (m * 1e-3) / 1e9, 2),
+                                 "frac_of_peak": round(byk / (m * 1e-3) / 1e9 / HBM_PEAK_GBS, 5), "plan": kplan}
+        # IC0 (64^3 Laplacian) and ILU0 (64^3 convection-diffusion)
+        sa = Schedule(h, A.rowptr, A.colidx, kind=_lib.KIND_IC0)
+        av = torch.as_tensor(A.values, device="cuda")
+        lu = torch.empty_like(av)
+        m, _ = time_events(lambda: sa.ic0(av, lu), 20, 3)
+        h.check()
+        nl = L.nnz
+        extra["ic0_lap3d64"] = {"ms": round(m, 4), "GB/s": round((20 * nl + 8 * L.n) / (m * 1e-3) / 1e9, 2)}
+        Cd = g.convdiff3d(64)
+        sc = Schedule(h, Cd.rowptr, Cd.colidx, kind=_lib.KIND_ILU0)
+        cv = torch.as_tensor(Cd.values, device="cuda")
+        luc = torch.empty_like(cv)
+        m, _ = time_events(lambda: sc.ilu0(cv, luc), 20, 3)
+        h.check()
+        extra["ilu0_convdiff64"] = {"ms": round(m, 4),
+                                    "GB/s": round((20 * Cd.nnz + 8 * Cd.n) / (m * 1e-3) / 1e9, 2)}
+        # ICCG time to 1e-8, FP64 factor vs FP32 factor (kNN 3x3-block stiffness, n = 36000)
+        K, kb = g.knn_stiffness()
+        sk = Schedule(h, K.rowptr, K.colidx, kind=_lib.KIND_IC0)
+        kv = torch.as_tensor(K.values, device="cuda")
+        kbd = torch.as_tensor(kb, device="cuda")
+        res = {}
+        for prec, name in ((0, "fp64"), (1, "fp32_factor")):
+            solver = PCG(h, sk, kv, precision=prec)
+            solver.solve(kbd, tol=1e-8, maxiter=5000)  # warm-up: factor, capture the graph
+            ts, its = [], 0
+            for _ in range(5):
+                xk, its, rel = solver.solve(kbd, tol=1e-8, maxiter=5000)
+                ts.append(solver.times()[1])
+            ic_s, _ = solver.times()
+            solver.solve(kbd, tol=1e-8, maxiter=5000, refactor=True)
+            res[name] = {"iterations": its, "time_to_1e-8_s": round(float(np.median(ts)), 6),
+                         "ic0_s": round(solver.times()[0], 6), "relres": rel}
+        res["speedup_fp32_factor"] = round(res["fp64"]["time_to_1e-8_s"] / res["fp32_factor"]["time_to_1e-8_s"], 3)
+        extra["iccg_knn36000"] = res
+        out["extra"] = extra
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(L, b)
+        out["cpu_baseline"] = cpu_baseline(L, b, args.cpu_seconds)
     print(json.dumps(out))
 
 

@@ -201,6 +201,11 @@ int baas_replan(baas_schedule *s, int mode, int rows_per_unit, int block_threads
  * waits, prefetch waves for parents of other units) when enable != 0 and the
  * unit's slots fit in LDS. */
 int baas_plan_dataflow(baas_schedule *s, int enable);
+/* Autotune the execution plan for sweeps with `uplo` and `nrhs` on these
+ * values: times RANGE and SCHEDULE packings over rows-per-unit, workgroup
+ * size and the dataflow kernel, keeps the fastest.  chosen (host int[5], may
+ * be NULL) = mode, rows per unit, threads per workgroup, dataflow, microseconds. */
+int baas_autotune(baas_handle *h, baas_schedule *s, const double *vals, int uplo, int nrhs, int *chosen);
 
 #ifdef __cplusplus
 }

@@ -196,3 +196,25 @@ def test_invalid_pattern_rejected(handle):
     with pytest.raises(_lib.BaasError) as e:
         Schedule(handle, rp, ci, kind=0)
     assert e.value.code == _lib.INVALID_ARG
+
+
+@pytest.mark.parametrize("name", ["C2_lap3d_64", "C2_lap3d_64_perm1"])
+def test_autotune_keeps_bits(handle, name):
+    from baas_amd.api import Schedule
+    announce("autotune", name)
+    L, _, _ = workload(name)
+    s = Schedule(handle, L.rowptr, L.colidx, kind=0)
+    lv = up(L.values)
+    for uplo in (0, 1):
+        vals = lv if uplo == 0 else s.transpose_values(lv)
+        chosen = s.autotune(vals, uplo=uplo)
+        print("[autotune]", name, uplo, chosen)
+        assert chosen["mode"] in (1, 2) and chosen["us"] > 0
+        b = g.rhs_uniform(L.n, 0)
+        got = down(s.solve(vals, up(b), uplo=uplo))
+        handle.check()
+        if uplo == 0:
+            np.testing.assert_array_equal(got, O.sptrsv_lower(L, b))
+        else:
+            U = g.Csr.from_scipy(L.to_scipy().T)
+            np.testing.assert_array_equal(got, O.sptrsv_upper(U, b))
