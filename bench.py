@@ -6,7 +6,7 @@ A step is one forward triangular solve x = L^-1 b with the lower triangle L of
 the 3D 7-point Laplacian on a 64^3 grid in natural ordering (n = 262144,
 nnz(L) = 1036288), one right-hand side b ~ U(-1,1) (seed 0), every input
 resident in HBM.  value = algorithmic GB/s = (12 nnz(L) + 20 n) bytes per
-solve / median time per solve (the reference's byte model,
+solve x K solves / the timed region's length (the reference's byte model,
 test/iccg_level_merge.cpp:354-355).  The execution plan is autotuned before
 the timed region (analysis time, reported separately, as the reference's
 Scheduling_Time).
@@ -39,8 +39,11 @@ def bytes_sptrsv(n, nnz, k=1, vbytes=8):
 
 
 def time_events(fn, steps, warmup):
-    """Median and mean ms per call, HIP events on torch's current stream (the
-    stream every library call of this process is issued on)."""
+    """Time `steps` back-to-back calls on torch's current stream (the stream
+    every library call of this process is issued on) with HIP events around
+    each call.  Returns (whole-region ms, mean ms per call, median ms per call);
+    the region runs from the first call's start event to the last call's end
+    event, bracketed by device synchronisation."""
     import torch
     for _ in range(warmup):
         fn()
@@ -53,7 +56,8 @@ def time_events(fn, steps, warmup):
         b.record(st)
     torch.cuda.synchronize()
     ms = [a.elapsed_time(b) for a, b in evs]
-    return float(np.median(ms)), float(np.mean(ms))
+    total = evs[0][0].elapsed_time(evs[-1][1])
+    return float(total), float(np.mean(ms)), float(np.median(ms))
 
 
 def cpu_baseline(L, b, seconds):
@@ -75,10 +79,10 @@ def traffic_from_profiles():
     """HBM bytes per launch measured with rocprofv3 --pmc (profiles/traffic.json), or None."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
     if not os.path.exists(p):
-        return None
+        return None, None
     with open(p) as f:
         d = json.load(f)
-    return d.get("sptrsv_lap3d64_bytes_per_launch")
+    return d.get("sptrsv_lap3d64_bytes_per_launch"), d.get("plan")
 
 
 def main():
@@ -89,6 +93,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only, no secondary measurements")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--plan", default="auto",
+                    help="auto (autotune) or MODE:ROWS_PER_UNIT:BLOCK:DATAFLOW, MODE 1 = RANGE, 2 = SCHEDULE")
+    ap.add_argument("--calib-copy-mib", type=int, default=0,
+                    help="first run 3 plain copy kernels of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU; this build has no multi-GPU path (DESIGN.md explains how the "
@@ -101,6 +109,13 @@ def main():
     from baas_amd.api import PCG, Handle, Schedule
 
     torch.cuda.set_device(0)
+    if args.calib_copy_mib:
+        src = torch.ones(args.calib_copy_mib << 17, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.neg(src, out=dst)  # reads and writes calib_copy_mib MiB
+        torch.cuda.synchronize()
+        del src, dst
     h = Handle()
     A = g.laplace3d(64)
     L = A.lower()
@@ -112,14 +127,21 @@ def main():
     bd = torch.as_tensor(b, device="cuda")
     x = torch.empty_like(bd)
     t0 = time.perf_counter()
-    plan = s.autotune(lv)
+    if args.plan == "auto":
+        plan = s.autotune(lv)
+    else:
+        mode, rpu, blk, df = (int(v) for v in args.plan.split(":"))
+        s.replan(mode, rpu, blk, dataflow=df)
+        plan = {"mode": mode, "rows_per_unit": rpu, "block": blk, "dataflow": df, "us": None}
     t_tune = time.perf_counter() - t0
     print(f"[bench] analysis {t_an:.3f} s, autotune {t_tune:.2f} s, plan {plan}", file=sys.stderr, flush=True)
 
-    med_ms, mean_ms = time_events(lambda: s.solve(lv, bd, x), args.steps, args.warmup)
+    total_ms, mean_ms, med_ms = time_events(lambda: s.solve(lv, bd, x), args.steps, args.warmup)
     h.check()
     by = bytes_sptrsv(L.n, L.nnz)
-    gbs = by / (med_ms * 1e-3) / 1e9
+    traffic, traffic_plan = traffic_from_profiles()
+    gbs = by * args.steps / (total_ms * 1e-3) / 1e9
+    kern_gbs = by / (mean_ms * 1e-3) / 1e9
     out = {
         "metric": METRIC,
         "value": round(gbs, 3),
@@ -127,7 +149,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(med_ms, 5),
+        "ms_per_step": round(total_ms / args.steps, 5),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -137,10 +159,12 @@ def main():
                                "on a 64^3 grid, natural ordering, 1 right-hand side",
                    "n": L.n, "nnz_L": L.nnz, "bytes_per_solve": by, "plan": plan,
                    "analysis_s": round(t_an, 4), "autotune_s": round(t_tune, 3)},
-        "roofline": {"bound": "hbm", "achieved": round(gbs, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(gbs / HBM_PEAK_GBS, 6), "traffic": traffic_from_profiles(),
+        "roofline": {"bound": "hbm", "achieved": round(kern_gbs, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(kern_gbs / HBM_PEAK_GBS, 6), "traffic": traffic,
+                     "traffic_plan": traffic_plan,
                      "kernel": "k_sweep_df (dataflow)" if plan["dataflow"] else "k_sweep",
-                     "mean_ms": round(mean_ms, 5)},
+                     "algorithmic_bytes_per_launch": by, "mean_launch_ms": round(mean_ms, 5),
+                     "median_launch_ms": round(med_ms, 5)},
     }
     if not args.quick:
         extra = {}
@@ -150,13 +174,13 @@ def main():
         sp = Schedule(h, LP.rowptr, LP.colidx)
         lpv = torch.as_tensor(LP.values, device="cuda")
         pplan = sp.autotune(lpv)
-        m, _ = time_events(lambda: sp.solve(lpv, bd, x), 50, 5)
+        _, m, _ = time_events(lambda: sp.solve(lpv, bd, x), 50, 5)
         extra["sptrsv_perm1"] = {"ms": round(m, 4), "GB/s": round(by / (m * 1e-3) / 1e9, 2), "plan": pplan}
         # 32 right-hand sides, natural ordering
         B = torch.as_tensor(g.rhs_uniform(L.n, 1, k=32), device="cuda")
         X = torch.empty_like(B)
         kplan = s.autotune(lv, nrhs=32)
-        m, _ = time_events(lambda: s.solve(lv, B, X), 20, 3)
+        _, m, _ = time_events(lambda: s.solve(lv, B, X), 20, 3)
         byk = bytes_sptrsv(L.n, L.nnz, k=32)
         extra["sptrsv_32rhs"] = {"ms": round(m, 4), "GB/s": round(byk / (m * 1e-3) / 1e9, 2),
                                  "frac_of_peak": round(byk / (m * 1e-3) / 1e9 / HBM_PEAK_GBS, 5), "plan": kplan}
@@ -164,7 +188,7 @@ def main():
         sa = Schedule(h, A.rowptr, A.colidx, kind=_lib.KIND_IC0)
         av = torch.as_tensor(A.values, device="cuda")
         lu = torch.empty_like(av)
-        m, _ = time_events(lambda: sa.ic0(av, lu), 20, 3)
+        _, m, _ = time_events(lambda: sa.ic0(av, lu), 20, 3)
         h.check()
         nl = L.nnz
         extra["ic0_lap3d64"] = {"ms": round(m, 4), "GB/s": round((20 * nl + 8 * L.n) / (m * 1e-3) / 1e9, 2)}
@@ -172,7 +196,7 @@ def main():
         sc = Schedule(h, Cd.rowptr, Cd.colidx, kind=_lib.KIND_ILU0)
         cv = torch.as_tensor(Cd.values, device="cuda")
         luc = torch.empty_like(cv)
-        m, _ = time_events(lambda: sc.ilu0(cv, luc), 20, 3)
+        _, m, _ = time_events(lambda: sc.ilu0(cv, luc), 20, 3)
         h.check()
         extra["ilu0_convdiff64"] = {"ms": round(m, 4),
                                     "GB/s": round((20 * Cd.nnz + 8 * Cd.n) / (m * 1e-3) / 1e9, 2)}
