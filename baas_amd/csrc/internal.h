@@ -82,10 +82,16 @@ struct Plan {
     DBuf<int> key;        // [n]        keymode 2 only
     // dataflow form (one lane per row, LDS slots instead of sub-level barriers)
     DBuf<int> unit_ptr;   // [nunits+1] unit boundaries in rows
-    DBuf<int> scol;       // [nnz]      LDS slot of each entry's parent (-1 on the diagonal)
+    DBuf<int> scol;       // [nnz]      LDS slot of each entry's parent (0 on the diagonal)
     DBuf<int> ext_ptr;    // [nunits+1] external parents of each unit ...
     DBuf<int> ext_cols;   // [...]      ... in order of first use
+    DBuf<int> meta;       // [4n] per position: row | ext flag, first off-diagonal entry, count, LDS slot
+    DBuf<int> out_ptr;    // [nunits+1] rows of each unit read by later units ...
+    DBuf<int> out_slot;   // [...]      ... as LDS slots, in execution order
+    size_t df_lds[2] = {0, 0};  // dataflow kernel LDS bytes per workgroup (FP32, FP64)
+    DBuf<int> key_row;    // [n]        row of each key (keymode 2 only)
     int max_slots = 0;    // max over units of keys + external parents
+    int max_deg = 0;      // max off-diagonal entries of a row
     int dataflow = 0;     // use the dataflow kernel for single-RHS sweeps
     DBuf<unsigned char> xp_raw;  // 2 x n published values (dataflow kernel)
     int xp_type = 0;             // sizeof the value type the halves are armed for

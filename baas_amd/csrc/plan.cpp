@@ -111,7 +111,9 @@ int make_plan(Plan &p, int n, const std::vector<int> &order, const std::vector<i
     sub_begin[nunits] = (int)sub_ptr.size() - 1;
     // dataflow slots: intra-unit parents live at key - key_begin, external
     // parents after the unit's keys in order of first use
-    std::vector<int> scol(full.idx.size(), -1), ext_ptr(nunits + 1, 0), ext_cols;
+    // (the diagonal's entry holds slot 0: a row without parents may load it,
+    // never use it)
+    std::vector<int> scol(full.idx.size(), 0), ext_ptr(nunits + 1, 0), ext_cols;
     std::vector<int> ext_slot(n, -1), ext_touched;
     int max_slots = 0;
     for (int u = 0; u < nunits; u++) {
@@ -141,6 +143,34 @@ int make_plan(Plan &p, int n, const std::vector<int> &order, const std::vector<i
         max_slots = std::max(max_slots, nk + (int)ext_cols.size() - ext_ptr[u]);
     }
     ext_ptr[nunits] = (int)ext_cols.size();
+    // rows each unit publishes for later units, in execution order, as LDS
+    // slots; and the dataflow kernel's LDS footprint per unit (one value per
+    // row of the unit and per external parent)
+    std::vector<int> out_ptr(nunits + 1, 0), out_slot;
+    size_t df_lds8 = 0, df_lds4 = 0;
+    for (int u = 0; u < nunits; u++) {
+        out_ptr[u] = (int)out_slot.size();
+        for (int q = unit_ptr[u]; q < unit_ptr[u + 1]; q++)
+            if (rows[q] < 0) out_slot.push_back(key[rows[q] & 0x7fffffff] - key_begin[u]);
+        const size_t nk = unit_ptr[u + 1] - unit_ptr[u], ne = ext_ptr[u + 1] - ext_ptr[u];
+        df_lds8 = std::max(df_lds8, (nk + ne) * 8);
+        df_lds4 = std::max(df_lds4, (nk + ne) * 4);
+    }
+    out_ptr[nunits] = (int)out_slot.size();
+    // per-position row records for the dataflow kernel: one 16-byte load
+    // gives a lane everything it needs to fetch its row
+    std::vector<int> meta(4 * (size_t)n);
+    p.max_deg = 0;
+    for (int u = 0; u < nunits; u++)
+        for (int q = unit_ptr[u]; q < unit_ptr[u + 1]; q++) {
+            const int r = rows[q] & 0x7fffffff;
+            const int rb = full.ptr[r], re = full.ptr[r + 1];
+            meta[4 * (size_t)q + 0] = rows[q];
+            meta[4 * (size_t)q + 1] = upper ? re - 1 : rb;
+            meta[4 * (size_t)q + 2] = re - rb - 1;
+            p.max_deg = std::max(p.max_deg, re - rb - 1);
+            meta[4 * (size_t)q + 3] = key[r] - key_begin[u];
+        }
     // unit DAG: producer units of each unit
     std::vector<int> udep_ptr(nunits + 1, 0), udeps, mark(nunits, -1);
     for (int u = 0; u < nunits; u++) {
@@ -173,6 +203,11 @@ int make_plan(Plan &p, int n, const std::vector<int> &order, const std::vector<i
     BAAS_TRY(upload(p.scol, scol, st));
     BAAS_TRY(upload(p.ext_ptr, ext_ptr, st));
     BAAS_TRY(upload(p.ext_cols, ext_cols, st));
+    BAAS_TRY(upload(p.meta, meta, st));
+    BAAS_TRY(upload(p.out_ptr, out_ptr, st));
+    BAAS_TRY(upload(p.out_slot, out_slot, st));
+    p.df_lds[0] = df_lds4;
+    p.df_lds[1] = df_lds8;
     BAAS_TRY(upload(p.unit_ptr, unit_ptr, st));
     p.max_slots = max_slots;
     p.n = n;
@@ -186,7 +221,10 @@ int make_plan(Plan &p, int n, const std::vector<int> &order, const std::vector<i
     BAAS_TRY(upload(p.sub_begin, sub_begin, st));
     BAAS_TRY(upload(p.sub_ptr, sub_ptr, st));
     BAAS_TRY(upload(p.rows, rows, st));
-    if (keymode == 2) BAAS_TRY(upload(p.key, key, st));
+    if (keymode == 2) {
+        BAAS_TRY(upload(p.key, key, st));
+        BAAS_TRY(upload(p.key_row, order, st));  // key = position in order
+    }
     p.h_unit_ptr = unit_ptr;
     p.h_rows.resize(n);
     for (int q = 0; q < n; q++) p.h_rows[q] = rows[q] & 0x7fffffff;

@@ -133,9 +133,10 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a, const T *__restrict
 // with the sentinel.  A sweep publishes into the half selected by the plan's
 // sweep counter and re-arms the same rows in the other half for the next
 // sweep, so a consumer needs one load per parent (the value is the flag) and
-// a producer never waits for its stores.  Dedicated prefetch waves stage
-// those parents into LDS while the compute waves work, and every compute
-// lane loads its next row's data before it starts waiting on the current one.
+// a producer never waits for its stores.  Two exchange waves per workgroup
+// move values between LDS and the publication array (one stages parents from
+// earlier units, one publishes rows later units read), so the compute waves
+// touch global memory only to load matrix data, a round ahead of its use.
 template <class T> struct Sentinel;
 template <> struct Sentinel<double> {
     static __device__ __forceinline__ double value() { return __longlong_as_double(0x7ff4deadbeef0001LL); }
@@ -156,9 +157,12 @@ __device__ __forceinline__ void st_lds(T *p, T v) {
 }
 
 struct DfArgs {
-    int n, nunits, npf;
-    const int *rowptr, *scol;
-    const int *key_begin, *unit_ptr, *rows, *key, *ext_ptr, *ext_cols;
+    int n, nunits, keymode;
+    const int4 *meta;  // per position: row | ext flag, first off-diagonal entry, count, LDS slot
+    const int *scol;
+    const int *unit_ptr, *key_begin, *ext_ptr, *ext_cols;
+    const int *out_ptr, *out_slot;  // slots of rows later units read
+    const int *key_row;             // keymode 2: row of each key
     int *pcount;  // the plan's sweep counter (parity selects the publication half)
     Ctx ctx;
     const int *skip;
@@ -166,46 +170,209 @@ struct DfArgs {
     long long spin_limit;
 };
 
-constexpr int DF_CHUNK = 8;
+constexpr int DF_PF = 8;  // entries an exchange lane polls at once
 
-// one row's data, loaded ahead of its turn
-template <class T>
+// one row's data, loaded a round ahead of its turn; CH parents per chunk
+template <class T, int CH>
 struct DfRow {
-    int rw, row, dg, left, jn, nc;
-    T bv;
-    int cs[DF_CHUNK];
-    T cv[DF_CHUNK];
+    int rw, slot, left, jn, nc;
+    T bv, dv;
+    int cs[CH];
+    T cv[CH];
 };
 
-template <class T, bool UPPER>
-__device__ __forceinline__ void df_load_row(const DfArgs &a, const T *__restrict__ vals, const T *__restrict__ b,
-                                            int q, int q1, DfRow<T> &R) {
-    R.nc = 0;
-    R.left = 0;
-    R.rw = -1;
-    if (q >= q1) return;
-    R.rw = a.rows[q];
-    R.row = R.rw & 0x7fffffff;
-    const int rb = a.rowptr[R.row], re = a.rowptr[R.row + 1];
-    R.dg = UPPER ? rb : re - 1;
-    R.left = re - rb - 1;
-    R.jn = UPPER ? re - 1 : rb;
-    R.bv = b[R.row];
-    R.nc = R.left < DF_CHUNK ? R.left : DF_CHUNK;
-#pragma unroll
-    for (int t = 0; t < DF_CHUNK; t++) {
-        if (t < R.nc) {
-            const int j = UPPER ? R.jn - t : R.jn + t;
-            R.cs[t] = a.scol[j];
-            R.cv[t] = vals[j];
-        }
-    }
-    R.jn = UPPER ? R.jn - R.nc : R.jn + R.nc;
-    R.left -= R.nc;
+// Row record of position q; positions past the unit's end read the unit's
+// last record (so every load stays unconditional and in bounds) and are
+// marked empty by the caller.
+__device__ __forceinline__ int4 df_meta(const DfArgs &a, int q, int q1) {
+    return a.meta[q < q1 ? q : q1 - 1];
 }
 
-template <class T, int KEYMODE, bool UPPER>
-__global__ __launch_bounds__(1024) void k_sweep_df(DfArgs a, const T *__restrict__ vals, const T *__restrict__ b,
+// Issue the loads of one row (b, diagonal, first chunk of parents); nothing
+// here waits for them.  The loads are unconditional (a short row re-reads its
+// last parent), which keeps the compiler from merging divergent paths into a
+// wait for every load in flight.
+template <class T, bool UPPER, int CH>
+__device__ __forceinline__ void df_load_row(const DfArgs &a, const T *__restrict__ vals, const T *__restrict__ b,
+                                            const int4 m, bool valid, DfRow<T, CH> &R) {
+    const int cnt = m.z;
+    R.slot = valid ? m.w : -1;
+    R.rw = m.x;
+    R.bv = b[m.x & 0x7fffffff];
+    R.dv = vals[UPPER ? m.y - cnt : m.y + cnt];
+    R.nc = valid ? (cnt < CH ? cnt : CH) : 0;
+    const int last = cnt > 0 ? (cnt < CH ? cnt : CH) - 1 : 0;
+#pragma unroll
+    for (int t = 0; t < CH; t++) {
+        const int o = t < last ? t : last;
+        const int j = UPPER ? m.y - o : m.y + o;
+        R.cs[t] = a.scol[j];  // no arithmetic on loaded values here: that would wait for them
+        R.cv[t] = vals[j];
+    }
+    R.jn = UPPER ? m.y - R.nc : m.y + R.nc;
+    R.left = valid ? cnt - R.nc : 0;
+}
+
+// Exchange wave, incoming side: stage the unit's parents in other units into
+// LDS slots [nk, nk + ne).  Each lane walks its own stride of the
+// first-use-ordered list (cols), DF_PF entries at a time: it loads their
+// columns, then polls all of them with one load each per pass and stages
+// each as soon as it is published.
+template <class T>
+__device__ __forceinline__ void df_incoming(const DfArgs &a, const T *pub, T *sl, const int *cols, int nk, int ne,
+                                            int lane) {
+    for (int e = lane; e < ne; e += DF_PF * 64) {
+        int c[DF_PF];
+#pragma unroll
+        for (int k = 0; k < DF_PF; k++) {
+            const int ek = e + k * 64;
+            c[k] = cols[ek < ne ? ek : ne - 1];
+        }
+        unsigned todo = 0;
+#pragma unroll
+        for (int k = 0; k < DF_PF; k++)
+            if (e + k * 64 < ne) todo |= 1u << k;
+        long long spins = 0;
+        while (todo) {
+            T v[DF_PF];
+#pragma unroll
+            for (int k = 0; k < DF_PF; k++) v[k] = ld_agent(&pub[c[k]]);
+#pragma unroll
+            for (int k = 0; k < DF_PF; k++) {
+                if (((todo >> k) & 1u) && !Sentinel<T>::is(v[k])) {
+                    st_lds(&sl[nk + e + k * 64], v[k]);
+                    todo &= ~(1u << k);
+                }
+            }
+            if (todo) {
+                if (++spins > a.spin_limit) {
+                    // give up: report, and let the unit finish on zeros
+                    raise_status(a.status, BAAS_TIMEOUT, c[__builtin_ctz(todo)]);
+#pragma unroll
+                    for (int k = 0; k < DF_PF; k++)
+                        if ((todo >> k) & 1u) st_lds(&sl[nk + e + k * 64], T(0));
+                    todo = 0;
+                } else {
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+        }
+    }
+}
+
+// Exchange wave, outgoing side: publish the unit's rows that later units read
+// (their LDS slots in execution order, outs) as soon as they are solved (the
+// value is the flag), and re-arm the same rows in the other half for the
+// next sweep.  Compute waves therefore issue no global stores.
+template <class T>
+__device__ __forceinline__ void df_outgoing(const DfArgs &a, T *pub, T *nxt, const T *sl, const int *outs, int no,
+                                            int kb, int lane) {
+    for (int o = lane; o < no; o += DF_PF * 64) {
+        int sk[DF_PF];
+#pragma unroll
+        for (int k = 0; k < DF_PF; k++) {
+            const int ok = o + k * 64;
+            sk[k] = outs[ok < no ? ok : no - 1];
+        }
+        unsigned todo = 0;
+#pragma unroll
+        for (int k = 0; k < DF_PF; k++)
+            if (o + k * 64 < no) todo |= 1u << k;
+        long long spins = 0;
+        while (todo) {
+            T v[DF_PF];
+#pragma unroll
+            for (int k = 0; k < DF_PF; k++) v[k] = ld_lds(&sl[sk[k]]);
+            unsigned got = 0;
+#pragma unroll
+            for (int k = 0; k < DF_PF; k++) {
+                if (((todo >> k) & 1u) && !Sentinel<T>::is(v[k])) {
+                    const int key = kb + sk[k];
+                    const int row = a.keymode == 0 ? key : a.keymode == 1 ? a.n - 1 - key : a.key_row[key];
+                    st_agent(&pub[row], v[k]);
+                    nxt[row] = Sentinel<T>::value();
+                    got |= 1u << k;
+                }
+            }
+            todo &= ~got;
+            if (todo && !got) {
+                if (++spins > a.spin_limit) {
+                    todo = 0;  // the unit's own rows are stuck: its compute waves report it
+                } else {
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+        }
+    }
+}
+
+// Solve the row in R (one lane's row of a round): read its parents' LDS
+// slots until every one is solved, consuming the ready prefix in order, then
+// write the result to the row's slot.  The wavefront leaves when all its
+// lanes are done.
+template <class T, bool UPPER, int CH>
+__device__ __forceinline__ void df_solve(const DfArgs &a, const T *__restrict__ vals, T *sl, DfRow<T, CH> &R) {
+    bool pending = R.slot >= 0;
+    T s = T(0);
+    int pc = 0;
+    long long spins = 0;
+    while (pending) {
+        if (pc == R.nc && R.left > 0) {  // rows longer than one chunk
+            R.nc = R.left < CH ? R.left : CH;
+            const int last = R.nc - 1;
+#pragma unroll
+            for (int t = 0; t < CH; t++) {
+                const int o = t < last ? t : last;
+                const int j = UPPER ? R.jn - o : R.jn + o;
+                R.cs[t] = a.scol[j];
+                R.cv[t] = vals[j];
+            }
+            R.jn = UPPER ? R.jn - R.nc : R.jn + R.nc;
+            R.left -= R.nc;
+            pc = 0;
+            // complete these loads here (this also waits for the loads of
+            // later rounds; rows this long are rare)
+            __builtin_amdgcn_s_waitcnt(0);
+        }
+        if (pc < R.nc) {
+            // read every parent slot of the chunk at once (slots past the
+            // row's count repeat its last parent), then consume the ready
+            // prefix in order
+            T v[CH];
+#pragma unroll
+            for (int t = 0; t < CH; t++) v[t] = ld_lds(&sl[R.cs[t]]);
+#pragma unroll
+            for (int t = 0; t < CH; t++) {
+                if (t == pc && t < R.nc && !Sentinel<T>::is(v[t])) {
+                    s = UPPER ? fma(v[t], R.cv[t], s) : fma(R.cv[t], v[t], s);
+                    pc++;
+                }
+            }
+            if (pc < R.nc && ++spins > a.spin_limit) {
+                raise_status(a.status, BAAS_TIMEOUT, R.rw & 0x7fffffff);
+                pc = R.nc;  // give up on this row's remaining parents
+                R.left = 0;
+            }
+        } else {
+            st_lds(&sl[R.slot], (R.bv - s) / R.dv);
+            pending = false;
+        }
+    }
+}
+
+// One round of a compute wave on buffer R (loaded DF_DEPTH rounds ago), then
+// refill R with the round DF_DEPTH ahead (position qn) from its record m, and
+// load the record of the round after that (position qm) into m.
+template <class T, bool UPPER, int CH>
+__device__ __forceinline__ void df_round(const DfArgs &a, const T *__restrict__ vals, const T *__restrict__ b,
+                                         T *sl, DfRow<T, CH> &R, int4 &m, int qn, int qm, int q1) {
+    df_solve<T, UPPER, CH>(a, vals, sl, R);
+    df_load_row<T, UPPER, CH>(a, vals, b, m, qn < q1, R);
+    m = df_meta(a, qm, q1);
+}
+
+template <class T, bool UPPER, int CH>
+__global__ __launch_bounds__(512) void k_sweep_df(DfArgs a, const T *__restrict__ vals, const T *__restrict__ b,
                                                   T *x, T *xp) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     T *sl = reinterpret_cast<T *>(smem_raw);
@@ -216,89 +383,56 @@ __global__ __launch_bounds__(1024) void k_sweep_df(DfArgs a, const T *__restrict
     T *pub = xp + (size_t)par * a.n;
     T *nxt = xp + (size_t)(par ^ 1) * a.n;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
-    const int npf = a.npf;
     for (;;) {
         const int u = claim_unit(a.ctx.ticket, &s_unit);
         if (u >= a.nunits) break;
-        const int kb = a.key_begin[u], nk = a.key_begin[u + 1] - kb;
-        const int e0 = a.ext_ptr[u], e1 = a.ext_ptr[u + 1];
-        const int nslots = nk + (e1 - e0);
+        const int q0 = a.unit_ptr[u], q1 = a.unit_ptr[u + 1], nk = q1 - q0;
+        const int e0 = a.ext_ptr[u], ne = a.ext_ptr[u + 1] - e0;
+        const int o0 = a.out_ptr[u], no = a.out_ptr[u + 1] - o0;
+        const int kb = a.key_begin[u];
+        // LDS: one value slot per row of the unit, then one per external parent
+        const int nslots = nk + ne;
         for (int i = threadIdx.x; i < nslots; i += blockDim.x) st_lds(&sl[i], Sentinel<T>::value());
         __syncthreads();
-        if (wv < npf) {
-            // prefetch waves: stage parents of other units in order of first use
-            for (int e = e0 + wv * 64 + lane; e < e1; e += npf * 64) {
-                const int c = a.ext_cols[e];
-                T v = ld_agent(&pub[c]);
-                long long spins = 0;
-                while (Sentinel<T>::is(v)) {
-                    if (++spins > a.spin_limit) {
-                        raise_status(a.status, BAAS_TIMEOUT, c);
-                        v = T(0);
-                        break;
-                    }
-                    __builtin_amdgcn_s_sleep(1);
-                    v = ld_agent(&pub[c]);
-                }
-                st_lds(&sl[nk + (e - e0)], v);
-            }
+        if (wv == 0) {
+            df_incoming<T>(a, pub, sl, a.ext_cols + e0, nk, ne, lane);
+        } else if (wv == 1) {
+            df_outgoing<T>(a, pub, nxt, sl, a.out_slot + o0, no, kb, lane);
         } else {
-            const int q0 = a.unit_ptr[u], q1 = a.unit_ptr[u + 1];
-            const int C = nw - npf, cw = wv - npf;
-            DfRow<T> cur, nxr;
-            df_load_row<T, UPPER>(a, vals, b, q0 + cw * 64 + lane, q1, cur);
-            for (int base = q0 + cw * 64; base < q1; base += C * 64) {
-                df_load_row<T, UPPER>(a, vals, b, base + C * 64 + lane, q1, nxr);
-                bool pending = base + lane < q1;
-                T s = T(0);
-                int pc = 0;
-                long long spins = 0;
-                while (pending) {
-                    if (pc == cur.nc && cur.left > 0) {  // rows longer than one chunk
-                        cur.nc = cur.left < DF_CHUNK ? cur.left : DF_CHUNK;
-#pragma unroll
-                        for (int t = 0; t < DF_CHUNK; t++) {
-                            if (t < cur.nc) {
-                                const int j = UPPER ? cur.jn - t : cur.jn + t;
-                                cur.cs[t] = a.scol[j];
-                                cur.cv[t] = vals[j];
-                            }
-                        }
-                        cur.jn = UPPER ? cur.jn - cur.nc : cur.jn + cur.nc;
-                        cur.left -= cur.nc;
-                        pc = 0;
-                    }
-                    if (pc < cur.nc) {
-#pragma unroll
-                        for (int t = 0; t < DF_CHUNK; t++) {
-                            if (t == pc && t < cur.nc) {
-                                const T v = ld_lds(&sl[cur.cs[t]]);
-                                if (!Sentinel<T>::is(v)) {
-                                    s = UPPER ? fma(v, cur.cv[t], s) : fma(cur.cv[t], v, s);
-                                    pc++;
-                                }
-                            }
-                        }
-                        if (pc < cur.nc && ++spins > a.spin_limit) {
-                            raise_status(a.status, BAAS_TIMEOUT, cur.row);
-                            pc = cur.nc;  // give up on this row's remaining parents
-                            cur.left = 0;
-                        }
-                    } else {
-                        const T xr = (cur.bv - s) / vals[cur.dg];
-                        st_lds(&sl[key_of<KEYMODE>(cur.row, a.n, a.key) - kb], xr);
-                        x[cur.row] = xr;
-                        if (cur.rw < 0) {
-                            st_agent(&pub[cur.row], xr);
-                            nxt[cur.row] = Sentinel<T>::value();
-                        }
-                        pending = false;
-                    }
-                }
-                cur = nxr;
+            // Compute waves: round r of wave cw covers positions
+            // q0 + (r * C + cw) * 64 + lane.  Each wave keeps DF_DEPTH rounds
+            // of row data in flight in fixed buffers (the round loop is
+            // unrolled DF_DEPTH times, so no buffer is ever copied while its
+            // loads are outstanding) and the row records DF_DEPTH rounds
+            // further ahead, so a round waits for memory only when it ran
+            // faster than a load's latency divided by DF_DEPTH.  Results go to
+            // LDS only; the unit's x is written out once it is complete.
+            const int C = nw - 2, cw = wv - 2, stride = C * 64;
+            const int qa = q0 + cw * 64 + lane;
+            DfRow<T, CH> r0, r1, r2;
+            int4 m0 = df_meta(a, qa, q1), m1 = df_meta(a, qa + stride, q1), m2 = df_meta(a, qa + 2 * stride, q1);
+            df_load_row<T, UPPER, CH>(a, vals, b, m0, qa < q1, r0);
+            df_load_row<T, UPPER, CH>(a, vals, b, m1, qa + stride < q1, r1);
+            df_load_row<T, UPPER, CH>(a, vals, b, m2, qa + 2 * stride < q1, r2);
+            m0 = df_meta(a, qa + 3 * stride, q1);
+            m1 = df_meta(a, qa + 4 * stride, q1);
+            m2 = df_meta(a, qa + 5 * stride, q1);
+            for (int base = q0 + cw * 64; base < q1; base += 3 * stride) {
+                const int q = base + lane;
+                df_round<T, UPPER, CH>(a, vals, b, sl, r0, m0, q + 3 * stride, q + 6 * stride, q1);
+                if (base + stride >= q1) break;
+                df_round<T, UPPER, CH>(a, vals, b, sl, r1, m1, q + 4 * stride, q + 7 * stride, q1);
+                if (base + 2 * stride >= q1) break;
+                df_round<T, UPPER, CH>(a, vals, b, sl, r2, m2, q + 5 * stride, q + 8 * stride, q1);
             }
         }
         __syncthreads();
+        // the unit's solution, slot i holding the row with key kb + i
+        for (int i = threadIdx.x; i < nk; i += blockDim.x) {
+            const int key = kb + i;
+            const int row = a.keymode == 0 ? key : a.keymode == 1 ? a.n - 1 - key : a.key_row[key];
+            x[row] = sl[i];
+        }
     }
     if (threadIdx.x == 0) {
         __threadfence();
@@ -398,10 +532,11 @@ static int launch_sweep_k(baas_handle *h, const Plan &p, const SweepArgs &a, con
 
 size_t lds_limit(baas_handle *h);
 
-template <class T, int KM, bool UP>
-static int launch_df_t(baas_handle *h, const Plan &p, const DfArgs &a, const T *vals, const T *b, T *x, T *xp,
-                       size_t lds, hipStream_t stream) {
-    auto kern = k_sweep_df<T, KM, UP>;
+template <class T, bool UP>
+static int launch_df(baas_handle *h, const Plan &p, const DfArgs &a, const T *vals, const T *b, T *x, T *xp,
+                     size_t lds, hipStream_t stream) {
+    // four parents per chunk cover stencil-like rows; wider rows take eight
+    auto kern = p.max_deg <= 4 ? k_sweep_df<T, UP, 4> : k_sweep_df<T, UP, 8>;
     int per_cu = 0;
     BAAS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, p.block, lds));
     if (per_cu <= 0) return BAAS_UNSUPPORTED;
@@ -413,23 +548,13 @@ static int launch_df_t(baas_handle *h, const Plan &p, const DfArgs &a, const T *
     return BAAS_OK;
 }
 
-template <class T, bool UP>
-static int launch_df(baas_handle *h, const Plan &p, const DfArgs &a, const T *vals, const T *b, T *x, T *xp,
-                     size_t lds, hipStream_t st) {
-    switch (p.keymode) {
-        case 0: return launch_df_t<T, 0, UP>(h, p, a, vals, b, x, xp, lds, st);
-        case 1: return launch_df_t<T, 1, UP>(h, p, a, vals, b, x, xp, lds, st);
-        default: return launch_df_t<T, 2, UP>(h, p, a, vals, b, x, xp, lds, st);
-    }
-}
-
 template <class T>
 int sweep(baas_handle *h, baas_schedule *s, int uplo, const T *vals, const T *b, T *x, int nrhs,
           hipStream_t stream, const int *skip) {
     if (!h || !s || !vals || !b || !x || nrhs < 1 || (uplo != 0 && uplo != 1)) return BAAS_INVALID_ARG;
     if ((const void *)b == (const void *)x) return BAAS_INVALID_ARG;
     Plan &p = uplo ? s->bwd : s->fwd;
-    if (nrhs == 1 && p.dataflow && p.block >= 128 && (size_t)p.max_slots * sizeof(T) <= lds_limit(h)) {
+    if (nrhs == 1 && p.dataflow && p.block >= 192 && p.block <= 512 && p.df_lds[sizeof(T) == 8] <= lds_limit(h)) {
         // publication halves, armed with this value type's sentinel
         if (p.xp_type != (int)sizeof(T)) {
             const size_t cnt = 2 * (size_t)s->n;
@@ -443,20 +568,21 @@ int sweep(baas_handle *h, baas_schedule *s, int uplo, const T *vals, const T *b,
         d.pcount = p.pcount.p;
         d.n = s->n;
         d.nunits = p.nunits;
-        d.npf = p.block >= 512 ? 2 : 1;
-        d.rowptr = uplo ? s->u_rowptr.p : s->l_rowptr.p;
+        d.keymode = p.keymode;
+        d.meta = reinterpret_cast<const int4 *>(p.meta.p);
         d.scol = p.scol.p;
-        d.key_begin = p.key_begin.p;
         d.unit_ptr = p.unit_ptr.p;
-        d.rows = p.rows.p;
-        d.key = p.key.p;
+        d.key_begin = p.key_begin.p;
+        d.out_ptr = p.out_ptr.p;
+        d.out_slot = p.out_slot.p;
+        d.key_row = p.key_row.p;
         d.ext_ptr = p.ext_ptr.p;
         d.ext_cols = p.ext_cols.p;
         d.ctx = Ctx{h->counters, h->counters + 1, h->counters + 2};
         d.skip = skip;
         d.status = h->status;
         d.spin_limit = h->spin_limit;
-        size_t lds = (size_t)p.max_slots * sizeof(T);
+        size_t lds = p.df_lds[sizeof(T) == 8];
         T *xp = reinterpret_cast<T *>(p.xp_raw.p);
         return uplo ? launch_df<T, true>(h, p, d, vals, b, x, xp, lds, stream)
                     : launch_df<T, false>(h, p, d, vals, b, x, xp, lds, stream);

@@ -42,11 +42,11 @@ extern "C" int baas_autotune(baas_handle *h, baas_schedule *s, const double *val
     BAAS_TRY(x.alloc((size_t)n * nrhs));
     hipLaunchKernelGGL(k_fill_d, dim3((n * nrhs + 255) / 256), dim3(256), 0, h->stream, n * nrhs, b.p, 1.0);
     std::vector<Cand> cands;
-    for (int rpu : {1024, 2048, 4096, 8192, 16384})
-        for (int blk : {256, 512, 1024}) {
+    for (int rpu : {1024, 2048, 4096, 8192, 12288, 16384})
+        for (int blk : {256, 384, 512, 1024}) {
             if (blk == 1024 && rpu < 4096) continue;
             cands.push_back({BAAS_PLAN_RANGE, rpu, blk, 0});
-            if (nrhs == 1) cands.push_back({BAAS_PLAN_RANGE, rpu, blk, 1});
+            if (nrhs == 1 && blk <= 512) cands.push_back({BAAS_PLAN_RANGE, rpu, blk, 1});
         }
     for (int rpu : {256, 1024, 4096})
         for (int blk : {256, 1024}) cands.push_back({BAAS_PLAN_SCHEDULE, rpu, blk, 0});
@@ -60,7 +60,7 @@ extern "C" int baas_autotune(baas_handle *h, baas_schedule *s, const double *val
         s->opts.block_threads = c.block;
         if (build_plans_mode(s, c.mode, c.rpu) != BAAS_OK) continue;
         Plan &p = uplo ? s->bwd : s->fwd;
-        if (c.df && (size_t)p.max_slots * sizeof(double) > lds_limit(h)) continue;
+        if (c.df && p.df_lds[1] > lds_limit(h)) continue;
         s->fwd.dataflow = s->bwd.dataflow = c.df;
         float t_best = 1e30f;
         bool ok = true;

@@ -64,7 +64,7 @@ def test_reduced_dag_matches_oracle(handle):
 # (plan mode, rows per unit, threads per workgroup, dataflow kernel)
 PLANS = [(1, 0, 256, False), (1, 1024, 256, False), (1, 16384, 512, False), (2, 0, 256, False),
          (2, 64, 64, False), (2, 1024, 1024, False), (1, 4096, 256, True), (1, 12288, 512, True),
-         (1, 1024, 128, True), (2, 256, 256, True), (2, 2048, 512, True), (1, 8192, 1024, True)]
+         (1, 2048, 192, True), (2, 256, 256, True), (2, 2048, 512, True), (1, 8192, 1024, True)]
 SOLVE_CASES = ["C1_lap2d_128", "C2_lap3d_64", "C2_lap3d_64_perm1", "C5_diff3d_48", "bcsstk18", "bcsstk38"]
 
 
