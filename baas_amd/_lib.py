@@ -14,7 +14,7 @@ LIB_PATH = os.path.join(HERE, "libbaas_hip.so")
 
 OK, INVALID_ARG, ZERO_PIVOT, NONPOSITIVE_PIVOT, HIP_ERROR, TIMEOUT, NOT_CONVERGED, ALLOC, UNSUPPORTED = range(9)
 KIND_TRSV, KIND_IC0, KIND_ILU0 = 0, 1, 2
-PLAN_AUTO, PLAN_RANGE, PLAN_SCHEDULE = 0, 1, 2
+PLAN_AUTO, PLAN_RANGE, PLAN_SCHEDULE, PLAN_LEVELS = 0, 1, 2, 3
 EXPORTS = {
     "group_ptr": 0, "group_set": 1, "level_ptr": 2, "perm_to_orig": 3, "node_to_level": 4,
     "nthreads": 5, "merged_ptr": 6, "row_level_ptr": 7, "row_perm": 8, "row_level": 9,
@@ -55,6 +55,8 @@ _SIGS = {
     "baas_replan": (_i, [_P, _i, _i, _i]),
     "baas_plan_dataflow": (_i, [_P, _i]),
     "baas_autotune": (_i, [_P, _P, _P, _i, _i, _P]),
+    "baas_autotune_f32": (_i, [_P, _P, _P, _i, _i, _P]),
+    "baas_pcg_autotune": (_i, [_P, _P]),
     "baas_sptrsv_f64": (_i, [_P, _P, _i, _P, _P, _P, _i]),
     "baas_sptrsv_f32": (_i, [_P, _P, _i, _P, _P, _P, _i]),
     "baas_sptrsv_levelsync_f64": (_i, [_P, _P, _P, _P, _P]),

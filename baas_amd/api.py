@@ -118,7 +118,8 @@ class Schedule:
     def autotune(self, vals, uplo=0, nrhs=1):
         """Time candidate plans on these values and keep the fastest."""
         c = (C.c_int * 5)()
-        L.call("baas_autotune", self.h.ptr, self._s, _ptr(vals), int(uplo), int(nrhs), C.cast(c, C.c_void_p))
+        f = "baas_autotune_f32" if vals.dtype == torch.float32 else "baas_autotune"
+        L.call(f, self.h.ptr, self._s, _ptr(vals), int(uplo), int(nrhs), C.cast(c, C.c_void_p))
         return {"mode": c[0], "rows_per_unit": c[1], "block": c[2], "dataflow": c[3], "us": c[4]}
 
     def upper_pattern(self):
@@ -228,6 +229,12 @@ class PCG:
         a, b = C.c_double(), C.c_double()
         L.call("baas_pcg_times", self._p, C.byref(a), C.byref(b))
         return a.value, b.value
+
+    def autotune(self):
+        """Tune the schedule's plans for this solver's triangular sweeps."""
+        c = (C.c_int * 5)()
+        L.call("baas_pcg_autotune", self._p, C.cast(c, C.c_void_p))
+        return {"mode": c[0], "rows_per_unit": c[1], "block": c[2], "dataflow": c[3], "us": c[4]}
 
     def __del__(self):
         try:

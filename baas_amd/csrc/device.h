@@ -86,4 +86,27 @@ __device__ __forceinline__ void retire_block(const Ctx &c, int epoch) {
     }
 }
 
+// Device-wide barrier for persistent grids whose workgroups are all
+// resident: each workgroup counts itself in on a monotonic counter and waits
+// (bounded) until the count reaches target.  Data crossing the barrier must
+// be written with st_agent and read with ld_agent: the barrier only waits
+// for this workgroup's stores to complete before counting in, so no cache
+// write-back or invalidation (which would cost the whole L2) is needed.
+__device__ __forceinline__ void grid_barrier(int *bar, int target, DevStatus *status, long long limit) {
+    wait_stores();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        atomicAdd(bar, 1);
+        long long spins = 0;
+        while (ld_agent(bar) < target) {
+            if (++spins > limit) {
+                raise_status(status, BAAS_TIMEOUT, -1);
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    __syncthreads();
+}
+
 }  // namespace baas

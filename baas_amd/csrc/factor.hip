@@ -14,6 +14,8 @@
 // Values another row reads are stored with agent-scope write-through stores
 // and read with agent-scope loads, so rows of the same unit (after the
 // sub-level barrier) and of earlier units (after the ready flag) see them.
+#include <algorithm>
+
 #include "device.h"
 #include "internal.h"
 
@@ -141,6 +143,32 @@ __global__ __launch_bounds__(1024) void k_factor(FactorArgs a, double *lu) {
     retire_block(a.ctx, epoch);
 }
 
+// Level-set factorisation (LEVELS plans): all workgroups share each row
+// level, a device-wide barrier separates levels (the reference's level-set
+// kernels, merge/IC.cpp:209-281 and merge/ILU.cpp:578-625).
+template <int KIND>
+__global__ __launch_bounds__(1024) void k_factor_levels(FactorArgs a, int nlev, const int *__restrict__ lptr,
+                                                       const int4 *__restrict__ meta, int *bar, double *lu) {
+    const int epoch = read_epoch(a.ctx);
+    const int nth = (int)(gridDim.x * blockDim.x);
+    const int gt = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    for (int l = 0; l < nlev; l++) {
+        for (int q = lptr[l] + gt; q < lptr[l + 1]; q += nth) factor_row<KIND>(a, meta[q].x, lu);
+        if (l + 1 < nlev) grid_barrier(bar, (l + 1) * (int)gridDim.x, a.status, a.spin_limit);
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const int d = atomicAdd(a.ctx.done, 1);
+        if (d == (int)gridDim.x - 1) {
+            atomicExch(bar, 0);
+            atomicExch(a.ctx.ticket, 0);
+            atomicExch(a.ctx.epoch, epoch == 0x7fffffff ? 1 : epoch + 1);
+            __threadfence();
+            atomicExch(a.ctx.done, 0);
+        }
+    }
+}
+
 __global__ void k_copy_f64(int n, const double *__restrict__ in, double *__restrict__ out) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = in[i];
@@ -194,6 +222,21 @@ static int factor(baas_handle *h, baas_schedule *s, const double *a_vals, double
     a.ctx = Ctx{h->counters, h->counters + 1, h->counters + 2};
     a.status = h->status;
     a.spin_limit = h->spin_limit;
+    if (p.mode == BAAS_PLAN_LEVELS) {
+        auto kern = k_factor_levels<KIND>;
+        int per_cu = 0;
+        BAAS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, p.block, 0));
+        if (per_cu <= 0) return BAAS_UNSUPPORTED;
+        int widest = 1;
+        for (int l = 0; l + 1 < (int)s->row_level_ptr.size(); l++)
+            widest = std::max(widest, s->row_level_ptr[l + 1] - s->row_level_ptr[l]);
+        long long grid = std::min<long long>((long long)per_cu * h->num_cus, (widest + p.block - 1) / p.block);
+        if (grid < 1) grid = 1;
+        hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(p.block), 0, h->stream, a, p.nlev, p.lptr.p,
+                           reinterpret_cast<const int4 *>(p.meta.p), h->counters + 3, lu);
+        BAAS_HIP_TRY(hipGetLastError());
+        return BAAS_OK;
+    }
     switch (p.keymode) {
         case 0: return launch_factor<KIND, 0>(h, p, a, lu);
         case 1: return launch_factor<KIND, 1>(h, p, a, lu);

@@ -92,6 +92,9 @@ struct Plan {
     DBuf<int> key_row;    // [n]        row of each key (keymode 2 only)
     int max_slots = 0;    // max over units of keys + external parents
     int max_deg = 0;      // max off-diagonal entries of a row
+    // LEVELS plans: row level sets in execution order (meta holds the rows)
+    DBuf<int> lptr;       // [nlev+1]
+    int nlev = 0;
     int dataflow = 0;     // use the dataflow kernel for single-RHS sweeps
     DBuf<unsigned char> xp_raw;  // 2 x n published values (dataflow kernel)
     int xp_type = 0;             // sizeof the value type the halves are armed for
@@ -106,6 +109,8 @@ struct Plan {
     std::vector<int> h_rows;      // host copy of rows (without flag bit)
 };
 
+template <class T>
+int autotune(baas_handle *h, baas_schedule *s, const T *vals, int uplo, int nrhs, int *chosen);
 }  // namespace baas
 
 struct baas_handle {
@@ -139,6 +144,7 @@ struct baas_schedule {
     int ngroups = 0, nlevels = 0, nmerged = 0, reduced_edges = 0, nrowlevels = 0;
     // plans (forward over L, backward over U) and ready flags
     baas::Plan fwd, bwd;
+    int plan_gen = 0;       // bumped whenever the plans change (captured graphs re-capture)
     baas::DBuf<int> flags;  // [n] epoch of the last sweep that finished the row
     baas::DBuf<int> row_level_ptr_d, row_perm_d;  // for the level-synchronous solve
     double t_gpu = 0, t_host = 0;

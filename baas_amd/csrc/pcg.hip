@@ -143,6 +143,7 @@ struct baas_pcg {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     bool factored = false;
+    int cap_gen = -1;  // schedule plan generation the graph was captured with
     double t_ic = 0, t_iter = 0;
 };
 
@@ -183,7 +184,14 @@ static int iteration(baas_pcg *P, int cur) {
 }
 
 static int capture(baas_pcg *P) {
-    if (P->exec) return BAAS_OK;
+    if (P->exec && P->cap_gen == P->s->plan_gen) return BAAS_OK;
+    // the graph holds the plans' kernels and buffers: re-capture after any
+    // change of plan
+    if (P->exec) (void)hipGraphExecDestroy(P->exec);
+    if (P->graph) (void)hipGraphDestroy(P->graph);
+    P->exec = nullptr;
+    P->graph = nullptr;
+    P->cap_gen = P->s->plan_gen;
     BAAS_HIP_TRY(hipStreamBeginCapture(P->st, hipStreamCaptureModeThreadLocal));
     int rc = BAAS_OK;
     for (int t = 0; t < P->chunk && rc == BAAS_OK; t++) rc = iteration(P, t & 1);
@@ -324,6 +332,20 @@ extern "C" int baas_pcg_solve(baas_pcg *P, const double *b, double *x, double to
     if (iters) *iters = P->h_state[ST_DONE] == 1 ? P->h_state[ST_FIRST] : P->h_state[ST_ITER];
     if (relres) *relres = rel;
     return P->h_state[ST_DONE] == 1 ? BAAS_OK : BAAS_NOT_CONVERGED;
+}
+
+extern "C" int baas_pcg_autotune(baas_pcg *P, int *chosen) {
+    if (!P) return BAAS_INVALID_ARG;
+    baas_handle *h = P->h;
+    BAAS_HIP_TRY(hipEventRecord(P->ev, h->stream));
+    BAAS_HIP_TRY(hipStreamWaitEvent(P->st, P->ev, 0));
+    if (!P->factored) BAAS_TRY(factorize(P));
+    hipStream_t keep = h->stream;
+    h->stream = P->st;
+    const int rc = P->precision == 1 ? autotune<float>(h, P->s, P->lv32.p, 0, 1, chosen)
+                                     : autotune<double>(h, P->s, P->lv.p, 0, 1, chosen);
+    h->stream = keep;
+    return rc;
 }
 
 extern "C" int baas_pcg_times(const baas_pcg *P, double *ic_s, double *iter_s) {

@@ -236,12 +236,54 @@ int make_plan(Plan &p, int n, const std::vector<int> &order, const std::vector<i
 
 // Strip the diagonal: parents of row r in L (lower CSR, diag last) and in
 // U = L^T (upper CSR, diag first).
+// LEVELS plan: rows in (level, id) order with a 16-byte record each (row,
+// first off-diagonal entry, off-diagonal count), and the level pointer.  The
+// backward plan walks the same levels in reverse over U.
+static int make_levels_plan(Plan &p, const baas_schedule *s, const HostCsr &M, bool upper, int block,
+                            hipStream_t st) {
+    const int n = s->n;
+    const auto &perm = s->row_perm, &lp = s->row_level_ptr;
+    if ((int)perm.size() != n || lp.empty() || lp.back() != n) return BAAS_INVALID_ARG;
+    std::vector<int> meta(4 * (size_t)n);
+    int max_deg = 0;
+    for (int q = 0; q < n; q++) {
+        const int r = perm[q], rb = M.ptr[r], re = M.ptr[r + 1];
+        meta[4 * (size_t)q + 0] = r;
+        meta[4 * (size_t)q + 1] = upper ? re - 1 : rb;
+        meta[4 * (size_t)q + 2] = re - rb - 1;
+        meta[4 * (size_t)q + 3] = 0;
+        max_deg = std::max(max_deg, re - rb - 1);
+    }
+    BAAS_TRY(upload(p.meta, meta, st));
+    BAAS_TRY(upload(p.lptr, lp, st));
+    p.nlev = (int)lp.size() - 1;
+    p.n = n;
+    p.nunits = p.nlev;
+    p.max_deg = max_deg;
+    p.mode = BAAS_PLAN_LEVELS;
+    p.block = block;
+    p.keymode = 0;
+    p.unitdep = 0;
+    p.max_slots = 0;
+    p.df_lds[0] = p.df_lds[1] = 0;
+    p.max_unit_keys = 0;
+    if (hipStreamSynchronize(st) != hipSuccess) return BAAS_HIP_ERROR;
+    return BAAS_OK;
+}
+
 int build_plans_mode(baas_schedule *s, int mode, int rows_per_unit) {
+    s->plan_gen++;
     hipStream_t st = s->h->stream;
     const int n = s->n;
     HostCsr L, U;
     BAAS_TRY(fetch(L, s->l_rowptr.p, s->l_colidx.p, n, st));
     BAAS_TRY(fetch(U, s->u_rowptr.p, s->u_colidx.p, n, st));
+    if (mode == BAAS_PLAN_LEVELS) {
+        const int block = s->opts.block_threads > 0 ? s->opts.block_threads : 256;
+        BAAS_TRY(make_levels_plan(s->fwd, s, L, false, block, st));
+        BAAS_TRY(make_levels_plan(s->bwd, s, U, true, block, st));
+        return BAAS_OK;
+    }
     std::vector<int> lp(n + 1), li, up(n + 1), ui;
     li.reserve(L.idx.size());
     ui.reserve(U.idx.size());
@@ -293,6 +335,7 @@ int build_plans_mode(baas_schedule *s, int mode, int rows_per_unit) {
 // RANGE plans over caller-chosen unit boundaries (row space, ascending);
 // the backward plan uses the same units in reverse order.
 int make_plan_custom(baas_schedule *s, const std::vector<int> &fu, int block) {
+    s->plan_gen++;
     hipStream_t st = s->h->stream;
     const int n = s->n;
     if (fu.empty() || fu.front() != 0 || fu.back() != n) return BAAS_INVALID_ARG;
@@ -340,7 +383,7 @@ int build_plans(baas_schedule *s) {
 }  // namespace baas
 
 extern "C" int baas_replan(baas_schedule *s, int mode, int rows_per_unit, int block_threads) {
-    if (!s || mode < 1 || mode > 2 || rows_per_unit < 0 || block_threads < 0 || block_threads > 1024 ||
+    if (!s || mode < 1 || mode > 3 || rows_per_unit < 0 || block_threads < 0 || block_threads > 1024 ||
         (block_threads % 64) != 0)
         return BAAS_INVALID_ARG;
     if (block_threads) s->opts.block_threads = block_threads;

@@ -548,12 +548,112 @@ static int launch_df(baas_handle *h, const Plan &p, const DfArgs &a, const T *va
     return BAAS_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Level-set sweep (LEVELS plans).  One persistent launch walks the row level
+// sets; every workgroup takes a grid-stride share of each level's (row, rhs)
+// items, and a device-wide barrier separates levels (merge/SpTRSV.cpp:41-80
+// with the OpenMP barrier replaced).  Rows of one level are independent, so
+// no per-row flag is needed; x is written through and read at device scope,
+// so the previous levels' values are visible once the barrier is passed.  The barrier counter lives in the handle's
+// counter block and is reset by the last workgroup out.
+struct LvArgs {
+    int n, nlev, nrhs;
+    const int *lptr;
+    const int4 *meta;  // per position: row, first off-diagonal entry, count
+    const int *colidx;
+    Ctx ctx;
+    int *bar;
+    const int *skip;
+    DevStatus *status;
+    long long spin_limit;
+};
+
+template <class T, bool UPPER>
+__global__ __launch_bounds__(1024) void k_sweep_levels(LvArgs a, const T *__restrict__ vals, const T *__restrict__ b,
+                                                      T *x) {
+    if (a.skip && ld_agent(a.skip)) return;
+    const int epoch = read_epoch(a.ctx);
+    const int nrhs = a.nrhs;
+    const long long nth = (long long)gridDim.x * blockDim.x;
+    const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (int li = 0; li < a.nlev; li++) {
+        const int l = UPPER ? a.nlev - 1 - li : li;
+        const int qb = a.lptr[l], qe = a.lptr[l + 1];
+        const long long items = (long long)(qe - qb) * nrhs;
+        for (long long t = gt; t < items; t += nth) {
+            const int q = qb + (int)(t / nrhs);
+            const int r = (int)(t - (long long)(q - qb) * nrhs);
+            const int4 m = a.meta[q];
+            const int row = m.x, j0 = m.y, cnt = m.z;
+            T s = T(0);
+            if (UPPER) {
+                for (int k = 0; k < cnt; k++) {
+                    const int j = j0 - k;
+                    s = fma(ld_agent(&x[(size_t)a.colidx[j] * nrhs + r]), vals[j], s);
+                }
+            } else {
+                for (int k = 0; k < cnt; k++) {
+                    const int j = j0 + k;
+                    s = fma(vals[j], ld_agent(&x[(size_t)a.colidx[j] * nrhs + r]), s);
+                }
+            }
+            const int dg = UPPER ? j0 - cnt : j0 + cnt;
+            st_agent(&x[(size_t)row * nrhs + r], (b[(size_t)row * nrhs + r] - s) / vals[dg]);
+        }
+        if (li + 1 < a.nlev) grid_barrier(a.bar, (li + 1) * (int)gridDim.x, a.status, a.spin_limit);
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const int d = atomicAdd(a.ctx.done, 1);
+        if (d == (int)gridDim.x - 1) {
+            atomicExch(a.bar, 0);
+            atomicExch(a.ctx.ticket, 0);
+            atomicExch(a.ctx.epoch, epoch == 0x7fffffff ? 1 : epoch + 1);
+            __threadfence();
+            atomicExch(a.ctx.done, 0);
+        }
+    }
+}
+
+template <class T>
+static int launch_levels(baas_handle *h, baas_schedule *s, const Plan &p, int uplo, const T *vals, const T *b, T *x,
+                         int nrhs, hipStream_t stream, const int *skip) {
+    LvArgs a;
+    a.n = s->n;
+    a.nlev = p.nlev;
+    a.nrhs = nrhs;
+    a.lptr = p.lptr.p;
+    a.meta = reinterpret_cast<const int4 *>(p.meta.p);
+    a.colidx = uplo ? s->u_colidx.p : s->l_colidx.p;
+    a.ctx = Ctx{h->counters, h->counters + 1, h->counters + 2};
+    a.bar = h->counters + 3;
+    a.skip = skip;
+    a.status = h->status;
+    a.spin_limit = h->spin_limit;
+    auto kern = uplo ? k_sweep_levels<T, true> : k_sweep_levels<T, false>;
+    int per_cu = 0;
+    BAAS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, p.block, 0));
+    if (per_cu <= 0) return BAAS_UNSUPPORTED;
+    // every workgroup must be resident for the barrier; size the grid to
+    // the widest level
+    int widest = 1;
+    for (int l = 0; l + 1 < (int)s->row_level_ptr.size(); l++)
+        widest = std::max(widest, s->row_level_ptr[l + 1] - s->row_level_ptr[l]);
+    long long need = ((long long)widest * nrhs + p.block - 1) / p.block;
+    long long grid = std::min<long long>((long long)per_cu * h->num_cus, need);
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(p.block), 0, stream, a, vals, b, x);
+    BAAS_HIP_TRY(hipGetLastError());
+    return BAAS_OK;
+}
+
 template <class T>
 int sweep(baas_handle *h, baas_schedule *s, int uplo, const T *vals, const T *b, T *x, int nrhs,
           hipStream_t stream, const int *skip) {
     if (!h || !s || !vals || !b || !x || nrhs < 1 || (uplo != 0 && uplo != 1)) return BAAS_INVALID_ARG;
     if ((const void *)b == (const void *)x) return BAAS_INVALID_ARG;
     Plan &p = uplo ? s->bwd : s->fwd;
+    if (p.mode == BAAS_PLAN_LEVELS) return launch_levels<T>(h, s, p, uplo, vals, b, x, nrhs, stream, skip);
     if (nrhs == 1 && p.dataflow && p.block >= 192 && p.block <= 512 && p.df_lds[sizeof(T) == 8] <= lds_limit(h)) {
         // publication halves, armed with this value type's sentinel
         if (p.xp_type != (int)sizeof(T)) {
@@ -690,5 +790,6 @@ extern "C" int baas_csr_outer_grad_f64(baas_handle *h, int m, const int *rp, con
 extern "C" int baas_plan_dataflow(baas_schedule *s, int enable) {
     if (!s) return BAAS_INVALID_ARG;
     s->fwd.dataflow = s->bwd.dataflow = enable ? 1 : 0;
+    s->plan_gen++;
     return BAAS_OK;
 }

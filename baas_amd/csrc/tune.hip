@@ -4,10 +4,10 @@
 // compile time (merge/LevelMerge.cpp:118-161, merge/include/MergeMarco.hpp).
 // Here the packing is chosen per matrix by timing real sweeps: RANGE plans
 // over several rows-per-unit and workgroup sizes, with and without the
-// dataflow kernel, and SCHEDULE plans (units cut from the merged levels,
-// waiting per unit) over several rows-per-unit.  The fastest plan stays on
-// the schedule.  Values need not be the final ones: any non-zero diagonal
-// gives the same timing.
+// dataflow kernel, SCHEDULE plans (units cut from the merged levels) over
+// several rows-per-unit, and LEVELS plans (level sets with device-wide
+// barriers).  The fastest plan stays on the schedule.  Values need not be
+// the final ones: any non-zero diagonal gives the same timing.
 #include <cstdio>
 #include <vector>
 
@@ -27,29 +27,38 @@ struct Cand {
     int mode, rpu, block, df;
 };
 
-__global__ void k_fill_d(int n, double *p, double v) {
+template <class T>
+__global__ void k_fill(int n, T *p, T v) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
 }
 
 }  // namespace
 
-extern "C" int baas_autotune(baas_handle *h, baas_schedule *s, const double *vals, int uplo, int nrhs, int *chosen) {
+namespace baas {
+
+// Time every candidate plan for value type T and keep the fastest.  FP32
+// sweeps need half the LDS per row, so their dataflow units may be larger.
+template <class T>
+int autotune(baas_handle *h, baas_schedule *s, const T *vals, int uplo, int nrhs, int *chosen) {
     if (!h || !s || !vals || nrhs < 1 || (uplo != 0 && uplo != 1)) return BAAS_INVALID_ARG;
     const int n = s->n;
-    DBuf<double> b, x;
+    DBuf<T> b, x;
     BAAS_TRY(b.alloc((size_t)n * nrhs));
     BAAS_TRY(x.alloc((size_t)n * nrhs));
-    hipLaunchKernelGGL(k_fill_d, dim3((n * nrhs + 255) / 256), dim3(256), 0, h->stream, n * nrhs, b.p, 1.0);
+    hipLaunchKernelGGL(k_fill<T>, dim3((n * nrhs + 255) / 256), dim3(256), 0, h->stream, n * nrhs, b.p, T(1));
     std::vector<Cand> cands;
-    for (int rpu : {1024, 2048, 4096, 8192, 12288, 16384})
+    std::vector<int> rpus = {1024, 2048, 4096, 8192, 12288, 16384};
+    if (sizeof(T) == 4) rpus.insert(rpus.end(), {24576, 32768});
+    for (int rpu : rpus)
         for (int blk : {256, 384, 512, 1024}) {
             if (blk == 1024 && rpu < 4096) continue;
-            cands.push_back({BAAS_PLAN_RANGE, rpu, blk, 0});
+            if (rpu <= 16384) cands.push_back({BAAS_PLAN_RANGE, rpu, blk, 0});
             if (nrhs == 1 && blk <= 512) cands.push_back({BAAS_PLAN_RANGE, rpu, blk, 1});
         }
     for (int rpu : {256, 1024, 4096})
         for (int blk : {256, 1024}) cands.push_back({BAAS_PLAN_SCHEDULE, rpu, blk, 0});
+    for (int blk : {64, 128, 256, 512}) cands.push_back({BAAS_PLAN_LEVELS, 0, blk, 0});
     hipEvent_t e0, e1;
     BAAS_HIP_TRY(hipEventCreate(&e0));
     BAAS_HIP_TRY(hipEventCreate(&e1));
@@ -60,13 +69,13 @@ extern "C" int baas_autotune(baas_handle *h, baas_schedule *s, const double *val
         s->opts.block_threads = c.block;
         if (build_plans_mode(s, c.mode, c.rpu) != BAAS_OK) continue;
         Plan &p = uplo ? s->bwd : s->fwd;
-        if (c.df && p.df_lds[1] > lds_limit(h)) continue;
+        if (c.df && p.df_lds[sizeof(T) == 8] > lds_limit(h)) continue;
         s->fwd.dataflow = s->bwd.dataflow = c.df;
         float t_best = 1e30f;
         bool ok = true;
         for (int rep = 0; rep < 4 && ok; rep++) {
             (void)hipEventRecord(e0, h->stream);
-            ok = sweep<double>(h, s, uplo, vals, b.p, x.p, nrhs, h->stream, nullptr) == BAAS_OK;
+            ok = sweep<T>(h, s, uplo, vals, b.p, x.p, nrhs, h->stream, nullptr) == BAAS_OK;
             (void)hipEventRecord(e1, h->stream);
             (void)hipEventSynchronize(e1);
             float ms = 0;
@@ -90,4 +99,18 @@ extern "C" int baas_autotune(baas_handle *h, baas_schedule *s, const double *val
         chosen[4] = (int)(best * 1000.0f);  // microseconds of the winning sweep
     }
     return baas_check(h) == BAAS_OK ? BAAS_OK : BAAS_HIP_ERROR;
+}
+
+template int autotune<double>(baas_handle *, baas_schedule *, const double *, int, int, int *);
+template int autotune<float>(baas_handle *, baas_schedule *, const float *, int, int, int *);
+
+}  // namespace baas
+
+extern "C" int baas_autotune(baas_handle *h, baas_schedule *s, const double *vals, int uplo, int nrhs, int *chosen) {
+    return baas::autotune<double>(h, s, vals, uplo, nrhs, chosen);
+}
+
+extern "C" int baas_autotune_f32(baas_handle *h, baas_schedule *s, const float *vals, int uplo, int nrhs,
+                                 int *chosen) {
+    return baas::autotune<float>(h, s, vals, uplo, nrhs, chosen);
 }

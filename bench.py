@@ -208,6 +208,7 @@ def main():
         res = {}
         for prec, name in ((0, "fp64"), (1, "fp32_factor")):
             solver = PCG(h, sk, kv, precision=prec)
+            pc_plan = solver.autotune()  # plans tuned for this precision's sweeps
             solver.solve(kbd, tol=1e-8, maxiter=5000)  # warm-up: factor, capture the graph
             ts, its = [], 0
             for _ in range(5):
@@ -216,7 +217,7 @@ def main():
             ic_s, _ = solver.times()
             solver.solve(kbd, tol=1e-8, maxiter=5000, refactor=True)
             res[name] = {"iterations": its, "time_to_1e-8_s": round(float(np.median(ts)), 6),
-                         "ic0_s": round(solver.times()[0], 6), "relres": rel}
+                         "ic0_s": round(solver.times()[0], 6), "relres": rel, "plan": pc_plan}
         res["speedup_fp32_factor"] = round(res["fp64"]["time_to_1e-8_s"] / res["fp32_factor"]["time_to_1e-8_s"], 3)
         extra["iccg_knn36000"] = res
         out["extra"] = extra

@@ -65,7 +65,11 @@ enum baas_export {
 };
 
 /* Execution-plan choice (the autotuner fills these when mode = AUTO). */
-enum baas_plan_mode { BAAS_PLAN_AUTO = 0, BAAS_PLAN_RANGE = 1, BAAS_PLAN_SCHEDULE = 2 };
+/* LEVELS: one persistent launch walks the row level sets, all workgroups
+ * sharing each level, with a device-wide barrier between levels: the GPU
+ * form of the reference's level-set solve (merge/SpTRSV.cpp:41-80), best
+ * when levels are few and wide. */
+enum baas_plan_mode { BAAS_PLAN_AUTO = 0, BAAS_PLAN_RANGE = 1, BAAS_PLAN_SCHEDULE = 2, BAAS_PLAN_LEVELS = 3 };
 
 typedef struct baas_opts {
     int ncore;            /* thread count the reference rule is evaluated at (default 8) */
@@ -176,6 +180,10 @@ int baas_pcg_solve(baas_pcg *p, const double *b, double *x, double tol, int maxi
                    int refactor, int *iters, double *relres);
 /* Seconds of the last solve split into IC0, solve loop (host). */
 int baas_pcg_times(const baas_pcg *p, double *ic_s, double *iter_s);
+/* Factor if needed, then autotune the schedule's plans for this PCG's
+ * triangular sweeps in its precision (no reference counterpart: the
+ * reference fixes its schedule at analysis).  chosen as baas_autotune. */
+int baas_pcg_autotune(baas_pcg *p, int *chosen);
 
 /* ---- batched solves -------------------------------------------------------
  * nsys independent SPD systems packed back to back (block-diagonal CSR of
@@ -206,6 +214,8 @@ int baas_plan_dataflow(baas_schedule *s, int enable);
  * size and the dataflow kernel, keeps the fastest.  chosen (host int[5], may
  * be NULL) = mode, rows per unit, threads per workgroup, dataflow, microseconds. */
 int baas_autotune(baas_handle *h, baas_schedule *s, const double *vals, int uplo, int nrhs, int *chosen);
+/* The same for FP32 sweeps (whose dataflow units may hold twice the rows). */
+int baas_autotune_f32(baas_handle *h, baas_schedule *s, const float *vals, int uplo, int nrhs, int *chosen);
 
 #ifdef __cplusplus
 }

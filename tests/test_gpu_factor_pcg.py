@@ -38,7 +38,7 @@ def test_ic0_bit_exact(handle, name):
     want = O.ic0(A)
     s = Schedule(handle, A.rowptr, A.colidx, kind=_lib.KIND_IC0)
     av = up(A.values)
-    for mode, rpu, blk in [(1, 0, 256), (2, 0, 256), (2, 64, 64), (1, 1024, 512)]:
+    for mode, rpu, blk in [(1, 0, 256), (2, 0, 256), (2, 64, 64), (1, 1024, 512), (3, 0, 256)]:
         announce("ic0", name, mode, rpu, blk)
         s.replan(mode, rpu, blk)
         lu = s.ic0(av)
@@ -70,7 +70,7 @@ def test_ilu0_bit_exact(handle, name):
     _, _, A = workload(name)
     want = O.ilu0(A)
     s = Schedule(handle, A.rowptr, A.colidx, kind=_lib.KIND_ILU0)
-    for mode in (1, 2):
+    for mode in (1, 2, 3):
         announce("ilu0", name, mode)
         s.replan(mode, 0, 256)
         lu = s.ilu0(up(A.values))
@@ -156,3 +156,26 @@ def test_pcg_mixed_error_vs_fp64(handle):
     err = np.linalg.norm(down(x32) - down(x64)) / np.linalg.norm(down(x64))
     print(f"[pcg] fp64 iters {k64}, fp32-factor iters {k32}, rel. difference of solutions {err:.3e}")
     assert err < 1e-6
+
+
+def test_pcg_autotune_keeps_iterates(handle):
+    """Re-planning under a captured PCG graph: the solver re-captures, and
+    since every plan gives the same sweep bits the iterates are unchanged."""
+    import torch
+    from baas_amd.api import PCG, Schedule
+    _, _, A = workload("C5_diff3d_48")
+    b = g.rhs_uniform(A.n, 5)
+    s = Schedule(handle, A.rowptr, A.colidx, kind=_lib.KIND_IC0)
+    av, bd = up(A.values), up(b)
+    for prec in (0, 1):
+        announce("pcg autotune", prec)
+        s.replan(1, 4096, 256)
+        solver = PCG(handle, s, av, precision=prec)
+        x0, it0, _ = solver.solve(bd, tol=1e-8, maxiter=2000)
+        chosen = solver.autotune()
+        x1, it1, _ = solver.solve(bd, tol=1e-8, maxiter=2000)
+        handle.check()
+        assert it0 == it1, (prec, it0, it1, chosen)
+        np.testing.assert_array_equal(down(x0), down(x1))
+        del solver
+        torch.cuda.synchronize()

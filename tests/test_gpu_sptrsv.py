@@ -64,7 +64,8 @@ def test_reduced_dag_matches_oracle(handle):
 # (plan mode, rows per unit, threads per workgroup, dataflow kernel)
 PLANS = [(1, 0, 256, False), (1, 1024, 256, False), (1, 16384, 512, False), (2, 0, 256, False),
          (2, 64, 64, False), (2, 1024, 1024, False), (1, 4096, 256, True), (1, 12288, 512, True),
-         (1, 2048, 192, True), (2, 256, 256, True), (2, 2048, 512, True), (1, 8192, 1024, True)]
+         (1, 2048, 192, True), (2, 256, 256, True), (2, 2048, 512, True), (1, 8192, 1024, True),
+         (3, 0, 256, False), (3, 0, 1024, False)]
 SOLVE_CASES = ["C1_lap2d_128", "C2_lap3d_64", "C2_lap3d_64_perm1", "C5_diff3d_48", "bcsstk18", "bcsstk38"]
 
 
@@ -115,7 +116,7 @@ def test_multi_rhs_bit_exact(handle, k):
         s = Schedule(handle, L.rowptr, L.colidx, kind=0)
         B = g.rhs_uniform(L.n, 3, k=k)
         X_ref = O.sptrsv_lower_multi(L, B)
-        for mode, rpu, blk in [(1, 0, 256), (1, 256, 256), (2, 0, 256)]:
+        for mode, rpu, blk in [(1, 0, 256), (1, 256, 256), (2, 0, 256), (3, 0, 512)]:
             announce("multi", name, k, mode, rpu, blk)
             s.replan(mode, rpu, blk)
             X = down(s.solve(up(L.values), up(B)))
@@ -135,8 +136,8 @@ def test_f32_bit_exact(handle):
         L, _, _ = workload(name)
         s = Schedule(handle, L.rowptr, L.colidx, kind=0)
         b = g.rhs_uniform(L.n, 1)
-        for df in (False, True):
-            s.replan(1, 4096, 256, dataflow=df)
+        for mode, df in ((1, False), (1, True), (3, False)):
+            s.replan(mode, 4096 if mode == 1 else 0, 256, dataflow=df)
             x = down(s.solve(up(L.values, torch.float32), up(b, torch.float32)))
             handle.check()
             np.testing.assert_array_equal(x, O.sptrsv_lower_f32(L, b))
@@ -209,7 +210,7 @@ def test_autotune_keeps_bits(handle, name):
         vals = lv if uplo == 0 else s.transpose_values(lv)
         chosen = s.autotune(vals, uplo=uplo)
         print("[autotune]", name, uplo, chosen)
-        assert chosen["mode"] in (1, 2) and chosen["us"] > 0
+        assert chosen["mode"] in (1, 2, 3) and chosen["us"] > 0
         b = g.rhs_uniform(L.n, 0)
         got = down(s.solve(vals, up(b), uplo=uplo))
         handle.check()
