@@ -265,8 +265,15 @@ __device__ __forceinline__ void df_incoming(const DfArgs &a, const T *pub, T *sl
 // value is the flag), and re-arm the same rows in the other half for the
 // next sweep.  Compute waves therefore issue no global stores.
 template <class T>
+__device__ __forceinline__ int df_row_of(const DfArgs &a, int kb, int slot) {
+    const int key = kb + slot;
+    return a.keymode == 0 ? key : a.keymode == 1 ? a.n - 1 - key : a.key_row[key];
+}
+
+template <class T>
 __device__ __forceinline__ void df_outgoing(const DfArgs &a, T *pub, T *nxt, const T *sl, const int *outs, int no,
                                             int kb, int lane) {
+    bool gave_up = false;
     for (int o = lane; o < no; o += DF_PF * 64) {
         int sk[DF_PF];
 #pragma unroll
@@ -279,7 +286,7 @@ __device__ __forceinline__ void df_outgoing(const DfArgs &a, T *pub, T *nxt, con
         for (int k = 0; k < DF_PF; k++)
             if (o + k * 64 < no) todo |= 1u << k;
         long long spins = 0;
-        while (todo) {
+        while (todo && !gave_up) {
             T v[DF_PF];
 #pragma unroll
             for (int k = 0; k < DF_PF; k++) v[k] = ld_lds(&sl[sk[k]]);
@@ -287,8 +294,7 @@ __device__ __forceinline__ void df_outgoing(const DfArgs &a, T *pub, T *nxt, con
 #pragma unroll
             for (int k = 0; k < DF_PF; k++) {
                 if (((todo >> k) & 1u) && !Sentinel<T>::is(v[k])) {
-                    const int key = kb + sk[k];
-                    const int row = a.keymode == 0 ? key : a.keymode == 1 ? a.n - 1 - key : a.key_row[key];
+                    const int row = df_row_of<T>(a, kb, sk[k]);
                     st_agent(&pub[row], v[k]);
                     nxt[row] = Sentinel<T>::value();
                     got |= 1u << k;
@@ -297,11 +303,19 @@ __device__ __forceinline__ void df_outgoing(const DfArgs &a, T *pub, T *nxt, con
             todo &= ~got;
             if (todo && !got) {
                 if (++spins > a.spin_limit) {
-                    todo = 0;  // the unit's own rows are stuck: its compute waves report it
+                    gave_up = true;  // the unit's own rows are stuck: its compute waves report it
                 } else {
                     __builtin_amdgcn_s_sleep(1);
                 }
             }
+        }
+        // After giving up, still re-arm the other half for every row left,
+        // so the next sweep starts from sentinels (its own half must hold
+        // no value from two sweeps back).
+        if (gave_up) {
+#pragma unroll
+            for (int k = 0; k < DF_PF; k++)
+                if ((todo >> k) & 1u) nxt[df_row_of<T>(a, kb, sk[k])] = Sentinel<T>::value();
         }
     }
 }

@@ -165,26 +165,32 @@ def test_division_and_sqrt_are_ieee(handle):
 
 
 def test_watchdog_reports_timeout_without_hanging(handle):
-    """A one-poll spin limit forces the wait to give up: the kernel still exits
-    and the error is reported (the values are then meaningless)."""
+    """A one-poll spin limit forces every kind of wait (per-row flags,
+    dataflow exchange and LDS polls, level barriers) to give up: the kernel
+    still exits and the error is reported (the values are then meaningless),
+    and once the limit is restored the next solves are exact again."""
     from baas_amd import _lib
     from baas_amd.api import Schedule
-    announce("watchdog")
     L, _, _ = workload("C2_lap3d_64_perm1")
     s = Schedule(handle, L.rowptr, L.colidx, kind=0, plan_mode=2)
     b = up(g.rhs_uniform(L.n, 0))
-    handle.set_spin_limit(1)
-    try:
-        for _ in range(4):
-            s.solve(up(L.values), b)
-        st = handle.status()
-    finally:
-        handle.set_spin_limit(1 << 22)
-    assert st in (_lib.OK, _lib.TIMEOUT)
-    # after restoring the limit the solve is exact again
-    x = down(s.solve(up(L.values), b))
-    handle.check()
-    np.testing.assert_array_equal(x, O.sptrsv_lower(L, down(b)))
+    want = O.sptrsv_lower(L, down(b))
+    lv = up(L.values)
+    for mode, rpu, blk, df in ((2, 0, 256, False), (1, 4096, 256, True), (3, 0, 256, False)):
+        announce("watchdog", mode, rpu, blk, df)
+        s.replan(mode, rpu, blk, dataflow=df)
+        handle.set_spin_limit(1)
+        try:
+            for _ in range(4):
+                s.solve(lv, b)
+            st = handle.status()
+        finally:
+            handle.set_spin_limit(1 << 22)
+        assert st in (_lib.OK, _lib.TIMEOUT)
+        for _ in range(3):  # both publication halves of the dataflow plan
+            x = down(s.solve(lv, b))
+            handle.check()
+            np.testing.assert_array_equal(x, want)
 
 
 def test_invalid_pattern_rejected(handle):
