@@ -20,6 +20,7 @@
 #include "internal.h"
 
 namespace baas {
+int build_plans_mode(baas_schedule *s, int mode, int rows_per_unit);
 
 int make_plan_custom(baas_schedule *s, const std::vector<int> &unit_ptr, int block);  // plan.cpp
 
@@ -189,6 +190,7 @@ struct baas_batch {
     hipGraph_t graph = nullptr;
     std::vector<int> tuned_rpu;
     bool tuned = false;
+    int cap_gen = -1;  // schedule plan generation the graph was captured with
     double t_tune = 0, t_ic = 0, t_iter = 0;
 };
 
@@ -234,7 +236,9 @@ static float time_sweeps(baas_batch *B, int reps) {
 }
 
 // Coordinate search over rows-per-unit, one size bucket at a time, timing
-// the FP32 forward + backward sweeps of the whole batch with all RHS.
+// the FP32 forward + backward sweeps of the whole batch with all RHS; then
+// the level-set plan (all systems' rows of one level at a time), which is
+// kept if faster.
 static int autotune(baas_batch *B) {
     auto t0 = std::chrono::steady_clock::now();
     const int cand[] = {512, 1024, 2048, 4096, 8192};
@@ -251,6 +255,23 @@ static int autotune(baas_batch *B) {
             float t = time_sweeps(B, 3);
             if (t < best) best = t, rpu = trial;
         }
+    }
+    int lev_block = 0;
+    for (int lb : {256, 512, 1024}) {
+        B->s->opts.block_threads = lb;
+        BAAS_TRY(build_plans_mode(B->s, BAAS_PLAN_LEVELS, 0));
+        const float t_lev = time_sweeps(B, 3);
+        if (t_lev < best) best = t_lev, lev_block = lb;
+    }
+    B->s->opts.block_threads = block;
+    if (lev_block) {
+        B->s->opts.block_threads = lev_block;
+        BAAS_TRY(build_plans_mode(B->s, BAAS_PLAN_LEVELS, 0));
+        B->s->opts.block_threads = block;
+        B->tuned_rpu = {0, 0, lev_block};
+        B->tuned = true;
+        B->t_tune = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return BAAS_OK;
     }
     BAAS_TRY(make_plan_custom(B->s, batch_units(B, rpu), block));
     B->tuned_rpu = rpu;
@@ -293,7 +314,12 @@ static int batch_iteration(baas_batch *B, int cur) {
 }
 
 static int batch_capture(baas_batch *B) {
-    if (B->exec) return BAAS_OK;
+    if (B->exec && B->cap_gen == B->s->plan_gen) return BAAS_OK;
+    if (B->exec) (void)hipGraphExecDestroy(B->exec);
+    if (B->graph) (void)hipGraphDestroy(B->graph);
+    B->exec = nullptr;
+    B->graph = nullptr;
+    B->cap_gen = B->s->plan_gen;
     BAAS_HIP_TRY(hipStreamBeginCapture(B->st, hipStreamCaptureModeThreadLocal));
     int rc = BAAS_OK;
     for (int t = 0; t < B->chunk_iters && rc == BAAS_OK; t++) rc = batch_iteration(B, t & 1);
@@ -334,6 +360,9 @@ extern "C" int baas_batch_create(baas_handle *h, int nsys, const int *sys_ptr_ho
     if (opts) o = *opts;
     o.plan_mode = BAAS_PLAN_RANGE;
     int rc = baas_analyze(h, N, rowptr, colidx, BAAS_KIND_IC0, &o, &B->s);
+    // independent systems: the level-set plan runs every system's level at
+    // once (the first factorisation uses it; the autotuner may keep it)
+    if (rc == BAAS_OK) rc = build_plans_mode(B->s, BAAS_PLAN_LEVELS, 0);
     std::vector<int> sys_of(N), sys_chunk(nsys + 1);
     std::vector<Chunk> ch;
     for (int s = 0; s < nsys && rc == BAAS_OK; s++) {

@@ -28,6 +28,15 @@ __device__ __forceinline__ void raise_status(DevStatus *st, int code, int detail
     if (atomicCAS(&st->code, 0, code) == 0) st->detail = detail;
 }
 
+// Pause between polls: short at first, longer once a wait has lasted, so
+// thousands of waiting lanes do not flood the memory system with polls.
+__device__ __forceinline__ void poll_pause(long long spins) {
+    if (spins < 32)
+        __builtin_amdgcn_s_sleep(1);
+    else
+        __builtin_amdgcn_s_sleep(8);
+}
+
 // Spin until flags[c] == epoch; a watchdog turns a missing producer into
 // BAAS_TIMEOUT instead of a hang.
 __device__ __forceinline__ void wait_ready(const int *flags, int c, int epoch, long long limit,
@@ -38,7 +47,7 @@ __device__ __forceinline__ void wait_ready(const int *flags, int c, int epoch, l
             raise_status(st, BAAS_TIMEOUT, c);
             break;
         }
-        __builtin_amdgcn_s_sleep(1);
+        poll_pause(spins);
     }
 }
 
@@ -87,26 +96,44 @@ __device__ __forceinline__ void retire_block(const Ctx &c, int epoch) {
 }
 
 // Device-wide barrier for persistent grids whose workgroups are all
-// resident: each workgroup counts itself in on a monotonic counter and waits
-// (bounded) until the count reaches target.  Data crossing the barrier must
-// be written with st_agent and read with ld_agent: the barrier only waits
-// for this workgroup's stores to complete before counting in, so no cache
-// write-back or invalidation (which would cost the whole L2) is needed.
+// resident: each workgroup counts itself in on one of eight partial
+// counters (by workgroup id, each counter on its own cache line, so the
+// arrivals do not all serialise on one address) and waits (bounded) until
+// the eight sum to target.  Data crossing the barrier must be written with
+// st_agent and read with ld_agent: the barrier only waits for this
+// workgroup's stores to complete before counting in, so no cache write-back
+// or invalidation (which would cost the whole L2) is needed.
+__device__ __forceinline__ int barrier_count(int *bar) {
+    int v[8];
+#pragma unroll
+    for (int p = 0; p < 8; p++) v[p] = ld_agent(&bar[32 * p]);
+    int s = 0;
+#pragma unroll
+    for (int p = 0; p < 8; p++) s += v[p];
+    return s;
+}
+
 __device__ __forceinline__ void grid_barrier(int *bar, int target, DevStatus *status, long long limit) {
     wait_stores();
     __syncthreads();
     if (threadIdx.x == 0) {
-        atomicAdd(bar, 1);
+        atomicAdd(&bar[32 * (blockIdx.x & 7)], 1);
         long long spins = 0;
-        while (ld_agent(bar) < target) {
+        while (barrier_count(bar) < target) {
             if (++spins > limit) {
                 raise_status(status, BAAS_TIMEOUT, -1);
                 break;
             }
-            __builtin_amdgcn_s_sleep(1);
+            poll_pause(spins);
         }
     }
     __syncthreads();
+}
+
+// Reset the barrier's partial counters (the last workgroup out does this).
+__device__ __forceinline__ void reset_barrier(int *bar) {
+#pragma unroll
+    for (int p = 0; p < 8; p++) atomicExch(&bar[32 * p], 0);
 }
 
 }  // namespace baas

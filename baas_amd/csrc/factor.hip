@@ -160,7 +160,7 @@ __global__ __launch_bounds__(1024) void k_factor_levels(FactorArgs a, int nlev, 
         __threadfence();
         const int d = atomicAdd(a.ctx.done, 1);
         if (d == (int)gridDim.x - 1) {
-            atomicExch(bar, 0);
+            reset_barrier(bar);
             atomicExch(a.ctx.ticket, 0);
             atomicExch(a.ctx.epoch, epoch == 0x7fffffff ? 1 : epoch + 1);
             __threadfence();
@@ -230,10 +230,11 @@ static int factor(baas_handle *h, baas_schedule *s, const double *a_vals, double
         int widest = 1;
         for (int l = 0; l + 1 < (int)s->row_level_ptr.size(); l++)
             widest = std::max(widest, s->row_level_ptr[l + 1] - s->row_level_ptr[l]);
-        long long grid = std::min<long long>((long long)per_cu * h->num_cus, (widest + p.block - 1) / p.block);
+        long long grid =
+            std::min<long long>((long long)std::min(per_cu, 2) * h->num_cus, (widest + p.block - 1) / p.block);
         if (grid < 1) grid = 1;
         hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(p.block), 0, h->stream, a, p.nlev, p.lptr.p,
-                           reinterpret_cast<const int4 *>(p.meta.p), h->counters + 3, lu);
+                           reinterpret_cast<const int4 *>(p.meta.p), h->counters + BAAS_BARRIER_BASE, lu);
         BAAS_HIP_TRY(hipGetLastError());
         return BAAS_OK;
     }

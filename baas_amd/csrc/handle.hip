@@ -36,13 +36,13 @@ extern "C" int baas_create(baas_handle **out, void *stream) {
     h->num_cus = prop.multiProcessorCount;
     h->stream = (hipStream_t)stream;
     if (hipMalloc(&h->status, sizeof(baas::DevStatus)) != hipSuccess ||
-        hipMalloc(&h->counters, 64 * sizeof(int)) != hipSuccess) {
+        hipMalloc(&h->counters, BAAS_COUNTER_INTS * sizeof(int)) != hipSuccess) {
         if (h->status) (void)hipFree(h->status);
         delete h;
         return BAAS_ALLOC;
     }
     if (hipMemsetAsync(h->status, 0, sizeof(baas::DevStatus), h->stream) != hipSuccess ||
-        hipMemsetAsync(h->counters, 0, 64 * sizeof(int), h->stream) != hipSuccess ||
+        hipMemsetAsync(h->counters, 0, BAAS_COUNTER_INTS * sizeof(int), h->stream) != hipSuccess ||
         hipStreamSynchronize(h->stream) != hipSuccess || !set_first_epoch(h)) {
         (void)hipFree(h->status);
         (void)hipFree(h->counters);

@@ -113,12 +113,17 @@ template <class T>
 int autotune(baas_handle *h, baas_schedule *s, const T *vals, int uplo, int nrhs, int *chosen);
 }  // namespace baas
 
+constexpr int BAAS_BARRIER_BASE = 64;
+constexpr int BAAS_COUNTER_INTS = BAAS_BARRIER_BASE + 8 * 32;
+
 struct baas_handle {
     hipStream_t stream = nullptr;
     int device = 0;
     int num_cus = 0;
     baas::DevStatus *status = nullptr;  // device
-    int *counters = nullptr;            // device scratch counters [64]
+    int *counters = nullptr;            // device counters [BAAS_COUNTER_INTS]: 0 ticket, 1 done,
+                                        // 2 epoch; the grid barrier's 8 partial counts at
+                                        // BAAS_BARRIER_BASE + 32 p (one 128-byte line each)
     double *dscratch = nullptr;         // device reduction scratch
     size_t dscratch_n = 0;
     int epoch = 0;                      // ready-flag epoch

@@ -620,7 +620,7 @@ __global__ __launch_bounds__(1024) void k_sweep_levels(LvArgs a, const T *__rest
         __threadfence();
         const int d = atomicAdd(a.ctx.done, 1);
         if (d == (int)gridDim.x - 1) {
-            atomicExch(a.bar, 0);
+            reset_barrier(a.bar);
             atomicExch(a.ctx.ticket, 0);
             atomicExch(a.ctx.epoch, epoch == 0x7fffffff ? 1 : epoch + 1);
             __threadfence();
@@ -640,7 +640,7 @@ static int launch_levels(baas_handle *h, baas_schedule *s, const Plan &p, int up
     a.meta = reinterpret_cast<const int4 *>(p.meta.p);
     a.colidx = uplo ? s->u_colidx.p : s->l_colidx.p;
     a.ctx = Ctx{h->counters, h->counters + 1, h->counters + 2};
-    a.bar = h->counters + 3;
+    a.bar = h->counters + BAAS_BARRIER_BASE;
     a.skip = skip;
     a.status = h->status;
     a.spin_limit = h->spin_limit;
@@ -653,8 +653,10 @@ static int launch_levels(baas_handle *h, baas_schedule *s, const Plan &p, int up
     int widest = 1;
     for (int l = 0; l + 1 < (int)s->row_level_ptr.size(); l++)
         widest = std::max(widest, s->row_level_ptr[l + 1] - s->row_level_ptr[l]);
+    // at most two workgroups per CU: every workgroup arrives at every
+    // barrier, so fewer, fuller workgroups make the barriers cheaper
     long long need = ((long long)widest * nrhs + p.block - 1) / p.block;
-    long long grid = std::min<long long>((long long)per_cu * h->num_cus, need);
+    long long grid = std::min<long long>((long long)std::min(per_cu, 2) * h->num_cus, need);
     if (grid < 1) grid = 1;
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(p.block), 0, stream, a, vals, b, x);
     BAAS_HIP_TRY(hipGetLastError());

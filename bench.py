@@ -220,6 +220,23 @@ def main():
                          "ic0_s": round(solver.times()[0], 6), "relres": rel, "plan": pc_plan}
         res["speedup_fp32_factor"] = round(res["fp64"]["time_to_1e-8_s"] / res["fp32_factor"]["time_to_1e-8_s"], 3)
         extra["iccg_knn36000"] = res
+        # batched ICCG: 256 2D Laplacians (sides 16..256, seed 3) x 32 RHS
+        from baas_amd.api import Batch
+        mats, rhs = g.batch_laplace2d()
+        bt = Batch(h, mats, 32)
+        Bt = torch.as_tensor(np.concatenate(rhs, axis=0), device="cuda")
+        del rhs
+        bt.solve(Bt, tol=1e-8, maxiter=3000)  # first call: IC0, plan tuning, graph capture
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, iters = bt.solve(Bt, tol=1e-8, maxiter=3000)
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t0
+        h.check()
+        extra["batch_iccg_256x32"] = {"systems": len(mats), "rows": int(sum(m.n for m in mats)), "rhs": 32,
+                                      "time_to_1e-8_s": round(dt, 4), "iterations_max": int(iters.max()),
+                                      "iterations_mean": round(float(iters.mean()), 1), "info": bt.info()}
+        del bt, Bt
         out["extra"] = extra
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(L, b, args.cpu_seconds)

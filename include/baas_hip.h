@@ -199,7 +199,8 @@ int baas_batch_destroy(baas_batch *b);
 int baas_batch_pcg(baas_batch *b, const double *B, double *X, double tol, int maxiter, int *iters);
 /* times[0..2] (host) = IC0 + setup, CG loop, autotuning seconds of the last
  * call; rpu[0..2] (host) = tuned rows per unit for systems of <= 4096,
- * <= 16384 and more rows. */
+ * <= 16384 and more rows, or {0, 0, workgroup size} when the level-set plan
+ * (all systems' rows of one level at a time) won. */
 int baas_batch_info(const baas_batch *b, double *times, int *rpu);
 
 /* Re-pack a schedule's execution plan (RANGE or SCHEDULE, rows per unit,
