@@ -582,7 +582,7 @@ struct LvArgs {
     long long spin_limit;
 };
 
-template <class T, bool UPPER>
+template <class T, bool UPPER, int CH>
 __global__ __launch_bounds__(1024) void k_sweep_levels(LvArgs a, const T *__restrict__ vals, const T *__restrict__ b,
                                                       T *x) {
     if (a.skip && ld_agent(a.skip)) return;
@@ -600,16 +600,26 @@ __global__ __launch_bounds__(1024) void k_sweep_levels(LvArgs a, const T *__rest
             const int4 m = a.meta[q];
             const int row = m.x, j0 = m.y, cnt = m.z;
             T s = T(0);
-            if (UPPER) {
-                for (int k = 0; k < cnt; k++) {
-                    const int j = j0 - k;
-                    s = fma(ld_agent(&x[(size_t)a.colidx[j] * nrhs + r]), vals[j], s);
+            // parents CH at a time: every column and value load of a chunk,
+            // then every x load, are in flight together; the sum keeps the
+            // reference order (slots past the count repeat the last entry
+            // and are not summed)
+            for (int k0 = 0; k0 < cnt; k0 += CH) {
+                const int nc = cnt - k0 < CH ? cnt - k0 : CH;
+                int c[CH];
+                T w[CH], xv[CH];
+#pragma unroll
+                for (int t = 0; t < CH; t++) {
+                    const int o = k0 + (t < nc ? t : nc - 1);
+                    const int j = UPPER ? j0 - o : j0 + o;
+                    c[t] = a.colidx[j];
+                    w[t] = vals[j];
                 }
-            } else {
-                for (int k = 0; k < cnt; k++) {
-                    const int j = j0 + k;
-                    s = fma(vals[j], ld_agent(&x[(size_t)a.colidx[j] * nrhs + r]), s);
-                }
+#pragma unroll
+                for (int t = 0; t < CH; t++) xv[t] = ld_agent(&x[(size_t)c[t] * nrhs + r]);
+#pragma unroll
+                for (int t = 0; t < CH; t++)
+                    if (t < nc) s = UPPER ? fma(xv[t], w[t], s) : fma(w[t], xv[t], s);
             }
             const int dg = UPPER ? j0 - cnt : j0 + cnt;
             st_agent(&x[(size_t)row * nrhs + r], (b[(size_t)row * nrhs + r] - s) / vals[dg]);
@@ -644,7 +654,8 @@ static int launch_levels(baas_handle *h, baas_schedule *s, const Plan &p, int up
     a.skip = skip;
     a.status = h->status;
     a.spin_limit = h->spin_limit;
-    auto kern = uplo ? k_sweep_levels<T, true> : k_sweep_levels<T, false>;
+    auto kern = p.max_deg <= 4 ? (uplo ? k_sweep_levels<T, true, 4> : k_sweep_levels<T, false, 4>)
+                               : (uplo ? k_sweep_levels<T, true, 8> : k_sweep_levels<T, false, 8>);
     int per_cu = 0;
     BAAS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, p.block, 0));
     if (per_cu <= 0) return BAAS_UNSUPPORTED;
