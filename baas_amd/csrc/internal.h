@@ -99,12 +99,6 @@ struct Plan {
     DBuf<unsigned char> xp_raw;  // 2 x n published values (dataflow kernel)
     int xp_type = 0;             // sizeof the value type the halves are armed for
     DBuf<int> pcount;            // [1] sweeps run on this plan
-    // unit-level dependencies (the task DAG): a unit starts once every unit
-    // that holds one of its parents has published its unit flag
-    DBuf<int> udep_ptr;   // [nunits+1]
-    DBuf<int> udeps;      // producer units, ascending
-    DBuf<int> uflags;     // [nunits] epoch of the last sweep that finished the unit
-    int unitdep = 0;      // wait per unit instead of per row
     std::vector<int> h_unit_ptr;  // host copy: unit boundaries in rows
     std::vector<int> h_rows;      // host copy of rows (without flag bit)
 };

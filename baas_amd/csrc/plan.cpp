@@ -171,35 +171,10 @@ int make_plan(Plan &p, int n, const std::vector<int> &order, const std::vector<i
             p.max_deg = std::max(p.max_deg, re - rb - 1);
             meta[4 * (size_t)q + 3] = key[r] - key_begin[u];
         }
-    // unit DAG: producer units of each unit
-    std::vector<int> udep_ptr(nunits + 1, 0), udeps, mark(nunits, -1);
-    for (int u = 0; u < nunits; u++) {
-        udep_ptr[u] = (int)udeps.size();
-        for (int q = unit_ptr[u]; q < unit_ptr[u + 1]; q++) {
-            const int r = rows[q] & 0x7fffffff;
-            for (int j = dptr[r]; j < dptr[r + 1]; j++) {
-                const int v = unit_of[didx[j]];
-                if (v != u && mark[v] != u) {
-                    mark[v] = u;
-                    udeps.push_back(v);
-                }
-            }
-        }
-        std::sort(udeps.begin() + udep_ptr[u], udeps.end());
-    }
-    udep_ptr[nunits] = (int)udeps.size();
-    BAAS_TRY(upload(p.udep_ptr, udep_ptr, st));
-    BAAS_TRY(upload(p.udeps, udeps, st));
     BAAS_TRY(p.pcount.alloc(1));
     if (hipMemsetAsync(p.pcount.p, 0, sizeof(int), st) != hipSuccess) return BAAS_HIP_ERROR;
     p.xp_raw.free();
     p.xp_type = 0;
-    BAAS_TRY(p.uflags.alloc(nunits));
-    if (hipMemsetAsync(p.uflags.p, 0, sizeof(int) * nunits, st) != hipSuccess) return BAAS_HIP_ERROR;
-    // Unit-level waiting is switched off: its first GPU run went silent in the
-    // forward/backward solve test and no cause has been shown yet; plans wait
-    // per row (the protocol that has passed every run).
-    p.unitdep = 0;
     BAAS_TRY(upload(p.scol, scol, st));
     BAAS_TRY(upload(p.ext_ptr, ext_ptr, st));
     BAAS_TRY(upload(p.ext_cols, ext_cols, st));
@@ -263,7 +238,6 @@ static int make_levels_plan(Plan &p, const baas_schedule *s, const HostCsr &M, b
     p.mode = BAAS_PLAN_LEVELS;
     p.block = block;
     p.keymode = 0;
-    p.unitdep = 0;
     p.max_slots = 0;
     p.df_lds[0] = p.df_lds[1] = 0;
     p.max_unit_keys = 0;

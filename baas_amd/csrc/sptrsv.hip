@@ -28,11 +28,9 @@ struct SweepArgs {
     const int *skip;  // optional device flag: nonzero = return at once (converged PCG)
     DevStatus *status;
     long long spin_limit;
-    const int *udep_ptr, *udeps;  // unit-level dependencies (UNITDEP kernels)
-    int *uflags;
 };
 
-template <class T, int KEYMODE, bool UPPER, bool LDS, bool UNITDEP>
+template <class T, int KEYMODE, bool UPPER, bool LDS>
 __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a, const T *__restrict__ vals,
                                                const T *__restrict__ b, T *x) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -46,12 +44,6 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a, const T *__restrict
         if (u >= a.nunits) break;
         const int kb = a.key_begin[u];
         const int s0 = a.sub_begin[u], s1 = a.sub_begin[u + 1];
-        if (UNITDEP) {
-            // one poll per producer unit, then every external parent is final
-            for (int d = a.udep_ptr[u] + threadIdx.x; d < a.udep_ptr[u + 1]; d += blockDim.x)
-                wait_ready(a.uflags, a.udeps[d], epoch, a.spin_limit, a.status);
-            __syncthreads();
-        }
         for (int si = s0; si < s1; si++) {
             const int qb = a.sub_ptr[si], qe = a.sub_ptr[si + 1];
             const int items = (qe - qb) * nrhs;
@@ -64,12 +56,10 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a, const T *__restrict
                 const int ob = UPPER ? rb + 1 : rb, oe = UPPER ? re : re - 1;  // off-diagonals
                 const int dg = UPPER ? rb : re - 1;
                 // wait for parents produced by other units
-                if (!UNITDEP)
-                    for (int j = ob; j < oe; j++) {
-                        const int c = a.colidx[j];
-                        if (key_of<KEYMODE>(c, a.n, a.key) < kb)
-                            wait_ready(a.flags, c, epoch, a.spin_limit, a.status);
-                    }
+                for (int j = ob; j < oe; j++) {
+                    const int c = a.colidx[j];
+                    if (key_of<KEYMODE>(c, a.n, a.key) < kb) wait_ready(a.flags, c, epoch, a.spin_limit, a.status);
+                }
                 T s = T(0);
                 if (UPPER) {
                     for (int j = oe - 1; j >= ob; j--) {
@@ -93,15 +83,12 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a, const T *__restrict
                 const T xr = (b[(size_t)row * nrhs + r] - s) / vals[dg];
                 st_agent(&x[(size_t)row * nrhs + r], xr);
                 if (LDS) xs[(size_t)(key_of<KEYMODE>(row, a.n, a.key) - kb) * nrhs + r] = xr;
-                if (!UNITDEP && nrhs == 1 && rw < 0) {
+                if (nrhs == 1 && rw < 0) {
                     wait_stores();
                     st_agent(&a.flags[row], epoch);
                 }
             }
-            if (UNITDEP) {
-                wait_stores();
-                __syncthreads();
-            } else if (nrhs > 1) {
+            if (nrhs > 1) {
                 wait_stores();
                 __syncthreads();
                 for (int t = threadIdx.x; t < qe - qb; t += blockDim.x) {
@@ -115,7 +102,6 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a, const T *__restrict
                 __syncthreads();
             }
         }
-        if (UNITDEP && threadIdx.x == 0) st_agent(&a.uflags[u], epoch);
     }
     retire_block(a.ctx, epoch);
 }
@@ -522,7 +508,7 @@ static inline int nblk(long long n, int b = 256) { return (int)((n + b - 1) / b)
 template <class T, int KM, bool UP, bool LDS>
 static int launch_sweep_t(baas_handle *h, const Plan &p, const SweepArgs &a, const T *vals, const T *b, T *x,
                           size_t lds, hipStream_t stream) {
-    auto kern = p.unitdep ? k_sweep<T, KM, UP, LDS, true> : k_sweep<T, KM, UP, LDS, false>;
+    auto kern = k_sweep<T, KM, UP, LDS>;
     int per_cu = 0;
     BAAS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, p.block, lds));
     if (per_cu <= 0) return BAAS_UNSUPPORTED;
@@ -730,9 +716,6 @@ int sweep(baas_handle *h, baas_schedule *s, int uplo, const T *vals, const T *b,
     a.skip = skip;
     a.status = h->status;
     a.spin_limit = h->spin_limit;
-    a.udep_ptr = p.udep_ptr.p;
-    a.udeps = p.udeps.p;
-    a.uflags = p.uflags.p;
     size_t lds = (size_t)p.max_unit_keys * nrhs * sizeof(T);
     bool use_lds = lds <= lds_limit(h);
     if (uplo)

@@ -179,3 +179,31 @@ def test_pcg_autotune_keeps_iterates(handle):
         np.testing.assert_array_equal(down(x0), down(x1))
         del solver
         torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_pcg_same_iterates_on_every_plan_kind(handle, prec):
+    """Unit plans with flags, dataflow plans and level-set plans (device-wide
+    barriers inside the captured graph) give the same PCG iterates, solve
+    after solve."""
+    import torch
+    from baas_amd.api import PCG, Schedule
+    _, _, A = workload("C3_knn")
+    b = up(g.knn_stiffness()[1])
+    s = Schedule(handle, A.rowptr, A.colidx, kind=_lib.KIND_IC0)
+    av = up(A.values)
+    ref = None
+    for mode, rpu, blk, df in ((1, 4096, 256, False), (3, 0, 256, False), (1, 2048, 384, True), (2, 0, 256, False)):
+        announce("pcg plans", prec, mode, rpu, blk, df)
+        s.replan(mode, rpu, blk, dataflow=df)
+        solver = PCG(handle, s, av, precision=prec)
+        for _ in range(2):
+            x, k, rel = solver.solve(b, tol=1e-8, maxiter=2000)
+            handle.check()
+            xh = down(x)
+            if ref is None:
+                ref = (k, xh)
+            assert k == ref[0], (mode, k, ref[0])
+            np.testing.assert_array_equal(xh, ref[1])
+        del solver
+        torch.cuda.synchronize()
