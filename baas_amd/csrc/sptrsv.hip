@@ -568,6 +568,65 @@ struct LvArgs {
     long long spin_limit;
 };
 
+// One (row, rhs) item of a level with its first CH parents' columns and
+// values, the right-hand side and the diagonal: everything but x, so it can
+// be loaded before the barrier that makes the parents' x final.
+template <class T, int CH>
+struct LvItem {
+    int row, r, j0, cnt;
+    int c[CH];
+    T w[CH];
+    T bv, dv;
+};
+
+template <class T, bool UPPER, int CH>
+__device__ __forceinline__ void lv_load(const LvArgs &a, const T *__restrict__ vals, const T *__restrict__ b, int q,
+                                        int r, LvItem<T, CH> &I) {
+    const int4 m = a.meta[q];
+    I.row = m.x;
+    I.j0 = m.y;
+    I.cnt = m.z;
+    I.r = r;
+    const int nc = m.z < CH ? m.z : CH;
+#pragma unroll
+    for (int t = 0; t < CH; t++) {
+        const int o = nc > 0 ? (t < nc ? t : nc - 1) : 0;
+        const int j = UPPER ? m.y - o : m.y + o;
+        I.c[t] = a.colidx[j];
+        I.w[t] = vals[j];
+    }
+    I.bv = b[(size_t)m.x * a.nrhs + r];
+    I.dv = vals[UPPER ? m.y - m.z : m.y + m.z];
+}
+
+// Solve one item: parents CH at a time (the first chunk already loaded);
+// every x load of a chunk is in flight together and the sum keeps the
+// reference order (slots past the count repeat the last entry, not summed).
+template <class T, bool UPPER, int CH>
+__device__ __forceinline__ void lv_solve(const LvArgs &a, const T *__restrict__ vals, T *x, LvItem<T, CH> &I) {
+    const int nrhs = a.nrhs;
+    T s = T(0);
+    for (int k0 = 0; k0 < I.cnt; k0 += CH) {
+        const int nc = I.cnt - k0 < CH ? I.cnt - k0 : CH;
+        if (k0 > 0) {
+#pragma unroll
+            for (int t = 0; t < CH; t++) {
+                const int o = k0 + (t < nc ? t : nc - 1);
+                const int j = UPPER ? I.j0 - o : I.j0 + o;
+                I.c[t] = a.colidx[j];
+                I.w[t] = vals[j];
+            }
+        }
+        T xv[CH];
+#pragma unroll
+        for (int t = 0; t < CH; t++) xv[t] = ld_agent(&x[(size_t)I.c[t] * nrhs + I.r]);
+#pragma unroll
+        for (int t = 0; t < CH; t++)
+            if (t < nc) s = UPPER ? fma(xv[t], I.w[t], s) : fma(I.w[t], xv[t], s);
+    }
+    st_agent(&x[(size_t)I.row * nrhs + I.r], (I.bv - s) / I.dv);
+}
+
 template <class T, bool UPPER, int CH>
 __global__ __launch_bounds__(1024) void k_sweep_levels(LvArgs a, const T *__restrict__ vals, const T *__restrict__ b,
                                                       T *x) {
@@ -576,41 +635,35 @@ __global__ __launch_bounds__(1024) void k_sweep_levels(LvArgs a, const T *__rest
     const int nrhs = a.nrhs;
     const long long nth = (long long)gridDim.x * blockDim.x;
     const long long gt = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    // this thread's first item of each level is loaded before the barrier
+    // that precedes the level
+    LvItem<T, CH> first;
+    bool have = false;
+    {
+        const int l = UPPER ? a.nlev - 1 : 0;
+        const int qb = a.lptr[l];
+        if (gt < (long long)(a.lptr[l + 1] - qb) * nrhs) {
+            lv_load<T, UPPER, CH>(a, vals, b, qb + (int)(gt / nrhs), (int)(gt % nrhs), first);
+            have = true;
+        }
+    }
     for (int li = 0; li < a.nlev; li++) {
         const int l = UPPER ? a.nlev - 1 - li : li;
         const int qb = a.lptr[l], qe = a.lptr[l + 1];
         const long long items = (long long)(qe - qb) * nrhs;
-        for (long long t = gt; t < items; t += nth) {
-            const int q = qb + (int)(t / nrhs);
-            const int r = (int)(t - (long long)(q - qb) * nrhs);
-            const int4 m = a.meta[q];
-            const int row = m.x, j0 = m.y, cnt = m.z;
-            T s = T(0);
-            // parents CH at a time: every column and value load of a chunk,
-            // then every x load, are in flight together; the sum keeps the
-            // reference order (slots past the count repeat the last entry
-            // and are not summed)
-            for (int k0 = 0; k0 < cnt; k0 += CH) {
-                const int nc = cnt - k0 < CH ? cnt - k0 : CH;
-                int c[CH];
-                T w[CH], xv[CH];
-#pragma unroll
-                for (int t = 0; t < CH; t++) {
-                    const int o = k0 + (t < nc ? t : nc - 1);
-                    const int j = UPPER ? j0 - o : j0 + o;
-                    c[t] = a.colidx[j];
-                    w[t] = vals[j];
-                }
-#pragma unroll
-                for (int t = 0; t < CH; t++) xv[t] = ld_agent(&x[(size_t)c[t] * nrhs + r]);
-#pragma unroll
-                for (int t = 0; t < CH; t++)
-                    if (t < nc) s = UPPER ? fma(xv[t], w[t], s) : fma(w[t], xv[t], s);
-            }
-            const int dg = UPPER ? j0 - cnt : j0 + cnt;
-            st_agent(&x[(size_t)row * nrhs + r], (b[(size_t)row * nrhs + r] - s) / vals[dg]);
+        if (have) lv_solve<T, UPPER, CH>(a, vals, x, first);
+        for (long long t = gt + nth; t < items; t += nth) {
+            LvItem<T, CH> it;
+            lv_load<T, UPPER, CH>(a, vals, b, qb + (int)(t / nrhs), (int)(t % nrhs), it);
+            lv_solve<T, UPPER, CH>(a, vals, x, it);
         }
-        if (li + 1 < a.nlev) grid_barrier(a.bar, (li + 1) * (int)gridDim.x, a.status, a.spin_limit);
+        if (li + 1 < a.nlev) {
+            const int ln = UPPER ? l - 1 : l + 1;
+            const int nb = a.lptr[ln];
+            have = gt < (long long)(a.lptr[ln + 1] - nb) * nrhs;
+            if (have) lv_load<T, UPPER, CH>(a, vals, b, nb + (int)(gt / nrhs), (int)(gt % nrhs), first);
+            grid_barrier(a.bar, (li + 1) * (int)gridDim.x, a.status, a.spin_limit);
+        }
     }
     if (threadIdx.x == 0) {
         __threadfence();
