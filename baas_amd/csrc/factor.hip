@@ -20,10 +20,13 @@
 #include "internal.h"
 
 namespace baas {
+int build_ic0_plans(baas_schedule *s);
 
 struct FactorArgs {
     int n, nunits;
     const int *rowptr, *colidx, *diagptr;
+    const int *ic_djpos, *ic_pptr;  // IC0 dot plans (may be null)
+    const int2 *ic_pairs;
     const int *key_begin, *sub_begin, *sub_ptr, *rows, *key;
     int *flags;
     Ctx ctx;
@@ -116,8 +119,82 @@ __device__ __forceinline__ void factor_row(const FactorArgs &a, int i, double *l
     }
 }
 
+// IC0 row from its dot plan, for rows of at most IC_MAXE lower entries and
+// IC_MAXT pairs: every load the row needs (its A values, the diagonals of
+// its columns, the other rows' entries of every dot) is issued at once, then
+// the entries are computed in order with the row's own values in LDS (own,
+// stride apart) -- the same products in the same order as the two-pointer
+// dots.  Returns false for rows outside these bounds.
+constexpr int IC_MAXE = 8, IC_MAXT = 24;
+
+__device__ __forceinline__ bool ic0_row_planned(const FactorArgs &a, int i, double *lu, double *own, int stride) {
+    const int rb = a.rowptr[i], dg = a.diagptr[i], ne = dg - rb;
+    const int p0 = a.ic_pptr[i], np = a.ic_pptr[i + 1] - p0;
+    if (ne > IC_MAXE || np > IC_MAXT) return false;
+    double av[IC_MAXE], dj[IC_MAXE], pv[IC_MAXT];
+    int djp[IC_MAXE];
+    int2 pr[IC_MAXT];
+#pragma unroll
+    for (int e = 0; e < IC_MAXE; e++) {
+        const int k = rb + (e < ne ? e : (ne > 0 ? ne - 1 : 0));
+        av[e] = ld_agent(&lu[k]);
+        djp[e] = a.ic_djpos[k];
+    }
+#pragma unroll
+    for (int t = 0; t < IC_MAXT; t++) pr[t] = a.ic_pairs[p0 + (t < np ? t : (np > 0 ? np - 1 : 0))];
+    const double aii = ld_agent(&lu[dg]);
+#pragma unroll
+    for (int e = 0; e < IC_MAXE; e++) dj[e] = ld_agent(&lu[djp[e]]);
+#pragma unroll
+    for (int t = 0; t < IC_MAXT; t++) pv[t] = ld_agent(&lu[pr[t].x]);
+#pragma unroll
+    for (int e = 0; e < IC_MAXE; e++) {
+        if (e < ne) {
+            double dp = 0.0;
+#pragma unroll
+            for (int t = 0; t < IC_MAXT; t++)
+                if (t < np && (pr[t].y >> 8) == e) dp = fma(own[(pr[t].y & 255) * stride], pv[t], dp);
+            const double v = (av[e] - dp) / dj[e];
+            own[e * stride] = v;
+            st_agent(&lu[rb + e], v);
+        }
+    }
+    // self dot in the reference build's order (blocks of four rounded
+    // squares added in order when there are more than three, FMA tail)
+    double r = 0.0;
+    const int stop = ne > 3 ? (ne & ~3) : 0;
+#pragma unroll
+    for (int e = 0; e < IC_MAXE; e++) {
+        if (e < ne) {
+            const double v = own[e * stride];
+            if (e < stop) {
+                const double p = v * v;
+                r = r + p;
+            } else {
+                r = fma(v, v, r);
+            }
+        }
+    }
+    if (aii - r > 0) {
+        st_agent(&lu[dg], sqrt(aii - r));
+    } else {
+        raise_status(a.status, BAAS_NONPOSITIVE_PIVOT, i);
+    }
+    return true;
+}
+
+// One row of the factorisation: the planned IC0 row when it applies, the
+// general two-pointer form otherwise.
+template <int KIND>
+__device__ __forceinline__ void factor_any(const FactorArgs &a, int i, double *lu, double *own, int stride) {
+    if (KIND == BAAS_KIND_IC0 && a.ic_pptr && ic0_row_planned(a, i, lu, own, stride)) return;
+    factor_row<KIND>(a, i, lu);
+}
+
 template <int KIND, int KEYMODE>
 __global__ __launch_bounds__(1024) void k_factor(FactorArgs a, double *lu) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double *own = reinterpret_cast<double *>(smem_raw) + threadIdx.x;
     __shared__ int s_unit;
     const int epoch = read_epoch(a.ctx);
     for (;;) {
@@ -133,7 +210,7 @@ __global__ __launch_bounds__(1024) void k_factor(FactorArgs a, double *lu) {
                     const int c = a.colidx[k];
                     if (key_of<KEYMODE>(c, a.n, a.key) < kb) wait_ready(a.flags, c, epoch, a.spin_limit, a.status);
                 }
-                factor_row<KIND>(a, i, lu);
+                factor_any<KIND>(a, i, lu, own, blockDim.x);
                 wait_stores();
                 if (rw < 0) st_agent(&a.flags[i], epoch);
             }
@@ -149,11 +226,13 @@ __global__ __launch_bounds__(1024) void k_factor(FactorArgs a, double *lu) {
 template <int KIND>
 __global__ __launch_bounds__(1024) void k_factor_levels(FactorArgs a, int nlev, const int *__restrict__ lptr,
                                                        const int4 *__restrict__ meta, int *bar, double *lu) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double *own = reinterpret_cast<double *>(smem_raw) + threadIdx.x;
     const int epoch = read_epoch(a.ctx);
     const int nth = (int)(gridDim.x * blockDim.x);
     const int gt = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     for (int l = 0; l < nlev; l++) {
-        for (int q = lptr[l] + gt; q < lptr[l + 1]; q += nth) factor_row<KIND>(a, meta[q].x, lu);
+        for (int q = lptr[l] + gt; q < lptr[l + 1]; q += nth) factor_any<KIND>(a, meta[q].x, lu, own, blockDim.x);
         if (l + 1 < nlev) grid_barrier(bar, (l + 1) * (int)gridDim.x, a.status, a.spin_limit);
     }
     if (threadIdx.x == 0) {
@@ -190,12 +269,13 @@ template <int KIND, int KM>
 static int launch_factor(baas_handle *h, const Plan &p, const FactorArgs &a, double *lu) {
     auto kern = k_factor<KIND, KM>;
     int per_cu = 0;
-    BAAS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, p.block, 0));
+    const size_t lds = KIND == BAAS_KIND_IC0 ? (size_t)IC_MAXE * p.block * sizeof(double) : 0;
+    BAAS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, p.block, lds));
     if (per_cu <= 0) return BAAS_UNSUPPORTED;
     long long grid = (long long)per_cu * h->num_cus;
     if (grid > p.nunits) grid = p.nunits;
     if (grid < 1) grid = 1;
-    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(p.block), 0, h->stream, a, lu);
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(p.block), lds, h->stream, a, lu);
     BAAS_HIP_TRY(hipGetLastError());
     return BAAS_OK;
 }
@@ -213,6 +293,14 @@ static int factor(baas_handle *h, baas_schedule *s, const double *a_vals, double
     a.rowptr = s->a_rowptr.p;
     a.colidx = s->a_colidx.p;
     a.diagptr = s->a_diagptr.p;
+    a.ic_djpos = a.ic_pptr = nullptr;
+    a.ic_pairs = nullptr;
+    if (KIND == BAAS_KIND_IC0) {
+        BAAS_TRY(build_ic0_plans(s));
+        a.ic_djpos = s->ic_djpos.p;
+        a.ic_pptr = s->ic_pptr.p;
+        a.ic_pairs = reinterpret_cast<const int2 *>(s->ic_pairs.p);
+    }
     a.key_begin = p.key_begin.p;
     a.sub_begin = p.sub_begin.p;
     a.sub_ptr = p.sub_ptr.p;
@@ -224,8 +312,9 @@ static int factor(baas_handle *h, baas_schedule *s, const double *a_vals, double
     a.spin_limit = h->spin_limit;
     if (p.mode == BAAS_PLAN_LEVELS) {
         auto kern = k_factor_levels<KIND>;
+        const size_t lds = KIND == BAAS_KIND_IC0 ? (size_t)IC_MAXE * p.block * sizeof(double) : 0;
         int per_cu = 0;
-        BAAS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, p.block, 0));
+        BAAS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, p.block, lds));
         if (per_cu <= 0) return BAAS_UNSUPPORTED;
         int widest = 1;
         for (int l = 0; l + 1 < (int)s->row_level_ptr.size(); l++)
@@ -233,7 +322,7 @@ static int factor(baas_handle *h, baas_schedule *s, const double *a_vals, double
         long long grid =
             std::min<long long>((long long)std::min(per_cu, 2) * h->num_cus, (widest + p.block - 1) / p.block);
         if (grid < 1) grid = 1;
-        hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(p.block), 0, h->stream, a, p.nlev, p.lptr.p,
+        hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(p.block), lds, h->stream, a, p.nlev, p.lptr.p,
                            reinterpret_cast<const int4 *>(p.meta.p), h->counters + BAAS_BARRIER_BASE, lu);
         BAAS_HIP_TRY(hipGetLastError());
         return BAAS_OK;

@@ -146,6 +146,12 @@ struct baas_schedule {
     int plan_gen = 0;       // bumped whenever the plans change (captured graphs re-capture)
     baas::DBuf<int> flags;  // [n] epoch of the last sweep that finished the row
     baas::DBuf<int> row_level_ptr_d, row_perm_d;  // for the level-synchronous solve
+    // IC0 dot plans (built at the first factorisation): per lower entry the
+    // position of its column's diagonal; per row the (other row's entry,
+    // entry << 8 | own entry) pairs of every sparse dot, in the reference's
+    // summation order
+    baas::DBuf<int> ic_djpos, ic_pptr, ic_pairs;
+    bool ic_plans = false;
     double t_gpu = 0, t_host = 0;
     baas_opts opts{};
 };

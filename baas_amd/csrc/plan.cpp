@@ -354,6 +354,53 @@ int build_plans(baas_schedule *s) {
     return build_plans_mode(s, mode, s->opts.rows_per_unit);
 }
 
+// IC0 dot plans: for row i and its lower entry e (column j), the matching
+// columns of row i's lower part and row j's lower part in ascending order --
+// exactly the pairs the up-looking sparse dot multiplies
+// (merge/IC.cpp:6-29) -- as (position in row j, e << 8 | position in row i
+// relative to its first entry).
+int build_ic0_plans(baas_schedule *s) {
+    if (s->ic_plans) return BAAS_OK;
+    hipStream_t st = s->h->stream;
+    const int n = s->n;
+    HostCsr A;
+    BAAS_TRY(fetch(A, s->a_rowptr.p, s->a_colidx.p, n, st));
+    std::vector<int> dg(n);
+    if (hipMemcpy(dg.data(), s->a_diagptr.p, sizeof(int) * n, hipMemcpyDeviceToHost) != hipSuccess)
+        return BAAS_HIP_ERROR;
+    std::vector<int> djpos(A.idx.size(), 0), pptr(n + 1, 0), pairs;
+    for (int i = 0; i < n; i++) {
+        pptr[i] = (int)(pairs.size() / 2);
+        const int rb = A.ptr[i], di = dg[i];
+        for (int k = rb; k < di; k++) {
+            const int j = A.idx[k];
+            djpos[k] = dg[j];
+            int l1 = rb, l2 = A.ptr[j];
+            const int u1 = di, u2 = dg[j];
+            while (l1 < u1 && l2 < u2) {
+                const int c1 = A.idx[l1], c2 = A.idx[l2];
+                if (c1 == c2) {
+                    pairs.push_back(l2);
+                    pairs.push_back(((k - rb) << 8) | (l1 - rb));
+                    l1++;
+                    l2++;
+                } else if (c1 < c2) {
+                    l1++;
+                } else {
+                    l2++;
+                }
+            }
+        }
+    }
+    pptr[n] = (int)(pairs.size() / 2);
+    BAAS_TRY(upload(s->ic_djpos, djpos, st));
+    BAAS_TRY(upload(s->ic_pptr, pptr, st));
+    BAAS_TRY(upload(s->ic_pairs, pairs, st));
+    if (hipStreamSynchronize(st) != hipSuccess) return BAAS_HIP_ERROR;
+    s->ic_plans = true;
+    return BAAS_OK;
+}
+
 }  // namespace baas
 
 extern "C" int baas_replan(baas_schedule *s, int mode, int rows_per_unit, int block_threads) {
