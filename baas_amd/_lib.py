@@ -56,6 +56,7 @@ _SIGS = {
     "baas_plan_dataflow": (_i, [_P, _i]),
     "baas_autotune": (_i, [_P, _P, _P, _i, _i, _P]),
     "baas_autotune_f32": (_i, [_P, _P, _P, _i, _i, _P]),
+    "baas_autotune_factor": (_i, [_P, _P, _P, _P]),
     "baas_pcg_autotune": (_i, [_P, _P]),
     "baas_sptrsv_f64": (_i, [_P, _P, _i, _P, _P, _P, _i]),
     "baas_sptrsv_f32": (_i, [_P, _P, _i, _P, _P, _P, _i]),

@@ -140,8 +140,10 @@ __device__ __forceinline__ bool ic0_row_planned(const FactorArgs &a, int i, doub
         av[e] = ld_agent(&lu[k]);
         djp[e] = a.ic_djpos[k];
     }
+    // a row without dot pairs reads no pair record at all (its slots point at
+    // its own diagonal); otherwise slots past the count repeat the last pair
 #pragma unroll
-    for (int t = 0; t < IC_MAXT; t++) pr[t] = a.ic_pairs[p0 + (t < np ? t : (np > 0 ? np - 1 : 0))];
+    for (int t = 0; t < IC_MAXT; t++) pr[t] = np > 0 ? a.ic_pairs[p0 + (t < np ? t : np - 1)] : make_int2(dg, 0);
     const double aii = ld_agent(&lu[dg]);
 #pragma unroll
     for (int e = 0; e < IC_MAXE; e++) dj[e] = ld_agent(&lu[djp[e]]);

@@ -393,6 +393,7 @@ int build_ic0_plans(baas_schedule *s) {
         }
     }
     pptr[n] = (int)(pairs.size() / 2);
+    if (pairs.empty()) pairs = {0, 0};  // keep one valid record (never read: every row has np = 0)
     BAAS_TRY(upload(s->ic_djpos, djpos, st));
     BAAS_TRY(upload(s->ic_pptr, pptr, st));
     BAAS_TRY(upload(s->ic_pairs, pairs, st));

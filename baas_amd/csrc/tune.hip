@@ -106,6 +106,64 @@ template int autotune<float>(baas_handle *, baas_schedule *, const float *, int,
 
 }  // namespace baas
 
+// The same search for the numeric factorisation of an IC0 or ILU0
+// schedule: unit plans, schedule plans and level-set plans, timed on the
+// caller's values (into a scratch factor).
+static int autotune_factor(baas_handle *h, baas_schedule *s, const double *a_vals, int *chosen) {
+    if (!h || !s || !a_vals || (s->kind != BAAS_KIND_IC0 && s->kind != BAAS_KIND_ILU0)) return BAAS_INVALID_ARG;
+    DBuf<double> lu;
+    BAAS_TRY(lu.alloc(s->nnz_full));
+    std::vector<Cand> cands;
+    for (int rpu : {1024, 2048, 4096, 8192})
+        for (int blk : {256, 512}) cands.push_back({BAAS_PLAN_RANGE, rpu, blk, 0});
+    for (int rpu : {0, 1024})
+        for (int blk : {256, 1024}) cands.push_back({BAAS_PLAN_SCHEDULE, rpu, blk, 0});
+    for (int blk : {64, 128, 256, 512}) cands.push_back({BAAS_PLAN_LEVELS, 0, blk, 0});
+    hipEvent_t e0, e1;
+    BAAS_HIP_TRY(hipEventCreate(&e0));
+    BAAS_HIP_TRY(hipEventCreate(&e1));
+    float best = 1e30f;
+    Cand win = cands[0];
+    for (const Cand &c : cands) {
+        if (c.mode == BAAS_PLAN_RANGE && c.rpu >= 2 * s->n && c.rpu != 1024) continue;
+        s->opts.block_threads = c.block;
+        if (build_plans_mode(s, c.mode, c.rpu) != BAAS_OK) continue;
+        float t_best = 1e30f;
+        bool ok = true;
+        for (int rep = 0; rep < 3 && ok; rep++) {
+            (void)hipEventRecord(e0, h->stream);
+            ok = (s->kind == BAAS_KIND_IC0 ? baas_ic0_f64(h, s, a_vals, lu.p) : baas_ilu0_f64(h, s, a_vals, lu.p)) ==
+                 BAAS_OK;
+            (void)hipEventRecord(e1, h->stream);
+            (void)hipEventSynchronize(e1);
+            float ms = 0;
+            (void)hipEventElapsedTime(&ms, e0, e1);
+            if (rep > 0 && ms < t_best) t_best = ms;
+        }
+        if (ok && t_best < best) best = t_best, win = c;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    s->opts.block_threads = win.block;
+    s->opts.plan_mode = win.mode;
+    s->opts.rows_per_unit = win.rpu;
+    BAAS_TRY(build_plans_mode(s, win.mode, win.rpu));
+    if (chosen) {
+        chosen[0] = win.mode;
+        chosen[1] = win.rpu;
+        chosen[2] = win.block;
+        chosen[3] = 0;
+        chosen[4] = (int)(best * 1000.0f);
+    }
+    // pivot reports of the trial runs are the caller's to see on the real call
+    const int st = baas_check(h);
+    return st == BAAS_OK || st == BAAS_NONPOSITIVE_PIVOT || st == BAAS_ZERO_PIVOT ? BAAS_OK : st;
+}
+
+extern "C" int baas_autotune_factor(baas_handle *h, baas_schedule *s, const double *a_vals, int *chosen) {
+    return autotune_factor(h, s, a_vals, chosen);
+}
+
 extern "C" int baas_autotune(baas_handle *h, baas_schedule *s, const double *vals, int uplo, int nrhs, int *chosen) {
     return baas::autotune<double>(h, s, vals, uplo, nrhs, chosen);
 }

@@ -188,18 +188,21 @@ def main():
         sa = Schedule(h, A.rowptr, A.colidx, kind=_lib.KIND_IC0)
         av = torch.as_tensor(A.values, device="cuda")
         lu = torch.empty_like(av)
+        fplan = sa.autotune_factor(av)
         _, m, _ = time_events(lambda: sa.ic0(av, lu), 20, 3)
         h.check()
         nl = L.nnz
-        extra["ic0_lap3d64"] = {"ms": round(m, 4), "GB/s": round((20 * nl + 8 * L.n) / (m * 1e-3) / 1e9, 2)}
+        extra["ic0_lap3d64"] = {"ms": round(m, 4), "GB/s": round((20 * nl + 8 * L.n) / (m * 1e-3) / 1e9, 2),
+                                "plan": fplan}
         Cd = g.convdiff3d(64)
         sc = Schedule(h, Cd.rowptr, Cd.colidx, kind=_lib.KIND_ILU0)
         cv = torch.as_tensor(Cd.values, device="cuda")
         luc = torch.empty_like(cv)
+        fplan = sc.autotune_factor(cv)
         _, m, _ = time_events(lambda: sc.ilu0(cv, luc), 20, 3)
         h.check()
         extra["ilu0_convdiff64"] = {"ms": round(m, 4),
-                                    "GB/s": round((20 * Cd.nnz + 8 * Cd.n) / (m * 1e-3) / 1e9, 2)}
+                                    "GB/s": round((20 * Cd.nnz + 8 * Cd.n) / (m * 1e-3) / 1e9, 2), "plan": fplan}
         # ICCG time to 1e-8, FP64 factor vs FP32 factor (kNN 3x3-block stiffness, n = 36000)
         K, kb = g.knn_stiffness()
         sk = Schedule(h, K.rowptr, K.colidx, kind=_lib.KIND_IC0)

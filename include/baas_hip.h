@@ -215,6 +215,10 @@ int baas_plan_dataflow(baas_schedule *s, int enable);
  * size and the dataflow kernel, keeps the fastest.  chosen (host int[5], may
  * be NULL) = mode, rows per unit, threads per workgroup, dataflow, microseconds. */
 int baas_autotune(baas_handle *h, baas_schedule *s, const double *vals, int uplo, int nrhs, int *chosen);
+/* Time the numeric factorisation of an IC0/ILU0 schedule over unit,
+ * schedule and level-set plans on these values and keep the fastest (no
+ * reference counterpart: the reference fixes its schedule at analysis). */
+int baas_autotune_factor(baas_handle *h, baas_schedule *s, const double *a_vals, int *chosen);
 /* The same for FP32 sweeps (whose dataflow units may hold twice the rows). */
 int baas_autotune_f32(baas_handle *h, baas_schedule *s, const float *vals, int uplo, int nrhs, int *chosen);
 

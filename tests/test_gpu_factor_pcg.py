@@ -207,3 +207,19 @@ def test_pcg_same_iterates_on_every_plan_kind(handle, prec):
             np.testing.assert_array_equal(xh, ref[1])
         del solver
         torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("kind,name", [(_lib.KIND_IC0, "C2_lap3d_64_perm1"), (_lib.KIND_ILU0, "C2_convdiff_64")])
+def test_factor_autotune_keeps_bits(handle, kind, name):
+    from baas_amd.api import Schedule
+    announce("factor autotune", kind, name)
+    _, _, A = workload(name)
+    want = O.ic0(A) if kind == _lib.KIND_IC0 else O.ilu0(A)
+    s = Schedule(handle, A.rowptr, A.colidx, kind=kind)
+    av = up(A.values)
+    chosen = s.autotune_factor(av)
+    print("[factor autotune]", name, chosen)
+    assert chosen["mode"] in (1, 2, 3) and chosen["us"] > 0
+    lu = s.ic0(av) if kind == _lib.KIND_IC0 else s.ilu0(av)
+    _status_ok(handle)
+    np.testing.assert_array_equal(down(lu), want)
