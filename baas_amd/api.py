@@ -115,18 +115,31 @@ class Schedule:
         if dataflow is not None:
             L.call("baas_plan_dataflow", self._s, int(bool(dataflow)))
 
+    def replan_tiles(self, row_range, band_levels, block_threads=0, dataflow=None):
+        """TILES plan: units of row_range consecutive rows by band_levels consecutive levels."""
+        L.call("baas_replan_tiles", self._s, int(row_range), int(band_levels), int(block_threads))
+        if dataflow is not None:
+            L.call("baas_plan_dataflow", self._s, int(bool(dataflow)))
+
+    def plan(self):
+        """The plan in use: mode, rows per unit, workgroup size, dataflow, band levels."""
+        c = (C.c_int * 5)()
+        L.call("baas_schedule_plan", self._s, C.cast(c, C.c_void_p))
+        return {"mode": c[0], "rows_per_unit": c[1], "block": c[2], "dataflow": c[3], "band": c[4]}
+
     def autotune(self, vals, uplo=0, nrhs=1):
         """Time candidate plans on these values and keep the fastest."""
         c = (C.c_int * 5)()
         f = "baas_autotune_f32" if vals.dtype == torch.float32 else "baas_autotune"
         L.call(f, self.h.ptr, self._s, _ptr(vals), int(uplo), int(nrhs), C.cast(c, C.c_void_p))
-        return {"mode": c[0], "rows_per_unit": c[1], "block": c[2], "dataflow": c[3], "us": c[4]}
+        return {"mode": c[0], "rows_per_unit": c[1], "block": c[2], "dataflow": c[3], "us": c[4],
+                "band": self.plan()["band"]}
 
     def autotune_factor(self, a_vals):
         """Time the IC0/ILU0 factorisation over plans and keep the fastest."""
         c = (C.c_int * 5)()
         L.call("baas_autotune_factor", self.h.ptr, self._s, _ptr(a_vals), C.cast(c, C.c_void_p))
-        return {"mode": c[0], "rows_per_unit": c[1], "block": c[2], "us": c[4]}
+        return {"mode": c[0], "rows_per_unit": c[1], "block": c[2], "us": c[4], "band": self.plan()["band"]}
 
     def upper_pattern(self):
         rp, ci = C.c_void_p(), C.c_void_p()

@@ -109,6 +109,8 @@ int autotune(baas_handle *h, baas_schedule *s, const T *vals, int uplo, int nrhs
 
 constexpr int BAAS_BARRIER_BASE = 64;
 constexpr int BAAS_COUNTER_INTS = BAAS_BARRIER_BASE + 8 * 32;
+// TILES plans: largest work unit (rows); its values fit the dataflow kernel's LDS
+constexpr int BAAS_TILE_MAX_ROWS = 6144;
 
 struct baas_handle {
     hipStream_t stream = nullptr;
@@ -144,6 +146,7 @@ struct baas_schedule {
     // plans (forward over L, backward over U) and ready flags
     baas::Plan fwd, bwd;
     int plan_gen = 0;       // bumped whenever the plans change (captured graphs re-capture)
+    int tile_band = 16;     // TILES plans: row levels per band
     baas::DBuf<int> flags;  // [n] epoch of the last sweep that finished the row
     baas::DBuf<int> row_level_ptr_d, row_perm_d;  // for the level-synchronous solve
     // IC0 dot plans (built at the first factorisation): per lower entry the

@@ -245,6 +245,59 @@ static int make_levels_plan(Plan &p, const baas_schedule *s, const HostCsr &M, b
     return BAAS_OK;
 }
 
+// SCHEDULE units: merged levels -> groups in level order -> nodes ascending,
+// each merged level cut into units of at most R rows.
+static int schedule_units(const baas_schedule *s, int R, std::vector<int> &order, std::vector<int> &fu) {
+    const int n = s->n;
+    const auto &gp = s->group_ptr, &gs = s->group_set, &pto = s->perm_to_orig, &mp = s->merged_ptr;
+    int q = 0;
+    for (int m = 0; m + 1 < (int)mp.size(); m++) {
+        int start = q;
+        for (int t = mp[m]; t < mp[m + 1]; t++) {
+            int g = pto[t];
+            for (int k = gp[g]; k < gp[g + 1]; k++) order[q++] = gs[k];
+        }
+        for (int b = start; b < q; b += R) fu.push_back(b);
+    }
+    fu.push_back(n);
+    return q == n ? BAAS_OK : BAAS_INVALID_ARG;
+}
+
+// TILES units: the (row level, row id) plane cut into tiles of hb
+// consecutive levels by S consecutive row ids, taken band by band and, in a
+// band, by ascending row range; rows inside a tile ascending.  A parent has a
+// lower level and a lower row id than its child, so it lies in the same
+// tile or in an earlier one: the order is topological.  On matrices whose
+// row numbering follows the geometry (natural orderings of stencils, banded
+// and RCM-ordered matrices) a tile is a compact block of the domain, so a
+// dependency chain crosses few tiles.  Tiles above BAAS_TILE_MAX_ROWS are cut
+// into pieces of consecutive rows (each piece's parents lie in earlier
+// pieces or tiles).
+static int tile_units(const baas_schedule *s, int S, int hb, std::vector<int> &order, std::vector<int> &fu) {
+    const int n = s->n;
+    const auto &lev = s->row_level;
+    if ((int)lev.size() != n || S < 1 || hb < 1) return BAAS_INVALID_ARG;
+    int maxlev = 1;
+    for (int r = 0; r < n; r++) maxlev = std::max(maxlev, lev[r]);
+    const long long nr = (n + (long long)S - 1) / S;
+    const long long ntiles = ((long long)(maxlev - 1) / hb + 1) * nr;
+    if (ntiles > (1LL << 28)) return BAAS_INVALID_ARG;
+    std::vector<int> start((size_t)ntiles + 1, 0);
+    auto tile_of = [&](int r) { return (long long)((lev[r] - 1) / hb) * nr + r / S; };
+    for (int r = 0; r < n; r++) start[tile_of(r) + 1]++;
+    for (long long t = 0; t < ntiles; t++) start[t + 1] += start[t];
+    std::vector<int> next(start.begin(), start.end() - 1);
+    for (int r = 0; r < n; r++) order[next[tile_of(r)]++] = r;
+    for (long long t = 0; t < ntiles; t++) {
+        const int b = start[t], sz = start[t + 1] - b;
+        if (sz == 0) continue;
+        const int pieces = (sz + BAAS_TILE_MAX_ROWS - 1) / BAAS_TILE_MAX_ROWS, per = (sz + pieces - 1) / pieces;
+        for (int q = b; q < b + sz; q += per) fu.push_back(q);
+    }
+    fu.push_back(n);
+    return BAAS_OK;
+}
+
 int build_plans_mode(baas_schedule *s, int mode, int rows_per_unit) {
     s->plan_gen++;
     hipStream_t st = s->h->stream;
@@ -281,24 +334,13 @@ int build_plans_mode(baas_schedule *s, int mode, int rows_per_unit) {
         BAAS_TRY(make_plan(s->bwd, n, bwd_order, bu, up, ui, lp, li, 1, mode, block, st, U, true));
         return BAAS_OK;
     }
-    // SCHEDULE: merged levels -> groups in level order -> nodes ascending
-    int R = rows_per_unit > 0 ? rows_per_unit : block;
-    const auto &gp = s->group_ptr, &gs = s->group_set, &lvp = s->level_ptr, &pto = s->perm_to_orig,
-               &mp = s->merged_ptr;
-    int q = 0;
-    for (int m = 0; m + 1 < (int)mp.size(); m++) {
-        int start = q;
-        for (int t = mp[m]; t < mp[m + 1]; t++) {
-            int g = pto[t];
-            for (int k = gp[g]; k < gp[g + 1]; k++) fwd_order[q++] = gs[k];
-        }
-        for (int b = start; b < q; b += R) fu.push_back(b);
+    if (mode == BAAS_PLAN_TILES) {
+        BAAS_TRY(tile_units(s, rows_per_unit > 0 ? rows_per_unit : 16384, s->tile_band, fwd_order, fu));
+    } else {
+        BAAS_TRY(schedule_units(s, rows_per_unit > 0 ? rows_per_unit : block, fwd_order, fu));
     }
-    (void)lvp;
-    fu.push_back(n);
-    if (q != n) return BAAS_INVALID_ARG;
     for (int i = 0; i < n; i++) bwd_order[i] = fwd_order[n - 1 - i];
-    int nu = (int)fu.size() - 1;
+    const int nu = (int)fu.size() - 1;
     bu.resize(nu + 1);
     for (int u = 0; u <= nu; u++) bu[u] = n - fu[nu - u];
     BAAS_TRY(make_plan(s->fwd, n, fwd_order, fu, lp, li, up, ui, 2, mode, block, st, L, false));
@@ -405,11 +447,30 @@ int build_ic0_plans(baas_schedule *s) {
 }  // namespace baas
 
 extern "C" int baas_replan(baas_schedule *s, int mode, int rows_per_unit, int block_threads) {
-    if (!s || mode < 1 || mode > 3 || rows_per_unit < 0 || block_threads < 0 || block_threads > 1024 ||
+    if (!s || mode < 1 || mode > 4 || rows_per_unit < 0 || block_threads < 0 || block_threads > 1024 ||
         (block_threads % 64) != 0)
         return BAAS_INVALID_ARG;
     if (block_threads) s->opts.block_threads = block_threads;
     s->opts.plan_mode = mode;
     s->opts.rows_per_unit = rows_per_unit;
     return baas::build_plans_mode(s, mode, rows_per_unit);
+}
+
+extern "C" int baas_replan_tiles(baas_schedule *s, int row_range, int band_levels, int block_threads) {
+    if (!s || row_range < 1 || band_levels < 1) return BAAS_INVALID_ARG;
+    const int old = s->tile_band;
+    s->tile_band = band_levels;
+    const int rc = baas_replan(s, BAAS_PLAN_TILES, row_range, block_threads);
+    if (rc != BAAS_OK) s->tile_band = old;
+    return rc;
+}
+
+extern "C" int baas_schedule_plan(const baas_schedule *s, int *plan) {
+    if (!s || !plan) return BAAS_INVALID_ARG;
+    plan[0] = s->fwd.mode;
+    plan[1] = s->opts.rows_per_unit;
+    plan[2] = s->fwd.block;
+    plan[3] = s->fwd.dataflow;
+    plan[4] = s->fwd.mode == BAAS_PLAN_TILES ? s->tile_band : 0;
+    return BAAS_OK;
 }

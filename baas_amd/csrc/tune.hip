@@ -24,8 +24,12 @@ using namespace baas;
 namespace {
 
 struct Cand {
-    int mode, rpu, block, df;
+    int mode, rpu, block, df, band;
 };
+
+// TILES candidates: row ranges by level bands
+constexpr int TILE_RANGES[] = {8192, 16384};
+constexpr int TILE_BANDS[] = {16, 64};
 
 template <class T>
 __global__ void k_fill(int n, T *p, T v) {
@@ -47,27 +51,46 @@ int autotune(baas_handle *h, baas_schedule *s, const T *vals, int uplo, int nrhs
     BAAS_TRY(b.alloc((size_t)n * nrhs));
     BAAS_TRY(x.alloc((size_t)n * nrhs));
     hipLaunchKernelGGL(k_fill<T>, dim3((n * nrhs + 255) / 256), dim3(256), 0, h->stream, n * nrhs, b.p, T(1));
+    // candidates sharing (mode, rows per unit, band) are adjacent: their
+    // plan is built once and only the workgroup size changes
     std::vector<Cand> cands;
     std::vector<int> rpus = {1024, 2048, 4096, 8192, 12288, 16384};
     if (sizeof(T) == 4) rpus.insert(rpus.end(), {24576, 32768});
     for (int rpu : rpus)
         for (int blk : {256, 384, 512, 1024}) {
             if (blk == 1024 && rpu < 4096) continue;
-            if (rpu <= 16384) cands.push_back({BAAS_PLAN_RANGE, rpu, blk, 0});
-            if (nrhs == 1 && blk <= 512) cands.push_back({BAAS_PLAN_RANGE, rpu, blk, 1});
+            if (rpu <= 16384) cands.push_back({BAAS_PLAN_RANGE, rpu, blk, 0, 0});
+            if (nrhs == 1 && blk <= 512) cands.push_back({BAAS_PLAN_RANGE, rpu, blk, 1, 0});
         }
     for (int rpu : {256, 1024, 4096})
-        for (int blk : {256, 1024}) cands.push_back({BAAS_PLAN_SCHEDULE, rpu, blk, 0});
-    for (int blk : {64, 128, 256, 512}) cands.push_back({BAAS_PLAN_LEVELS, 0, blk, 0});
+        for (int blk : {256, 1024}) cands.push_back({BAAS_PLAN_SCHEDULE, rpu, blk, 0, 0});
+    for (int blk : {64, 128, 256, 512}) cands.push_back({BAAS_PLAN_LEVELS, 0, blk, 0, 0});
+    for (int S : TILE_RANGES) {
+        if (S >= 2 * n && S != TILE_RANGES[0]) continue;
+        for (int hb : TILE_BANDS)
+            for (int blk : {256, 512, 1024}) {
+                if (blk != 512) cands.push_back({BAAS_PLAN_TILES, S, blk, 0, hb});
+                if (nrhs == 1 && blk <= 512) cands.push_back({BAAS_PLAN_TILES, S, blk, 1, hb});
+            }
+    }
     hipEvent_t e0, e1;
     BAAS_HIP_TRY(hipEventCreate(&e0));
     BAAS_HIP_TRY(hipEventCreate(&e1));
     float best = 1e30f;
     Cand win = cands[0];
+    const int band0 = s->tile_band;
+    Cand built = {-1, -1, 0, 0, -1};
     for (const Cand &c : cands) {
         if (c.rpu >= 2 * n && c.mode == BAAS_PLAN_RANGE && c.rpu != 1024) continue;
         s->opts.block_threads = c.block;
-        if (build_plans_mode(s, c.mode, c.rpu) != BAAS_OK) continue;
+        if (c.mode != built.mode || c.rpu != built.rpu || c.band != built.band ||
+            (c.mode == BAAS_PLAN_SCHEDULE && c.rpu == 0 && c.block != built.block)) {
+            if (c.mode == BAAS_PLAN_TILES) s->tile_band = c.band;
+            built = {-1, -1, 0, 0, -1};
+            if (build_plans_mode(s, c.mode, c.rpu) != BAAS_OK) continue;
+            built = c;
+        }
+        s->fwd.block = s->bwd.block = c.block;
         Plan &p = uplo ? s->bwd : s->fwd;
         if (c.df && p.df_lds[sizeof(T) == 8] > lds_limit(h)) continue;
         s->fwd.dataflow = s->bwd.dataflow = c.df;
@@ -89,6 +112,7 @@ int autotune(baas_handle *h, baas_schedule *s, const T *vals, int uplo, int nrhs
     s->opts.block_threads = win.block;
     s->opts.plan_mode = win.mode;
     s->opts.rows_per_unit = win.rpu;
+    s->tile_band = win.mode == BAAS_PLAN_TILES ? win.band : band0;
     BAAS_TRY(build_plans_mode(s, win.mode, win.rpu));
     s->fwd.dataflow = s->bwd.dataflow = win.df;
     if (chosen) {
@@ -115,19 +139,30 @@ static int autotune_factor(baas_handle *h, baas_schedule *s, const double *a_val
     BAAS_TRY(lu.alloc(s->nnz_full));
     std::vector<Cand> cands;
     for (int rpu : {1024, 2048, 4096, 8192})
-        for (int blk : {256, 512}) cands.push_back({BAAS_PLAN_RANGE, rpu, blk, 0});
+        for (int blk : {256, 512}) cands.push_back({BAAS_PLAN_RANGE, rpu, blk, 0, 0});
     for (int rpu : {0, 1024})
-        for (int blk : {256, 1024}) cands.push_back({BAAS_PLAN_SCHEDULE, rpu, blk, 0});
-    for (int blk : {64, 128, 256, 512}) cands.push_back({BAAS_PLAN_LEVELS, 0, blk, 0});
+        for (int blk : {256, 1024}) cands.push_back({BAAS_PLAN_SCHEDULE, rpu, blk, 0, 0});
+    for (int blk : {64, 128, 256, 512}) cands.push_back({BAAS_PLAN_LEVELS, 0, blk, 0, 0});
+    for (int hb : {16, 64})
+        for (int blk : {256, 512}) cands.push_back({BAAS_PLAN_TILES, 16384, blk, 0, hb});
     hipEvent_t e0, e1;
     BAAS_HIP_TRY(hipEventCreate(&e0));
     BAAS_HIP_TRY(hipEventCreate(&e1));
     float best = 1e30f;
     Cand win = cands[0];
+    const int band0 = s->tile_band;
+    Cand built = {-1, -1, 0, 0, -1};
     for (const Cand &c : cands) {
         if (c.mode == BAAS_PLAN_RANGE && c.rpu >= 2 * s->n && c.rpu != 1024) continue;
         s->opts.block_threads = c.block;
-        if (build_plans_mode(s, c.mode, c.rpu) != BAAS_OK) continue;
+        if (c.mode != built.mode || c.rpu != built.rpu || c.band != built.band ||
+            (c.mode == BAAS_PLAN_SCHEDULE && c.rpu == 0 && c.block != built.block)) {
+            if (c.mode == BAAS_PLAN_TILES) s->tile_band = c.band;
+            built = {-1, -1, 0, 0, -1};
+            if (build_plans_mode(s, c.mode, c.rpu) != BAAS_OK) continue;
+            built = c;
+        }
+        s->fwd.block = s->bwd.block = c.block;
         float t_best = 1e30f;
         bool ok = true;
         for (int rep = 0; rep < 3 && ok; rep++) {
@@ -147,6 +182,7 @@ static int autotune_factor(baas_handle *h, baas_schedule *s, const double *a_val
     s->opts.block_threads = win.block;
     s->opts.plan_mode = win.mode;
     s->opts.rows_per_unit = win.rpu;
+    s->tile_band = win.mode == BAAS_PLAN_TILES ? win.band : band0;
     BAAS_TRY(build_plans_mode(s, win.mode, win.rpu));
     if (chosen) {
         chosen[0] = win.mode;

@@ -12,9 +12,11 @@ the timed region (analysis time, reported separately, as the reference's
 Scheduling_Time).
 
 Further fields (not the headline): the same solve on the seed-1 random
-symmetric permutation, 32 right-hand sides, IC0 and ILU0 (64^3), and ICCG
+symmetric permutation, 32 right-hand sides, IC0 and ILU0 (64^3), ICCG
 time to ||r||/||r0|| < 1e-8 on the kNN stiffness matrix (n = 36000) with
-the FP64 factor and with the FP32 factor.
+the FP64 factor and with the FP32 factor, batched ICCG, and rocSPARSE's
+csrsv / csric0 / csrilu0 timed on the same inputs as a vendor-library
+comparator (in place of the reference's MKL baselines).
 """
 from __future__ import annotations
 
@@ -75,6 +77,124 @@ def cpu_baseline(L, b, seconds):
                       f"ascending FMA sums), {dt * 1e3:.3f} ms per solve, ~{seconds:.0f} s"}
 
 
+def apply_plan(s, plan):
+    """Re-pack schedule s into the plan a previous autotune (or --plan) chose."""
+    if plan["mode"] == 4:
+        s.replan_tiles(plan["rows_per_unit"], plan["band"], plan["block"], dataflow=bool(plan["dataflow"]))
+    else:
+        s.replan(plan["mode"], plan["rows_per_unit"], plan["block"], dataflow=bool(plan["dataflow"]))
+
+
+def rocsparse_comparators(L, b, x_ours, A, lu_ours, Cd, luc_ours, steps=50):
+    """rocSPARSE on the same inputs, timed as a black box: csrsv (FP64 lower
+    solve, analysis timed apart), csric0 on A and csrilu0 on the
+    convection-diffusion matrix.  Its summation order is its own, so the
+    largest relative difference from this library's (bit-exact) result is
+    reported, not an equality.  Factorisations overwrite their input, so each
+    timed call copies A's values first; the copy's own time is subtracted."""
+    import ctypes as C
+    import torch
+    try:
+        rs = C.CDLL("librocsparse.so.1")
+    except OSError:
+        try:
+            rs = C.CDLL("/opt/rocm/lib/librocsparse.so.1")
+        except OSError as e:
+            return {"error": str(e)}
+    P, I = C.c_void_p, C.c_int
+    NONE, REUSE, AUTO = 111, 0, 0
+
+    def chk(st, what):
+        if st != 0:
+            raise RuntimeError(f"rocsparse {what}: status {st}")
+
+    def dptr(t):
+        return P(t.data_ptr())
+
+    def reldiff(a, b_):
+        a, b_ = a.double(), b_.double()
+        return float(((a - b_).abs().max() / b_.abs().max().clamp_min(1e-300)).item())
+
+    handle = P()
+    chk(rs.rocsparse_create_handle(C.byref(handle)), "create_handle")
+    chk(rs.rocsparse_set_stream(handle, P(torch.cuda.current_stream().cuda_stream)), "set_stream")
+    out = {}
+    objs = []
+    try:
+        # lower triangular solve
+        descr, info, sz = P(), P(), C.c_size_t()
+        chk(rs.rocsparse_create_mat_descr(C.byref(descr)), "descr")
+        objs.append(("rocsparse_destroy_mat_descr", descr))
+        chk(rs.rocsparse_set_mat_fill_mode(descr, I(0)), "fill mode")
+        chk(rs.rocsparse_create_mat_info(C.byref(info)), "info")
+        objs.append(("rocsparse_destroy_mat_info", info))
+        rp = torch.as_tensor(L.rowptr, device="cuda")
+        ci = torch.as_tensor(L.colidx, device="cuda")
+        lv = torch.as_tensor(L.values, device="cuda")
+        m, nnz = I(L.n), I(L.nnz)
+        chk(rs.rocsparse_dcsrsv_buffer_size(handle, I(NONE), m, nnz, descr, dptr(lv), dptr(rp), dptr(ci), info,
+                                            C.byref(sz)), "csrsv_buffer_size")
+        buf = torch.empty(max(sz.value, 8), dtype=torch.uint8, device="cuda")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        chk(rs.rocsparse_dcsrsv_analysis(handle, I(NONE), m, nnz, descr, dptr(lv), dptr(rp), dptr(ci), info,
+                                         I(REUSE), I(AUTO), dptr(buf)), "csrsv_analysis")
+        torch.cuda.synchronize()
+        t_an = time.perf_counter() - t0
+        y = torch.empty_like(b)
+        alpha = C.c_double(1.0)
+        solve = lambda: chk(rs.rocsparse_dcsrsv_solve(handle, I(NONE), m, nnz, C.byref(alpha), descr, dptr(lv),
+                                                       dptr(rp), dptr(ci), info, dptr(b), dptr(y), I(AUTO),
+                                                       dptr(buf)), "csrsv_solve")
+        _, ms, _ = time_events(solve, steps, 5)
+        by = bytes_sptrsv(L.n, L.nnz)
+        out["csrsv_lap3d64"] = {"ms": round(ms, 4), "GB/s": round(by / (ms * 1e-3) / 1e9, 2),
+                                "analysis_s": round(t_an, 4), "max_rel_diff": reldiff(y, x_ours)}
+
+        def factor(name, M, ours, fn_pre):
+            d2, inf2, sz2 = P(), P(), C.c_size_t()
+            chk(rs.rocsparse_create_mat_descr(C.byref(d2)), "descr")
+            objs.append(("rocsparse_destroy_mat_descr", d2))
+            chk(rs.rocsparse_create_mat_info(C.byref(inf2)), "info")
+            objs.append(("rocsparse_destroy_mat_info", inf2))
+            arp = torch.as_tensor(M.rowptr, device="cuda")
+            aci = torch.as_tensor(M.colidx, device="cuda")
+            av = torch.as_tensor(M.values, device="cuda")
+            work = av.clone()
+            mm, nz = I(M.n), I(M.nnz)
+            chk(getattr(rs, f"rocsparse_d{fn_pre}_buffer_size")(handle, mm, nz, d2, dptr(work), dptr(arp), dptr(aci),
+                                                               inf2, C.byref(sz2)), f"{fn_pre}_buffer_size")
+            fb = torch.empty(max(sz2.value, 8), dtype=torch.uint8, device="cuda")
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            chk(getattr(rs, f"rocsparse_d{fn_pre}_analysis")(handle, mm, nz, d2, dptr(work), dptr(arp), dptr(aci),
+                                                            inf2, I(REUSE), I(AUTO), dptr(fb)), f"{fn_pre}_analysis")
+            torch.cuda.synchronize()
+            t_a = time.perf_counter() - t0
+
+            def run():
+                work.copy_(av)
+                chk(getattr(rs, f"rocsparse_d{fn_pre}")(handle, mm, nz, d2, dptr(work), dptr(arp), dptr(aci), inf2,
+                                                        I(AUTO), dptr(fb)), fn_pre)
+            _, ms_all, _ = time_events(run, 20, 3)
+            _, ms_copy, _ = time_events(lambda: work.copy_(av), 20, 3)
+            run()
+            torch.cuda.synchronize()
+            rows = np.repeat(np.arange(M.n), np.diff(M.rowptr))
+            sel = torch.as_tensor(M.colidx <= rows if fn_pre == "csric0" else np.ones(M.nnz, bool), device="cuda")
+            out[name] = {"ms": round(ms_all - ms_copy, 4), "analysis_s": round(t_a, 4),
+                         "max_rel_diff": reldiff(work[sel], ours[sel])}
+
+        factor("csric0_lap3d64", A, lu_ours, "csric0")
+        factor("csrilu0_convdiff64", Cd, luc_ours, "csrilu0")
+    finally:
+        torch.cuda.synchronize()
+        for fn, o in reversed(objs):
+            getattr(rs, fn)(o)
+        rs.rocsparse_destroy_handle(handle)
+    return out
+
+
 def traffic_from_profiles():
     """HBM bytes per launch measured with rocprofv3 --pmc (profiles/traffic.json), or None."""
     p = os.path.join(ROOT, "profiles", "traffic.json")
@@ -94,7 +214,8 @@ def main():
     ap.add_argument("--quick", action="store_true", help="headline only, no secondary measurements")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--plan", default="auto",
-                    help="auto (autotune) or MODE:ROWS_PER_UNIT:BLOCK:DATAFLOW, MODE 1 = RANGE, 2 = SCHEDULE")
+                    help="auto (autotune) or MODE:ROWS_PER_UNIT:BLOCK:DATAFLOW[:BAND], MODE 1 = RANGE, "
+                         "2 = SCHEDULE, 3 = LEVELS, 4 = TILES (ROWS_PER_UNIT = row range, BAND = levels)")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run 3 plain copy kernels of this many MiB (HBM counter calibration)")
     args = ap.parse_args()
@@ -130,9 +251,10 @@ def main():
     if args.plan == "auto":
         plan = s.autotune(lv)
     else:
-        mode, rpu, blk, df = (int(v) for v in args.plan.split(":"))
-        s.replan(mode, rpu, blk, dataflow=df)
-        plan = {"mode": mode, "rows_per_unit": rpu, "block": blk, "dataflow": df, "us": None}
+        f = [int(v) for v in args.plan.split(":")]
+        plan = {"mode": f[0], "rows_per_unit": f[1], "block": f[2], "dataflow": f[3],
+                "band": f[4] if len(f) > 4 else 16, "us": None}
+        apply_plan(s, plan)
     t_tune = time.perf_counter() - t0
     print(f"[bench] analysis {t_an:.3f} s, autotune {t_tune:.2f} s, plan {plan}", file=sys.stderr, flush=True)
 
@@ -203,6 +325,13 @@ def main():
         h.check()
         extra["ilu0_convdiff64"] = {"ms": round(m, 4),
                                     "GB/s": round((20 * Cd.nnz + 8 * Cd.n) / (m * 1e-3) / 1e9, 2), "plan": fplan}
+        # the vendor library on the same inputs (headline plan restored first)
+        apply_plan(s, plan)
+        s.solve(lv, bd, x)
+        sa.ic0(av, lu)
+        sc.ilu0(cv, luc)
+        h.check()
+        extra["rocsparse"] = rocsparse_comparators(L, bd, x, A, lu, Cd, luc)
         # ICCG time to 1e-8, FP64 factor vs FP32 factor (kNN 3x3-block stiffness, n = 36000)
         K, kb = g.knn_stiffness()
         sk = Schedule(h, K.rowptr, K.colidx, kind=_lib.KIND_IC0)

@@ -69,7 +69,19 @@ enum baas_export {
  * sharing each level, with a device-wide barrier between levels: the GPU
  * form of the reference's level-set solve (merge/SpTRSV.cpp:41-80), best
  * when levels are few and wide. */
-enum baas_plan_mode { BAAS_PLAN_AUTO = 0, BAAS_PLAN_RANGE = 1, BAAS_PLAN_SCHEDULE = 2, BAAS_PLAN_LEVELS = 3 };
+/* TILES: work units are tiles of the (row level, row id) plane -- a band of
+ * consecutive levels by a range of consecutive row ids, band by band -- so
+ * on matrices numbered along their geometry (natural stencil orderings,
+ * banded and RCM-ordered matrices) a unit is a compact block of the domain
+ * and a dependency chain crosses few units.  rows_per_unit is the row range;
+ * the band height is set with baas_replan_tiles (default 16 levels). */
+enum baas_plan_mode {
+    BAAS_PLAN_AUTO = 0,
+    BAAS_PLAN_RANGE = 1,
+    BAAS_PLAN_SCHEDULE = 2,
+    BAAS_PLAN_LEVELS = 3,
+    BAAS_PLAN_TILES = 4
+};
 
 typedef struct baas_opts {
     int ncore;            /* thread count the reference rule is evaluated at (default 8) */
@@ -203,17 +215,26 @@ int baas_batch_pcg(baas_batch *b, const double *B, double *X, double tol, int ma
  * (all systems' rows of one level at a time) won. */
 int baas_batch_info(const baas_batch *b, double *times, int *rpu);
 
-/* Re-pack a schedule's execution plan (RANGE or SCHEDULE, rows per unit,
- * threads per workgroup: a multiple of 64 up to 1024). */
+/* Re-pack a schedule's execution plan (baas_plan_mode RANGE, SCHEDULE,
+ * LEVELS or TILES; rows per unit; threads per workgroup: a multiple of 64 up
+ * to 1024). */
 int baas_replan(baas_schedule *s, int mode, int rows_per_unit, int block_threads);
+/* Re-pack into TILES units of row_range consecutive row ids by band_levels
+ * consecutive row levels (no reference counterpart: a packing of the
+ * reference's level sets for GPU work units). */
+int baas_replan_tiles(baas_schedule *s, int row_range, int band_levels, int block_threads);
+/* The plan in use (host int[5]): mode, rows per unit (TILES: row range),
+ * threads per workgroup, dataflow, band levels (TILES, else 0). */
+int baas_schedule_plan(const baas_schedule *s, int *plan);
 /* Single-RHS sweeps of this schedule use the dataflow kernel (per-row LDS
  * waits, prefetch waves for parents of other units) when enable != 0 and the
  * unit's slots fit in LDS. */
 int baas_plan_dataflow(baas_schedule *s, int enable);
 /* Autotune the execution plan for sweeps with `uplo` and `nrhs` on these
- * values: times RANGE and SCHEDULE packings over rows-per-unit, workgroup
- * size and the dataflow kernel, keeps the fastest.  chosen (host int[5], may
- * be NULL) = mode, rows per unit, threads per workgroup, dataflow, microseconds. */
+ * values: times RANGE, SCHEDULE, LEVELS and TILES packings over their unit
+ * sizes, workgroup size and the dataflow kernel, keeps the fastest.  chosen
+ * (host int[5], may be NULL) = mode, rows per unit, threads per workgroup,
+ * dataflow, microseconds (the TILES band: baas_schedule_plan). */
 int baas_autotune(baas_handle *h, baas_schedule *s, const double *vals, int uplo, int nrhs, int *chosen);
 /* Time the numeric factorisation of an IC0/ILU0 schedule over unit,
  * schedule and level-set plans on these values and keep the fastest (no

@@ -65,7 +65,8 @@ def test_reduced_dag_matches_oracle(handle):
 PLANS = [(1, 0, 256, False), (1, 1024, 256, False), (1, 16384, 512, False), (2, 0, 256, False),
          (2, 64, 64, False), (2, 1024, 1024, False), (1, 4096, 256, True), (1, 12288, 512, True),
          (1, 2048, 192, True), (2, 256, 256, True), (2, 2048, 512, True), (1, 8192, 1024, True),
-         (3, 0, 256, False), (3, 0, 1024, False)]
+         (3, 0, 256, False), (3, 0, 1024, False), (4, 16384, 256, True), (4, 32768, 512, True),
+         (4, 8192, 1024, False)]
 SOLVE_CASES = ["C1_lap2d_128", "C2_lap3d_64", "C2_lap3d_64_perm1", "C5_diff3d_48", "bcsstk18", "bcsstk38"]
 
 
@@ -116,7 +117,7 @@ def test_multi_rhs_bit_exact(handle, k):
         s = Schedule(handle, L.rowptr, L.colidx, kind=0)
         B = g.rhs_uniform(L.n, 3, k=k)
         X_ref = O.sptrsv_lower_multi(L, B)
-        for mode, rpu, blk in [(1, 0, 256), (1, 256, 256), (2, 0, 256), (3, 0, 512)]:
+        for mode, rpu, blk in [(1, 0, 256), (1, 256, 256), (2, 0, 256), (3, 0, 512), (4, 16384, 256)]:
             announce("multi", name, k, mode, rpu, blk)
             s.replan(mode, rpu, blk)
             X = down(s.solve(up(L.values), up(B)))
@@ -136,8 +137,8 @@ def test_f32_bit_exact(handle):
         L, _, _ = workload(name)
         s = Schedule(handle, L.rowptr, L.colidx, kind=0)
         b = g.rhs_uniform(L.n, 1)
-        for mode, df in ((1, False), (1, True), (3, False)):
-            s.replan(mode, 4096 if mode == 1 else 0, 256, dataflow=df)
+        for mode, df in ((1, False), (1, True), (3, False), (4, True)):
+            s.replan(mode, {1: 4096, 3: 0, 4: 16384}[mode], 256, dataflow=df)
             x = down(s.solve(up(L.values, torch.float32), up(b, torch.float32)))
             handle.check()
             np.testing.assert_array_equal(x, O.sptrsv_lower_f32(L, b))
@@ -145,6 +146,32 @@ def test_f32_bit_exact(handle):
         uv32 = s.transpose_values(up(L.values)).to(torch.float32)
         y = down(s.solve(uv32, up(b, torch.float32), uplo=1))
         np.testing.assert_array_equal(y, O.sptrsv_upper_f32(U, b))
+
+
+@pytest.mark.parametrize("name", ["C1_lap2d_128", "C2_lap3d_64", "C3_knn", "bcsstk38"])
+def test_tile_plans_bit_exact(handle, name):
+    """TILES plans over row ranges and level bands, from one-level bands to
+    a single row range, forward and backward, dataflow and unit kernels."""
+    from baas_amd.api import Schedule
+    L, _, _ = workload(name)
+    s = Schedule(handle, L.rowptr, L.colidx, kind=0)
+    b = g.rhs_uniform(L.n, 5)
+    x_ref = O.sptrsv_lower(L, b)
+    U = g.Csr.from_scipy(L.to_scipy().T)
+    y_ref = O.sptrsv_upper(U, b)
+    lv, bd = up(L.values), up(b)
+    uv = s.transpose_values(lv)
+    for rr, band, blk, df in [(8192, 8, 384, True), (32768, 32, 256, True), (65536, 1, 512, True),
+                              (1 << 30, 16, 256, True), (4096, 64, 512, False), (16384, 16, 1024, False)]:
+        announce("tiles", name, rr, band, blk, df)
+        s.replan_tiles(rr, band, blk, dataflow=df)
+        pl = s.plan()
+        assert pl["mode"] == 4 and pl["band"] == band and pl["block"] == blk
+        x = s.solve(lv, bd)
+        y = s.solve(uv, bd, uplo=1)
+        handle.check()
+        np.testing.assert_array_equal(down(x), x_ref, err_msg=f"fwd {name} {rr} {band} {blk} {df}")
+        np.testing.assert_array_equal(down(y), y_ref, err_msg=f"bwd {name} {rr} {band} {blk} {df}")
 
 
 def test_division_and_sqrt_are_ieee(handle):
@@ -216,7 +243,7 @@ def test_autotune_keeps_bits(handle, name):
         vals = lv if uplo == 0 else s.transpose_values(lv)
         chosen = s.autotune(vals, uplo=uplo)
         print("[autotune]", name, uplo, chosen)
-        assert chosen["mode"] in (1, 2, 3) and chosen["us"] > 0
+        assert chosen["mode"] in (1, 2, 3, 4) and chosen["us"] > 0
         b = g.rhs_uniform(L.n, 0)
         got = down(s.solve(vals, up(b), uplo=uplo))
         handle.check()
