@@ -113,13 +113,13 @@ class Schedule:
     def replan(self, mode, rows_per_unit=0, block_threads=0, dataflow=None):
         L.call("baas_replan", self._s, int(mode), int(rows_per_unit), int(block_threads))
         if dataflow is not None:
-            L.call("baas_plan_dataflow", self._s, int(bool(dataflow)))
+            L.call("baas_plan_dataflow", self._s, int(dataflow))
 
     def replan_tiles(self, row_range, band_levels, block_threads=0, dataflow=None):
         """TILES plan: units of row_range consecutive rows by band_levels consecutive levels."""
         L.call("baas_replan_tiles", self._s, int(row_range), int(band_levels), int(block_threads))
         if dataflow is not None:
-            L.call("baas_plan_dataflow", self._s, int(bool(dataflow)))
+            L.call("baas_plan_dataflow", self._s, int(dataflow))
 
     def plan(self):
         """The plan in use: mode, rows per unit, workgroup size, dataflow, band levels."""

@@ -95,7 +95,12 @@ struct Plan {
     // LEVELS plans: row level sets in execution order (meta holds the rows)
     DBuf<int> lptr;       // [nlev+1]
     int nlev = 0;
-    int dataflow = 0;     // use the dataflow kernel for single-RHS sweeps
+    int dataflow = 0;     // single-RHS sweeps: 1 dataflow kernel, 2 lane kernel
+    // lane kernel: per position, the offset of its entries in its unit's LDS
+    // entry area; per unit, the running entry count; LDS bytes (FP32, FP64)
+    DBuf<int> lane_eoff;  // [n]
+    DBuf<int> lane_eptr;  // [nunits+1]
+    size_t lane_lds[2] = {0, 0};
     DBuf<unsigned char> xp_raw;  // 2 x n published values (dataflow kernel)
     int xp_type = 0;             // sizeof the value type the halves are armed for
     DBuf<int> pcount;            // [1] sweeps run on this plan
@@ -105,6 +110,28 @@ struct Plan {
 
 template <class T>
 int autotune(baas_handle *h, baas_schedule *s, const T *vals, int uplo, int nrhs, int *chosen);
+
+// LDS layout of one unit in the lane kernel (byte offsets, 8-byte aligned
+// sections): value slots (keys, then external parents), per position b, the
+// diagonal, {entry offset, count | external-child flag}, LDS slot and row,
+// then the unit's off-diagonal values and their slots.
+struct LaneLayout {
+    size_t sl, bv, dv, rec, slot, row, ev, es, total;
+};
+__host__ __device__ inline size_t lane_align8(size_t v) { return (v + 7) & ~size_t(7); }
+__host__ __device__ inline LaneLayout lane_layout(size_t nk, size_t ne, size_t E, size_t ts) {
+    LaneLayout l;
+    l.sl = 0;
+    l.bv = lane_align8((nk + ne) * ts);
+    l.dv = l.bv + lane_align8(nk * ts);
+    l.rec = l.dv + lane_align8(nk * ts);
+    l.slot = l.rec + nk * 8;
+    l.row = l.slot + lane_align8(nk * 4);
+    l.ev = l.row + lane_align8(nk * 4);
+    l.es = l.ev + lane_align8(E * ts);
+    l.total = l.es + lane_align8(E * 4);
+    return l;
+}
 }  // namespace baas
 
 constexpr int BAAS_BARRIER_BASE = 64;

@@ -171,6 +171,24 @@ int make_plan(Plan &p, int n, const std::vector<int> &order, const std::vector<i
             p.max_deg = std::max(p.max_deg, re - rb - 1);
             meta[4 * (size_t)q + 3] = key[r] - key_begin[u];
         }
+    // lane kernel: unit-relative entry offsets and the LDS footprint
+    std::vector<int> lane_eoff(n), lane_eptr(nunits + 1, 0);
+    size_t lane4 = 0, lane8 = 0;
+    for (int u = 0; u < nunits; u++) {
+        int acc = 0;
+        for (int q = unit_ptr[u]; q < unit_ptr[u + 1]; q++) {
+            lane_eoff[q] = acc;
+            acc += meta[4 * (size_t)q + 2];
+        }
+        lane_eptr[u + 1] = lane_eptr[u] + acc;
+        const size_t nk = unit_ptr[u + 1] - unit_ptr[u], ne = ext_ptr[u + 1] - ext_ptr[u];
+        lane4 = std::max(lane4, lane_layout(nk, ne, acc, 4).total);
+        lane8 = std::max(lane8, lane_layout(nk, ne, acc, 8).total);
+    }
+    BAAS_TRY(upload(p.lane_eoff, lane_eoff, st));
+    BAAS_TRY(upload(p.lane_eptr, lane_eptr, st));
+    p.lane_lds[0] = lane4;
+    p.lane_lds[1] = lane8;
     BAAS_TRY(p.pcount.alloc(1));
     if (hipMemsetAsync(p.pcount.p, 0, sizeof(int), st) != hipSuccess) return BAAS_HIP_ERROR;
     p.xp_raw.free();

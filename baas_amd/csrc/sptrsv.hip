@@ -447,6 +447,180 @@ __global__ __launch_bounds__(512) void k_sweep_df(DfArgs a, const T *__restrict_
     }
 }
 
+// ---------------------------------------------------------------------------
+// Lane kernel (dataflow plans, variant 2).  Same protocol and the same
+// summation order as k_sweep_df, with three differences:
+//  * the unit's row records and off-diagonal entries are copied into LDS
+//    when the unit starts, so the solving loop issues no global loads and
+//    never waits on the memory counter;
+//  * every lane walks its own positions (position p, p + stride, ...) in one
+//    flat loop, so it moves to its next row as soon as its current row is
+//    solved instead of waiting for the other lanes of its wavefront;
+//  * a lane publishes its own row for later units (the value is the flag,
+//    as in the outgoing exchange), so only wavefront 0 exchanges (incoming).
+template <class T, bool UPPER, int CH>
+__device__ __forceinline__ void lane_chunk(const T *lev, const int *les, int eo, int k, int nc, int (&cs)[CH],
+                                           T (&cv)[CH]) {
+    const int last = nc > 0 ? nc - 1 : 0;
+#pragma unroll
+    for (int t = 0; t < CH; t++) {
+        const int i = eo + k + (t < last ? t : last);
+        cs[t] = nc > 0 ? les[i] : 0;
+        cv[t] = nc > 0 ? lev[i] : T(0);
+    }
+}
+
+template <class T, bool UPPER, int CH>
+__device__ __forceinline__ void lane_solve(const DfArgs &a, T *sl, const T *lb, const T *lg, const int2 *lrec,
+                                           const int *lslot, const int *lrow, const T *lev, const int *les, T *pub,
+                                           T *nxt, int p, int stride, int nk) {
+    bool have = p < nk;
+    int eo = 0, cnt = 0, k = 0, nc = 0, pc = 0, slot = 0, row = 0;
+    bool ext = false;
+    T s = T(0), bv = T(0), dv = T(1);
+    int cs[CH];
+    T cv[CH];
+    long long spins = 0;
+    if (have) {
+        const int2 rc = lrec[p];
+        eo = rc.x;
+        cnt = rc.y & 0x7fffffff;
+        ext = rc.y < 0;
+        slot = lslot[p];
+        row = lrow[p];
+        bv = lb[p];
+        dv = lg[p];
+        nc = cnt < CH ? cnt : CH;
+        lane_chunk<T, UPPER, CH>(lev, les, eo, 0, nc, cs, cv);
+    }
+    // One loop whose exit is uniform across the wavefront and whose body is
+    // a single divergent step, so no lane waits at an inner loop's exit for
+    // the others: a lane whose row is solved starts its next row in the
+    // same iteration in which the others keep polling.
+    while (__ballot(have) != 0) {
+        if (!have) continue;
+        {
+            T v[CH];
+#pragma unroll
+            for (int t = 0; t < CH; t++) v[t] = ld_lds(&sl[cs[t]]);
+#pragma unroll
+            for (int t = 0; t < CH; t++) {
+                if (t == pc && t < nc && !Sentinel<T>::is(v[t])) {
+                    s = UPPER ? fma(v[t], cv[t], s) : fma(cv[t], v[t], s);
+                    pc++;
+                }
+            }
+        }
+        if (pc == nc) {
+            k += nc;
+            if (k < cnt) {  // rows longer than one chunk
+                nc = cnt - k < CH ? cnt - k : CH;
+                pc = 0;
+                lane_chunk<T, UPPER, CH>(lev, les, eo, k, nc, cs, cv);
+            } else {
+                const T xv = (bv - s) / dv;
+                st_lds(&sl[slot], xv);
+                if (ext) {
+                    st_agent(&pub[row], xv);
+                    nxt[row] = Sentinel<T>::value();
+                }
+                p += stride;
+                have = p < nk;
+                if (have) {
+                    const int2 rc = lrec[p];
+                    eo = rc.x;
+                    cnt = rc.y & 0x7fffffff;
+                    ext = rc.y < 0;
+                    slot = lslot[p];
+                    row = lrow[p];
+                    bv = lb[p];
+                    dv = lg[p];
+                    k = 0;
+                    pc = 0;
+                    s = T(0);
+                    spins = 0;
+                    nc = cnt < CH ? cnt : CH;
+                    lane_chunk<T, UPPER, CH>(lev, les, eo, 0, nc, cs, cv);
+                }
+            }
+        } else if (++spins > a.spin_limit) {
+            raise_status(a.status, BAAS_TIMEOUT, row);
+            pc = nc;  // give up on this row's remaining parents
+            cnt = k + nc;
+        }
+    }
+}
+
+template <class T, bool UPPER, int CH>
+__global__ __launch_bounds__(1024) void k_sweep_lane(DfArgs a, const int *__restrict__ eoff,
+                                                    const int *__restrict__ eptr, const T *__restrict__ vals,
+                                                    const T *__restrict__ b, T *x, T *xp) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ int s_unit;
+    if (a.skip && ld_agent(a.skip)) return;
+    const int epoch = read_epoch(a.ctx);
+    const int par = ld_agent(a.pcount) & 1;
+    T *pub = xp + (size_t)par * a.n;
+    T *nxt = xp + (size_t)(par ^ 1) * a.n;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (;;) {
+        const int u = claim_unit(a.ctx.ticket, &s_unit);
+        if (u >= a.nunits) break;
+        const int q0 = a.unit_ptr[u], q1 = a.unit_ptr[u + 1], nk = q1 - q0;
+        const int e0 = a.ext_ptr[u], ne = a.ext_ptr[u + 1] - e0;
+        const int kb = a.key_begin[u];
+        const LaneLayout lay = lane_layout(nk, ne, eptr[u + 1] - eptr[u], sizeof(T));
+        T *sl = reinterpret_cast<T *>(smem_raw + lay.sl);
+        T *lb = reinterpret_cast<T *>(smem_raw + lay.bv);
+        T *lg = reinterpret_cast<T *>(smem_raw + lay.dv);
+        int2 *lrec = reinterpret_cast<int2 *>(smem_raw + lay.rec);
+        int *lslot = reinterpret_cast<int *>(smem_raw + lay.slot);
+        int *lrow = reinterpret_cast<int *>(smem_raw + lay.row);
+        T *lev = reinterpret_cast<T *>(smem_raw + lay.ev);
+        int *les = reinterpret_cast<int *>(smem_raw + lay.es);
+        for (int i = threadIdx.x; i < nk + ne; i += blockDim.x) st_lds(&sl[i], Sentinel<T>::value());
+        for (int q = threadIdx.x; q < nk; q += blockDim.x) {
+            const int4 m = a.meta[q0 + q];
+            const int cnt = m.z, eo = eoff[q0 + q];
+            lb[q] = b[m.x & 0x7fffffff];
+            lg[q] = vals[UPPER ? m.y - cnt : m.y + cnt];
+            lrec[q] = make_int2(eo, cnt | (m.x & (int)0x80000000));
+            lslot[q] = m.w;
+            lrow[q] = m.x & 0x7fffffff;
+            for (int e = 0; e < cnt; e++) {
+                const int j = UPPER ? m.y - e : m.y + e;
+                lev[eo + e] = vals[j];
+                les[eo + e] = a.scol[j];
+            }
+        }
+        __syncthreads();
+        if (wv == 0) {
+            df_incoming<T>(a, pub, sl, a.ext_cols + e0, nk, ne, lane);
+        } else {
+            lane_solve<T, UPPER, CH>(a, sl, lb, lg, lrec, lslot, lrow, lev, les, pub, nxt, (wv - 1) * 64 + lane,
+                                     (nw - 1) * 64, nk);
+        }
+        __syncthreads();
+        // the unit's solution, slot i holding the row with key kb + i
+        for (int i = threadIdx.x; i < nk; i += blockDim.x) {
+            const int key = kb + i;
+            const int row = a.keymode == 0 ? key : a.keymode == 1 ? a.n - 1 - key : a.key_row[key];
+            x[row] = sl[i];
+        }
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const int d = atomicAdd(a.ctx.done, 1);
+        if (d == (int)gridDim.x - 1) {
+            atomicExch(a.ctx.ticket, 0);
+            atomicExch(a.ctx.epoch, epoch == 0x7fffffff ? 1 : epoch + 1);
+            atomicAdd(a.pcount, 1);
+            __threadfence();
+            atomicExch(a.ctx.done, 0);
+        }
+    }
+}
+
 template <class T>
 __global__ void k_fill_sentinel(size_t n, T *p) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -544,6 +718,22 @@ static int launch_df(baas_handle *h, const Plan &p, const DfArgs &a, const T *va
     if (grid > p.nunits) grid = p.nunits;
     if (grid < 1) grid = 1;
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(p.block), lds, stream, a, vals, b, x, xp);
+    BAAS_HIP_TRY(hipGetLastError());
+    return BAAS_OK;
+}
+
+template <class T, bool UP>
+static int launch_lane(baas_handle *h, const Plan &p, const DfArgs &a, const T *vals, const T *b, T *x, T *xp,
+                       size_t lds, hipStream_t stream) {
+    auto kern = p.max_deg <= 4 ? k_sweep_lane<T, UP, 4> : k_sweep_lane<T, UP, 8>;
+    int per_cu = 0;
+    BAAS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, p.block, lds));
+    if (per_cu <= 0) return BAAS_UNSUPPORTED;
+    long long grid = (long long)per_cu * h->num_cus;
+    if (grid > p.nunits) grid = p.nunits;
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(p.block), lds, stream, a, p.lane_eoff.p, p.lane_eptr.p, vals,
+                       b, x, xp);
     BAAS_HIP_TRY(hipGetLastError());
     return BAAS_OK;
 }
@@ -720,7 +910,9 @@ int sweep(baas_handle *h, baas_schedule *s, int uplo, const T *vals, const T *b,
     if ((const void *)b == (const void *)x) return BAAS_INVALID_ARG;
     Plan &p = uplo ? s->bwd : s->fwd;
     if (p.mode == BAAS_PLAN_LEVELS) return launch_levels<T>(h, s, p, uplo, vals, b, x, nrhs, stream, skip);
-    if (nrhs == 1 && p.dataflow && p.block >= 192 && p.block <= 512 && p.df_lds[sizeof(T) == 8] <= lds_limit(h)) {
+    const bool lane = nrhs == 1 && p.dataflow == 2 && p.block >= 128 && p.lane_lds[sizeof(T) == 8] <= lds_limit(h);
+    if (lane || (nrhs == 1 && p.dataflow == 1 && p.block >= 192 && p.block <= 512 &&
+                 p.df_lds[sizeof(T) == 8] <= lds_limit(h))) {
         // publication halves, armed with this value type's sentinel
         if (p.xp_type != (int)sizeof(T)) {
             const size_t cnt = 2 * (size_t)s->n;
@@ -748,8 +940,13 @@ int sweep(baas_handle *h, baas_schedule *s, int uplo, const T *vals, const T *b,
         d.skip = skip;
         d.status = h->status;
         d.spin_limit = h->spin_limit;
-        size_t lds = p.df_lds[sizeof(T) == 8];
         T *xp = reinterpret_cast<T *>(p.xp_raw.p);
+        if (lane) {
+            const size_t lds = p.lane_lds[sizeof(T) == 8];
+            return uplo ? launch_lane<T, true>(h, p, d, vals, b, x, xp, lds, stream)
+                        : launch_lane<T, false>(h, p, d, vals, b, x, xp, lds, stream);
+        }
+        const size_t lds = p.df_lds[sizeof(T) == 8];
         return uplo ? launch_df<T, true>(h, p, d, vals, b, x, xp, lds, stream)
                     : launch_df<T, false>(h, p, d, vals, b, x, xp, lds, stream);
     }
@@ -852,7 +1049,8 @@ extern "C" int baas_csr_outer_grad_f64(baas_handle *h, int m, const int *rp, con
 
 extern "C" int baas_plan_dataflow(baas_schedule *s, int enable) {
     if (!s) return BAAS_INVALID_ARG;
-    s->fwd.dataflow = s->bwd.dataflow = enable ? 1 : 0;
+    if (enable < 0 || enable > 2) return BAAS_INVALID_ARG;
+    s->fwd.dataflow = s->bwd.dataflow = enable;
     s->plan_gen++;
     return BAAS_OK;
 }

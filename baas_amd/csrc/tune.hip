@@ -58,10 +58,14 @@ int autotune(baas_handle *h, baas_schedule *s, const T *vals, int uplo, int nrhs
     if (sizeof(T) == 4) rpus.insert(rpus.end(), {24576, 32768});
     for (int rpu : rpus)
         for (int blk : {256, 384, 512, 1024}) {
-            if (blk == 1024 && rpu < 4096) continue;
-            if (rpu <= 16384) cands.push_back({BAAS_PLAN_RANGE, rpu, blk, 0, 0});
+            const bool big = blk == 1024 && rpu < 4096;
+            if (rpu <= 16384 && !big) cands.push_back({BAAS_PLAN_RANGE, rpu, blk, 0, 0});
             if (nrhs == 1 && blk <= 512) cands.push_back({BAAS_PLAN_RANGE, rpu, blk, 1, 0});
         }
+    // lane kernel: units small enough for their rows to live in LDS
+    if (nrhs == 1)
+        for (int rpu : {384, 512, 768, 1024, 1536})
+            for (int blk : {512, 768, 1024}) cands.push_back({BAAS_PLAN_RANGE, rpu, blk, 2, 0});
     for (int rpu : {256, 1024, 4096})
         for (int blk : {256, 1024}) cands.push_back({BAAS_PLAN_SCHEDULE, rpu, blk, 0, 0});
     for (int blk : {64, 128, 256, 512}) cands.push_back({BAAS_PLAN_LEVELS, 0, blk, 0, 0});
@@ -71,6 +75,7 @@ int autotune(baas_handle *h, baas_schedule *s, const T *vals, int uplo, int nrhs
             for (int blk : {256, 512, 1024}) {
                 if (blk != 512) cands.push_back({BAAS_PLAN_TILES, S, blk, 0, hb});
                 if (nrhs == 1 && blk <= 512) cands.push_back({BAAS_PLAN_TILES, S, blk, 1, hb});
+                if (nrhs == 1 && blk >= 512) cands.push_back({BAAS_PLAN_TILES, S, blk, 2, hb});
             }
     }
     hipEvent_t e0, e1;
@@ -92,7 +97,8 @@ int autotune(baas_handle *h, baas_schedule *s, const T *vals, int uplo, int nrhs
         }
         s->fwd.block = s->bwd.block = c.block;
         Plan &p = uplo ? s->bwd : s->fwd;
-        if (c.df && p.df_lds[sizeof(T) == 8] > lds_limit(h)) continue;
+        if (c.df == 1 && p.df_lds[sizeof(T) == 8] > lds_limit(h)) continue;
+        if (c.df == 2 && p.lane_lds[sizeof(T) == 8] > lds_limit(h)) continue;
         s->fwd.dataflow = s->bwd.dataflow = c.df;
         float t_best = 1e30f;
         bool ok = true;

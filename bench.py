@@ -80,9 +80,9 @@ def cpu_baseline(L, b, seconds):
 def apply_plan(s, plan):
     """Re-pack schedule s into the plan a previous autotune (or --plan) chose."""
     if plan["mode"] == 4:
-        s.replan_tiles(plan["rows_per_unit"], plan["band"], plan["block"], dataflow=bool(plan["dataflow"]))
+        s.replan_tiles(plan["rows_per_unit"], plan["band"], plan["block"], dataflow=int(plan["dataflow"]))
     else:
-        s.replan(plan["mode"], plan["rows_per_unit"], plan["block"], dataflow=bool(plan["dataflow"]))
+        s.replan(plan["mode"], plan["rows_per_unit"], plan["block"], dataflow=int(plan["dataflow"]))
 
 
 def rocsparse_comparators(L, b, x_ours, A, lu_ours, Cd, luc_ours, steps=50):
@@ -284,7 +284,8 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(kern_gbs, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(kern_gbs / HBM_PEAK_GBS, 6), "traffic": traffic,
                      "traffic_plan": traffic_plan,
-                     "kernel": "k_sweep_df (dataflow)" if plan["dataflow"] else "k_sweep",
+                     "kernel": {0: "k_sweep", 1: "k_sweep_df (dataflow)", 2: "k_sweep_lane (dataflow, lanes)"}[
+                         int(plan["dataflow"])] if plan["mode"] != 3 else "k_sweep_levels",
                      "algorithmic_bytes_per_launch": by, "mean_launch_ms": round(mean_ms, 5),
                      "median_launch_ms": round(med_ms, 5)},
     }

@@ -66,7 +66,8 @@ PLANS = [(1, 0, 256, False), (1, 1024, 256, False), (1, 16384, 512, False), (2, 
          (2, 64, 64, False), (2, 1024, 1024, False), (1, 4096, 256, True), (1, 12288, 512, True),
          (1, 2048, 192, True), (2, 256, 256, True), (2, 2048, 512, True), (1, 8192, 1024, True),
          (3, 0, 256, False), (3, 0, 1024, False), (4, 16384, 256, True), (4, 32768, 512, True),
-         (4, 8192, 1024, False)]
+         (4, 8192, 1024, False), (1, 1024, 1024, 2), (1, 512, 512, 2), (1, 4096, 128, 2), (2, 256, 512, 2),
+         (4, 16384, 1024, 2)]
 SOLVE_CASES = ["C1_lap2d_128", "C2_lap3d_64", "C2_lap3d_64_perm1", "C5_diff3d_48", "bcsstk18", "bcsstk38"]
 
 
@@ -137,8 +138,8 @@ def test_f32_bit_exact(handle):
         L, _, _ = workload(name)
         s = Schedule(handle, L.rowptr, L.colidx, kind=0)
         b = g.rhs_uniform(L.n, 1)
-        for mode, df in ((1, False), (1, True), (3, False), (4, True)):
-            s.replan(mode, {1: 4096, 3: 0, 4: 16384}[mode], 256, dataflow=df)
+        for mode, rpu, df in ((1, 4096, False), (1, 4096, True), (3, 0, False), (4, 16384, True), (1, 1024, 2)):
+            s.replan(mode, rpu, 256, dataflow=df)
             x = down(s.solve(up(L.values, torch.float32), up(b, torch.float32)))
             handle.check()
             np.testing.assert_array_equal(x, O.sptrsv_lower_f32(L, b))
@@ -162,7 +163,8 @@ def test_tile_plans_bit_exact(handle, name):
     lv, bd = up(L.values), up(b)
     uv = s.transpose_values(lv)
     for rr, band, blk, df in [(8192, 8, 384, True), (32768, 32, 256, True), (65536, 1, 512, True),
-                              (1 << 30, 16, 256, True), (4096, 64, 512, False), (16384, 16, 1024, False)]:
+                              (1 << 30, 16, 256, True), (4096, 64, 512, False), (16384, 16, 1024, False),
+                              (8192, 16, 1024, 2)]:
         announce("tiles", name, rr, band, blk, df)
         s.replan_tiles(rr, band, blk, dataflow=df)
         pl = s.plan()
@@ -203,7 +205,7 @@ def test_watchdog_reports_timeout_without_hanging(handle):
     b = up(g.rhs_uniform(L.n, 0))
     want = O.sptrsv_lower(L, down(b))
     lv = up(L.values)
-    for mode, rpu, blk, df in ((2, 0, 256, False), (1, 4096, 256, True), (3, 0, 256, False)):
+    for mode, rpu, blk, df in ((2, 0, 256, False), (1, 4096, 256, True), (3, 0, 256, False), (1, 1024, 512, 2)):
         announce("watchdog", mode, rpu, blk, df)
         s.replan(mode, rpu, blk, dataflow=df)
         handle.set_spin_limit(1)
