@@ -235,7 +235,14 @@ __global__ __launch_bounds__(1024) void k_factor_levels(FactorArgs a, int nlev, 
     const int gt = (int)(blockIdx.x * blockDim.x + threadIdx.x);
     for (int l = 0; l < nlev; l++) {
         for (int q = lptr[l] + gt; q < lptr[l + 1]; q += nth) factor_any<KIND>(a, meta[q].x, lu, own, blockDim.x);
-        if (l + 1 < nlev) grid_barrier(bar, (l + 1) * (int)gridDim.x, a.status, a.spin_limit);
+        if (l + 1 < nlev) {
+            if (gridDim.x == 1) {
+                wait_stores();  // a single workgroup needs only its own barrier
+                __syncthreads();
+            } else {
+                grid_barrier(bar, (l + 1) * (int)gridDim.x, a.status, a.spin_limit);
+            }
+        }
     }
     if (threadIdx.x == 0) {
         __threadfence();
@@ -323,6 +330,7 @@ static int factor(baas_handle *h, baas_schedule *s, const double *a_vals, double
             widest = std::max(widest, s->row_level_ptr[l + 1] - s->row_level_ptr[l]);
         long long grid =
             std::min<long long>((long long)std::min(per_cu, 2) * h->num_cus, (widest + p.block - 1) / p.block);
+        if (p.lv_cap > 0 && grid > p.lv_cap) grid = p.lv_cap;
         if (grid < 1) grid = 1;
         hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(p.block), lds, h->stream, a, p.nlev, p.lptr.p,
                            reinterpret_cast<const int4 *>(p.meta.p), h->counters + BAAS_BARRIER_BASE, lu);

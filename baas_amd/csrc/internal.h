@@ -95,6 +95,7 @@ struct Plan {
     // LEVELS plans: row level sets in execution order (meta holds the rows)
     DBuf<int> lptr;       // [nlev+1]
     int nlev = 0;
+    int lv_cap = 0;       // at most this many workgroups (0: as the widest level needs)
     int dataflow = 0;     // single-RHS sweeps: 1 dataflow kernel, 2 lane kernel
     // lane kernel: per position, the offset of its entries in its unit's LDS
     // entry area; per unit, the running entry count; LDS bytes (FP32, FP64)

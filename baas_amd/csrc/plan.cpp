@@ -232,7 +232,7 @@ int make_plan(Plan &p, int n, const std::vector<int> &order, const std::vector<i
 // LEVELS plan: rows in (level, id) order with a 16-byte record each (row,
 // first off-diagonal entry, off-diagonal count), and the level pointer.  The
 // backward plan walks the same levels in reverse over U.
-static int make_levels_plan(Plan &p, const baas_schedule *s, const HostCsr &M, bool upper, int block,
+static int make_levels_plan(Plan &p, const baas_schedule *s, const HostCsr &M, bool upper, int block, int cap,
                             hipStream_t st) {
     const int n = s->n;
     const auto &perm = s->row_perm, &lp = s->row_level_ptr;
@@ -255,6 +255,7 @@ static int make_levels_plan(Plan &p, const baas_schedule *s, const HostCsr &M, b
     p.max_deg = max_deg;
     p.mode = BAAS_PLAN_LEVELS;
     p.block = block;
+    p.lv_cap = cap;
     p.keymode = 0;
     p.max_slots = 0;
     p.df_lds[0] = p.df_lds[1] = 0;
@@ -325,8 +326,8 @@ int build_plans_mode(baas_schedule *s, int mode, int rows_per_unit) {
     BAAS_TRY(fetch(U, s->u_rowptr.p, s->u_colidx.p, n, st));
     if (mode == BAAS_PLAN_LEVELS) {
         const int block = s->opts.block_threads > 0 ? s->opts.block_threads : 256;
-        BAAS_TRY(make_levels_plan(s->fwd, s, L, false, block, st));
-        BAAS_TRY(make_levels_plan(s->bwd, s, U, true, block, st));
+        BAAS_TRY(make_levels_plan(s->fwd, s, L, false, block, rows_per_unit, st));
+        BAAS_TRY(make_levels_plan(s->bwd, s, U, true, block, rows_per_unit, st));
         return BAAS_OK;
     }
     std::vector<int> lp(n + 1), li, up(n + 1), ui;

@@ -852,7 +852,12 @@ __global__ __launch_bounds__(1024) void k_sweep_levels(LvArgs a, const T *__rest
             const int nb = a.lptr[ln];
             have = gt < (long long)(a.lptr[ln + 1] - nb) * nrhs;
             if (have) lv_load<T, UPPER, CH>(a, vals, b, nb + (int)(gt / nrhs), (int)(gt % nrhs), first);
-            grid_barrier(a.bar, (li + 1) * (int)gridDim.x, a.status, a.spin_limit);
+            if (gridDim.x == 1) {
+                wait_stores();  // a single workgroup needs only its own barrier
+                __syncthreads();
+            } else {
+                grid_barrier(a.bar, (li + 1) * (int)gridDim.x, a.status, a.spin_limit);
+            }
         }
     }
     if (threadIdx.x == 0) {
@@ -897,6 +902,7 @@ static int launch_levels(baas_handle *h, baas_schedule *s, const Plan &p, int up
     // barrier, so fewer, fuller workgroups make the barriers cheaper
     long long need = ((long long)widest * nrhs + p.block - 1) / p.block;
     long long grid = std::min<long long>((long long)std::min(per_cu, 2) * h->num_cus, need);
+    if (p.lv_cap > 0 && grid > p.lv_cap) grid = p.lv_cap;
     if (grid < 1) grid = 1;
     hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(p.block), 0, stream, a, vals, b, x);
     BAAS_HIP_TRY(hipGetLastError());

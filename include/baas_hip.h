@@ -74,7 +74,10 @@ enum baas_export {
  * on matrices numbered along their geometry (natural stencil orderings,
  * banded and RCM-ordered matrices) a unit is a compact block of the domain
  * and a dependency chain crosses few units.  rows_per_unit is the row range;
- * the band height is set with baas_replan_tiles (default 16 levels). */
+ * the band height is set with baas_replan_tiles (default 16 levels).
+ * For LEVELS plans rows_per_unit caps the workgroups (0: as many as the
+ * widest level needs); with one workgroup the levels are separated by the
+ * workgroup's own barrier instead of a device-wide one. */
 enum baas_plan_mode {
     BAAS_PLAN_AUTO = 0,
     BAAS_PLAN_RANGE = 1,

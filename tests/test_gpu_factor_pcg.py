@@ -38,7 +38,8 @@ def test_ic0_bit_exact(handle, name):
     want = O.ic0(A)
     s = Schedule(handle, A.rowptr, A.colidx, kind=_lib.KIND_IC0)
     av = up(A.values)
-    for mode, rpu, blk in [(1, 0, 256), (2, 0, 256), (2, 64, 64), (1, 1024, 512), (3, 0, 256), (4, 16384, 256)]:
+    for mode, rpu, blk in [(1, 0, 256), (2, 0, 256), (2, 64, 64), (1, 1024, 512), (3, 0, 256), (4, 16384, 256),
+                           (3, 1, 1024)]:
         announce("ic0", name, mode, rpu, blk)
         s.replan(mode, rpu, blk)
         lu = s.ic0(av)
@@ -70,9 +71,9 @@ def test_ilu0_bit_exact(handle, name):
     _, _, A = workload(name)
     want = O.ilu0(A)
     s = Schedule(handle, A.rowptr, A.colidx, kind=_lib.KIND_ILU0)
-    for mode in (1, 2, 3, 4):
-        announce("ilu0", name, mode)
-        s.replan(mode, 0, 256)
+    for mode, rpu in ((1, 0), (2, 0), (3, 0), (4, 0), (3, 1)):
+        announce("ilu0", name, mode, rpu)
+        s.replan(mode, rpu, 256)
         lu = s.ilu0(up(A.values))
         _status_ok(handle)
         np.testing.assert_array_equal(down(lu), want)
@@ -194,7 +195,7 @@ def test_pcg_same_iterates_on_every_plan_kind(handle, prec):
     av = up(A.values)
     ref = None
     for mode, rpu, blk, df in ((1, 4096, 256, False), (3, 0, 256, False), (1, 2048, 384, True), (2, 0, 256, False),
-                               (4, 8192, 256, True), (1, 1024, 1024, 2)):
+                               (4, 8192, 256, True), (1, 1024, 1024, 2), (3, 1, 1024, False)):
         announce("pcg plans", prec, mode, rpu, blk, df)
         s.replan(mode, rpu, blk, dataflow=df)
         solver = PCG(handle, s, av, precision=prec)
