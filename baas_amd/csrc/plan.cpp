@@ -17,6 +17,7 @@
 //
 // Built on the host once per analysis (like the reference's task build).
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <vector>
 
@@ -317,6 +318,60 @@ static int tile_units(const baas_schedule *s, int S, int hb, std::vector<int> &o
     return BAAS_OK;
 }
 
+// BOXES units: rows of a structured grid numbered in natural (lexicographic)
+// order, recognised from the lower triangle itself.  When every
+// off-diagonal entry's offset (row - column) is one of at most three values
+// 1 = s1 < s2 < s3 with s2 dividing s3, row r sits at (x, y, z) =
+// (r mod s2, (r div s2) mod (s3 / s2), r div s3) -- the 5- and 7-point
+// stencil orderings -- and units are boxes of about R rows, as cubic as the
+// grid allows, taken in (z, y, x) box order with rows ascending inside a
+// box.  A parent then lies in the same box or an earlier one (the plan
+// builder checks this and rejects the order otherwise), and a dependency
+// chain crosses a unit once per box edge in every direction instead of at
+// every plane.  Matrices without this structure get BAAS_UNSUPPORTED.
+static int box_units(const baas_schedule *s, const HostCsr &L, int R, std::vector<int> &order, std::vector<int> &fu) {
+    const int n = s->n;
+    if (R < 1) return BAAS_INVALID_ARG;
+    std::vector<long long> offs;
+    for (int r = 0; r < n; r++)
+        for (int j = L.ptr[r]; j < L.ptr[r + 1] - 1; j++) {
+            const long long d = (long long)r - L.idx[j];
+            if (std::find(offs.begin(), offs.end(), d) == offs.end()) {
+                offs.push_back(d);
+                if (offs.size() > 3) return BAAS_UNSUPPORTED;
+            }
+        }
+    std::sort(offs.begin(), offs.end());
+    if (offs.empty() || offs[0] != 1) return BAAS_UNSUPPORTED;
+    const long long sy = offs.size() >= 2 ? offs[1] : 0, sz = offs.size() == 3 ? offs[2] : 0;
+    if (sz && sz % sy != 0) return BAAS_UNSUPPORTED;
+    const long long nx = sy ? sy : n;
+    const long long ny = sz ? sz / sy : (sy ? (n + sy - 1) / sy : 1);
+    const long long nz = sz ? (n + sz - 1) / sz : 1;
+    const int dims = (nx > 1) + (ny > 1) + (nz > 1);
+    const double edge = std::pow((double)R, 1.0 / std::max(dims, 1));
+    auto clampl = [](long long v, long long hi) { return std::max(1LL, std::min(v, hi)); };
+    const long long bx = clampl(std::llround(edge), nx);
+    const long long by = ny > 1 ? clampl(std::llround(edge), ny) : 1;
+    const long long bz = nz > 1 ? clampl(R / (bx * by), nz) : 1;
+    const long long gx = (nx + bx - 1) / bx, gy = (ny + by - 1) / by, gz = (nz + bz - 1) / bz;
+    const long long nbox = gx * gy * gz;
+    if (nbox > (1LL << 26)) return BAAS_INVALID_ARG;
+    auto box_of = [&](int r) {
+        const long long x = r % nx, y = (r / nx) % ny, z = r / (nx * ny);
+        return ((z / bz) * gy + y / by) * gx + x / bx;
+    };
+    std::vector<int> start((size_t)nbox + 1, 0);
+    for (int r = 0; r < n; r++) start[box_of(r) + 1]++;
+    for (long long t = 0; t < nbox; t++) start[t + 1] += start[t];
+    std::vector<int> next(start.begin(), start.end() - 1);
+    for (int r = 0; r < n; r++) order[next[box_of(r)]++] = r;
+    for (long long t = 0; t < nbox; t++)
+        if (start[t + 1] > start[t]) fu.push_back(start[t]);
+    fu.push_back(n);
+    return BAAS_OK;
+}
+
 int build_plans_mode(baas_schedule *s, int mode, int rows_per_unit) {
     s->plan_gen++;
     hipStream_t st = s->h->stream;
@@ -355,6 +410,8 @@ int build_plans_mode(baas_schedule *s, int mode, int rows_per_unit) {
     }
     if (mode == BAAS_PLAN_TILES) {
         BAAS_TRY(tile_units(s, rows_per_unit > 0 ? rows_per_unit : 16384, s->tile_band, fwd_order, fu));
+    } else if (mode == BAAS_PLAN_BOXES) {
+        BAAS_TRY(box_units(s, L, rows_per_unit > 0 ? rows_per_unit : 512, fwd_order, fu));
     } else {
         BAAS_TRY(schedule_units(s, rows_per_unit > 0 ? rows_per_unit : block, fwd_order, fu));
     }
@@ -466,13 +523,16 @@ int build_ic0_plans(baas_schedule *s) {
 }  // namespace baas
 
 extern "C" int baas_replan(baas_schedule *s, int mode, int rows_per_unit, int block_threads) {
-    if (!s || mode < 1 || mode > 4 || rows_per_unit < 0 || block_threads < 0 || block_threads > 1024 ||
+    if (!s || mode < 1 || mode > 5 || rows_per_unit < 0 || block_threads < 0 || block_threads > 1024 ||
         (block_threads % 64) != 0)
         return BAAS_INVALID_ARG;
+    const baas_opts old = s->opts;
     if (block_threads) s->opts.block_threads = block_threads;
     s->opts.plan_mode = mode;
     s->opts.rows_per_unit = rows_per_unit;
-    return baas::build_plans_mode(s, mode, rows_per_unit);
+    const int rc = baas::build_plans_mode(s, mode, rows_per_unit);
+    if (rc != BAAS_OK) s->opts = old;  // the plans are rebuilt only on success
+    return rc;
 }
 
 extern "C" int baas_replan_tiles(baas_schedule *s, int row_range, int band_levels, int block_threads) {

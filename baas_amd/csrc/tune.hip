@@ -69,6 +69,14 @@ int autotune(baas_handle *h, baas_schedule *s, const T *vals, int uplo, int nrhs
     for (int rpu : {256, 1024, 4096})
         for (int blk : {256, 1024}) cands.push_back({BAAS_PLAN_SCHEDULE, rpu, blk, 0, 0});
     for (int blk : {64, 128, 256, 512}) cands.push_back({BAAS_PLAN_LEVELS, 0, blk, 0, 0});
+    // grid boxes (matrices without stencil-grid structure fail to build and
+    // are skipped)
+    for (int R : {512, 1000, 1331})
+        for (int blk : {256, 384, 512, 1024}) {
+            if (blk == 256) cands.push_back({BAAS_PLAN_BOXES, R, blk, 0, 0});
+            if (nrhs == 1 && blk == 384) cands.push_back({BAAS_PLAN_BOXES, R, blk, 1, 0});
+            if (nrhs == 1 && blk >= 512) cands.push_back({BAAS_PLAN_BOXES, R, blk, 2, 0});
+        }
     for (int S : TILE_RANGES) {
         if (S >= 2 * n && S != TILE_RANGES[0]) continue;
         for (int hb : TILE_BANDS)
@@ -151,6 +159,8 @@ static int autotune_factor(baas_handle *h, baas_schedule *s, const double *a_val
     for (int blk : {64, 128, 256, 512}) cands.push_back({BAAS_PLAN_LEVELS, 0, blk, 0, 0});
     for (int hb : {16, 64})
         for (int blk : {256, 512}) cands.push_back({BAAS_PLAN_TILES, 16384, blk, 0, hb});
+    for (int R : {512, 1024})
+        for (int blk : {256, 512}) cands.push_back({BAAS_PLAN_BOXES, R, blk, 0, 0});
     hipEvent_t e0, e1;
     BAAS_HIP_TRY(hipEventCreate(&e0));
     BAAS_HIP_TRY(hipEventCreate(&e1));

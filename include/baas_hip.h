@@ -75,6 +75,11 @@ enum baas_export {
  * banded and RCM-ordered matrices) a unit is a compact block of the domain
  * and a dependency chain crosses few units.  rows_per_unit is the row range;
  * the band height is set with baas_replan_tiles (default 16 levels).
+ * BOXES: for a matrix whose lower triangle has at most three distinct
+ * offsets (row - column) 1 < sy < sz with sy dividing sz -- a 5- or
+ * 7-point stencil grid in natural order, recognised from the pattern alone
+ * -- units are boxes of the grid of about rows_per_unit rows (default 512);
+ * other matrices get BAAS_UNSUPPORTED.
  * For LEVELS plans rows_per_unit caps the workgroups (0: as many as the
  * widest level needs); with one workgroup the levels are separated by the
  * workgroup's own barrier instead of a device-wide one. */
@@ -83,7 +88,8 @@ enum baas_plan_mode {
     BAAS_PLAN_RANGE = 1,
     BAAS_PLAN_SCHEDULE = 2,
     BAAS_PLAN_LEVELS = 3,
-    BAAS_PLAN_TILES = 4
+    BAAS_PLAN_TILES = 4,
+    BAAS_PLAN_BOXES = 5
 };
 
 typedef struct baas_opts {

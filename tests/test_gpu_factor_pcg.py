@@ -38,8 +38,9 @@ def test_ic0_bit_exact(handle, name):
     want = O.ic0(A)
     s = Schedule(handle, A.rowptr, A.colidx, kind=_lib.KIND_IC0)
     av = up(A.values)
+    grid = name in ("C2_lap3d_64", "C5_diff3d_48")
     for mode, rpu, blk in [(1, 0, 256), (2, 0, 256), (2, 64, 64), (1, 1024, 512), (3, 0, 256), (4, 16384, 256),
-                           (3, 1, 1024)]:
+                           (3, 1, 1024)] + ([(5, 512, 256)] if grid else []):
         announce("ic0", name, mode, rpu, blk)
         s.replan(mode, rpu, blk)
         lu = s.ic0(av)
@@ -71,7 +72,7 @@ def test_ilu0_bit_exact(handle, name):
     _, _, A = workload(name)
     want = O.ilu0(A)
     s = Schedule(handle, A.rowptr, A.colidx, kind=_lib.KIND_ILU0)
-    for mode, rpu in ((1, 0), (2, 0), (3, 0), (4, 0), (3, 1)):
+    for mode, rpu in ((1, 0), (2, 0), (3, 0), (4, 0), (3, 1)) + (((5, 512),) if name != "bcsstk18" else ()):
         announce("ilu0", name, mode, rpu)
         s.replan(mode, rpu, 256)
         lu = s.ilu0(up(A.values))

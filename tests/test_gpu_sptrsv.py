@@ -176,6 +176,47 @@ def test_tile_plans_bit_exact(handle, name):
         np.testing.assert_array_equal(down(y), y_ref, err_msg=f"bwd {name} {rr} {band} {blk} {df}")
 
 
+@pytest.mark.parametrize("name", ["C1_lap2d_128", "C2_lap3d_64", "C5_diff3d_48"])
+def test_box_plans_bit_exact(handle, name):
+    """BOXES plans on stencil grids in natural order (recognised from the
+    pattern): every kernel, forward and backward, bit for bit."""
+    from baas_amd.api import Schedule
+    L, _, _ = workload(name)
+    s = Schedule(handle, L.rowptr, L.colidx, kind=0)
+    b = g.rhs_uniform(L.n, 6)
+    x_ref = O.sptrsv_lower(L, b)
+    U = g.Csr.from_scipy(L.to_scipy().T)
+    y_ref = O.sptrsv_upper(U, b)
+    lv, bd = up(L.values), up(b)
+    uv = s.transpose_values(lv)
+    for rpu, blk, df in [(512, 1024, 2), (512, 512, 2), (1024, 384, 1), (512, 256, 0), (64, 256, 2), (100, 128, 2)]:
+        announce("boxes", name, rpu, blk, df)
+        s.replan(5, rpu, blk, dataflow=df)
+        x = s.solve(lv, bd)
+        y = s.solve(uv, bd, uplo=1)
+        handle.check()
+        np.testing.assert_array_equal(down(x), x_ref, err_msg=f"fwd {name} {rpu} {blk} {df}")
+        np.testing.assert_array_equal(down(y), y_ref, err_msg=f"bwd {name} {rpu} {blk} {df}")
+
+
+@pytest.mark.parametrize("name", ["C2_lap3d_64_perm1", "bcsstk18", "C3_knn"])
+def test_box_plans_reject_unstructured(handle, name):
+    """Matrices that are not stencil grids in natural order get UNSUPPORTED,
+    and the schedule keeps its previous plan."""
+    from baas_amd import _lib
+    from baas_amd.api import Schedule
+    announce("boxes reject", name)
+    L, _, _ = workload(name)
+    s = Schedule(handle, L.rowptr, L.colidx, kind=0)
+    s.replan(1, 1024, 256)
+    with pytest.raises(_lib.BaasError) as e:
+        s.replan(5, 512, 512)
+    assert e.value.code == _lib.UNSUPPORTED
+    assert s.plan()["mode"] == 1
+    b = g.rhs_uniform(L.n, 0)
+    np.testing.assert_array_equal(down(s.solve(up(L.values), up(b))), O.sptrsv_lower(L, b))
+
+
 def test_division_and_sqrt_are_ieee(handle):
     """The kernels' f64 division matches the CPU's correctly rounded one."""
     import torch
@@ -245,7 +286,7 @@ def test_autotune_keeps_bits(handle, name):
         vals = lv if uplo == 0 else s.transpose_values(lv)
         chosen = s.autotune(vals, uplo=uplo)
         print("[autotune]", name, uplo, chosen)
-        assert chosen["mode"] in (1, 2, 3, 4) and chosen["us"] > 0
+        assert chosen["mode"] in (1, 2, 3, 4, 5) and chosen["us"] > 0
         b = g.rhs_uniform(L.n, 0)
         got = down(s.solve(vals, up(b), uplo=uplo))
         handle.check()
