@@ -196,7 +196,7 @@ def test_pcg_same_iterates_on_every_plan_kind(handle, prec):
     av = up(A.values)
     ref = None
     for mode, rpu, blk, df in ((1, 4096, 256, False), (3, 0, 256, False), (1, 2048, 384, True), (2, 0, 256, False),
-                               (4, 8192, 256, True), (1, 1024, 1024, 2), (3, 1, 1024, False)):
+                               (4, 8192, 256, True), (1, 1024, 1024, 2), (3, 1, 1024, False), (2, 64, 1024, 2)):
         announce("pcg plans", prec, mode, rpu, blk, df)
         s.replan(mode, rpu, blk, dataflow=df)
         solver = PCG(handle, s, av, precision=prec)

@@ -67,7 +67,7 @@ PLANS = [(1, 0, 256, False), (1, 1024, 256, False), (1, 16384, 512, False), (2, 
          (1, 2048, 192, True), (2, 256, 256, True), (2, 2048, 512, True), (1, 8192, 1024, True),
          (3, 0, 256, False), (3, 0, 1024, False), (4, 16384, 256, True), (4, 32768, 512, True),
          (4, 8192, 1024, False), (1, 1024, 1024, 2), (1, 512, 512, 2), (1, 4096, 128, 2), (2, 256, 512, 2),
-         (4, 16384, 1024, 2), (3, 1, 1024, False), (3, 4, 512, False)]
+         (4, 16384, 1024, 2), (3, 1, 1024, False), (3, 4, 512, False), (2, 64, 1024, 2), (2, 32, 512, 2)]
 SOLVE_CASES = ["C1_lap2d_128", "C2_lap3d_64", "C2_lap3d_64_perm1", "C5_diff3d_48", "bcsstk18", "bcsstk38"]
 
 
