@@ -33,13 +33,30 @@ __global__ void k_reduce(int np, const double *__restrict__ part, double *out, c
     if (threadIdx.x == 0) *out = s;
 }
 
+// Sum of np partials in the order k_reduce uses (256 threads, strided, then
+// the block sum), computed by every workgroup of a consumer kernel so the
+// value needs no launch of its own; every workgroup gets the same bits.
+__device__ __forceinline__ double sum_partials(int np, const double *part, double *red, double *bcast) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < np; i += blockDim.x) s += ld_agent(&part[i]);
+    s = block_sum(s, red);
+    if (threadIdx.x == 0) *bcast = s;
+    __syncthreads();
+    return *bcast;
+}
+
 // x += alpha p; r -= alpha Ap; partials of r.r; r32 = (float) r
+// (alpha = rz / pAp, pAp summed here from the SpMV's partials)
 __global__ __launch_bounds__(256) void k_update_xr(int n, double *x, double *r, const double *__restrict__ p,
-                                                   const double *__restrict__ Ap, const double *sc, int cur,
-                                                   double *part_rr, float *r32, const int *st) {
+                                                   const double *__restrict__ Ap, double *sc, int cur,
+                                                   int np_pap, const double *part_pap, double *part_rr,
+                                                   float *r32, const int *st) {
     __shared__ double red[4];
+    __shared__ double bc;
     if (ld_agent(&st[ST_DONE])) return;
-    const double alpha = sc[SC_RZ0 + cur] / sc[SC_PAP];
+    const double pap = sum_partials(np_pap, part_pap, red, &bc);
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc[SC_PAP] = pap;
+    const double alpha = sc[SC_RZ0 + cur] / pap;
     double acc = 0.0;
     for (int i = blockIdx.x * 256 + threadIdx.x; i < n; i += gridDim.x * 256) {
         x[i] = fma(alpha, p[i], x[i]);
@@ -93,11 +110,16 @@ __global__ __launch_bounds__(256) void k_rz(int n, const double *__restrict__ r,
     if (threadIdx.x == 0) part[blockIdx.x] = acc;
 }
 
-// p = z + beta p, beta = rz_new / rz_old
-__global__ void k_update_p(int n, double *p, const double *__restrict__ z, const double *sc, int cur,
-                           const int *st) {
+// p = z + beta p, beta = rz_new / rz_old (rz_new summed here from the
+// partials of k_rz; workgroup 0 keeps it for the next iteration)
+__global__ __launch_bounds__(256) void k_update_p(int n, double *p, const double *__restrict__ z, double *sc,
+                                                  int cur, int np_rz, const double *part_rz, const int *st) {
+    __shared__ double red[4];
+    __shared__ double bc;
     if (st && ld_agent(&st[ST_DONE])) return;
-    const double beta = sc[SC_RZ0 + (cur ^ 1)] / sc[SC_RZ0 + cur];
+    const double rz_new = sum_partials(np_rz, part_rz, red, &bc);
+    if (blockIdx.x == 0 && threadIdx.x == 0) sc[SC_RZ0 + (cur ^ 1)] = rz_new;
+    const double beta = rz_new / sc[SC_RZ0 + cur];
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
         p[i] = fma(beta, p[i], z[i]);
 }
@@ -168,17 +190,15 @@ static int iteration(baas_pcg *P, int cur) {
     const int *done = P->state.p + ST_DONE;
     BAAS_TRY(spmv_dev(n, s->a_rowptr.p, s->a_colidx.p, P->a_vals, P->p.p, P->Ap.p, 1.0, 0.0, P->p.p, P->part_pap.p,
                       done, P->st));
-    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, P->st, P->nb_spmv, P->part_pap.p, P->sc.p + SC_PAP, done);
     hipLaunchKernelGGL(k_update_xr, dim3(P->nb_vec), dim3(256), 0, P->st, n, P->x.p, P->r.p, P->p.p, P->Ap.p,
-                       P->sc.p, cur, P->part_rr.p, P->precision == 1 ? P->r32.p : nullptr, P->state.p);
+                       P->sc.p, cur, P->nb_spmv, P->part_pap.p, P->part_rr.p,
+                       P->precision == 1 ? P->r32.p : nullptr, P->state.p);
     hipLaunchKernelGGL(k_conv, dim3(1), dim3(256), 0, P->st, P->nb_vec, P->part_rr.p, P->sc.p, P->state.p);
     BAAS_TRY(precond(P, done));
     hipLaunchKernelGGL(k_rz, dim3(P->nb_vec), dim3(256), 0, P->st, n, P->r.p, P->z.p,
                        P->precision == 1 ? P->z32.p : nullptr, P->part_rz.p, P->state.p);
-    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, P->st, P->nb_vec, P->part_rz.p, P->sc.p + SC_RZ0 + (cur ^ 1),
-                       done);
     hipLaunchKernelGGL(k_update_p, dim3(P->nb_vec), dim3(256), 0, P->st, n, P->p.p, P->z.p, P->sc.p, cur,
-                       P->state.p);
+                       P->nb_vec, P->part_rz.p, P->state.p);
     BAAS_HIP_TRY(hipGetLastError());
     return BAAS_OK;
 }
