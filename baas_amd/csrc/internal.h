@@ -101,9 +101,13 @@ struct Plan {
     // entry area; per unit, the running entry count; LDS bytes (FP32, FP64)
     DBuf<int> lane_eoff;  // [n]
     DBuf<int> lane_eptr;  // [nunits+1]
-    size_t lane_lds[2] = {0, 0};
-    DBuf<unsigned char> xp_raw;  // 2 x n published values (dataflow kernel)
+    std::vector<int> h_lane_units;  // per unit: rows, external parents, entries
+    size_t lane_lds(int ts, int nrhs);  // LDS bytes per workgroup (cached per ts, nrhs)
+    int lane_lds_key = -1;
+    size_t lane_lds_val = 0;
+    DBuf<unsigned char> xp_raw;  // 2 x n x nrhs published values (dataflow kernels)
     int xp_type = 0;             // sizeof the value type the halves are armed for
+    int xp_nrhs = 0;             // ... and their right-hand sides per row
     DBuf<int> pcount;            // [1] sweeps run on this plan
     std::vector<int> h_unit_ptr;  // host copy: unit boundaries in rows
     std::vector<int> h_rows;      // host copy of rows (without flag bit)
@@ -116,15 +120,16 @@ int autotune(baas_handle *h, baas_schedule *s, const T *vals, int uplo, int nrhs
 // sections): value slots (keys, then external parents), per position b, the
 // diagonal, {entry offset, count | external-child flag}, LDS slot and row,
 // then the unit's off-diagonal values and their slots.
+// With k right-hand sides the value slots and b hold k values per row.
 struct LaneLayout {
     size_t sl, bv, dv, rec, slot, row, ev, es, total;
 };
 __host__ __device__ inline size_t lane_align8(size_t v) { return (v + 7) & ~size_t(7); }
-__host__ __device__ inline LaneLayout lane_layout(size_t nk, size_t ne, size_t E, size_t ts) {
+__host__ __device__ inline LaneLayout lane_layout(size_t nk, size_t ne, size_t E, size_t ts, size_t nrhs = 1) {
     LaneLayout l;
     l.sl = 0;
-    l.bv = lane_align8((nk + ne) * ts);
-    l.dv = l.bv + lane_align8(nk * ts);
+    l.bv = lane_align8((nk + ne) * nrhs * ts);
+    l.dv = l.bv + lane_align8(nk * nrhs * ts);
     l.rec = l.dv + lane_align8(nk * ts);
     l.slot = l.rec + nk * 8;
     l.row = l.slot + lane_align8(nk * 4);

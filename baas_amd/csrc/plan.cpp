@@ -174,7 +174,7 @@ int make_plan(Plan &p, int n, const std::vector<int> &order, const std::vector<i
         }
     // lane kernel: unit-relative entry offsets and the LDS footprint
     std::vector<int> lane_eoff(n), lane_eptr(nunits + 1, 0);
-    size_t lane4 = 0, lane8 = 0;
+    p.h_lane_units.assign(3 * (size_t)nunits, 0);
     for (int u = 0; u < nunits; u++) {
         int acc = 0;
         for (int q = unit_ptr[u]; q < unit_ptr[u + 1]; q++) {
@@ -182,18 +182,18 @@ int make_plan(Plan &p, int n, const std::vector<int> &order, const std::vector<i
             acc += meta[4 * (size_t)q + 2];
         }
         lane_eptr[u + 1] = lane_eptr[u] + acc;
-        const size_t nk = unit_ptr[u + 1] - unit_ptr[u], ne = ext_ptr[u + 1] - ext_ptr[u];
-        lane4 = std::max(lane4, lane_layout(nk, ne, acc, 4).total);
-        lane8 = std::max(lane8, lane_layout(nk, ne, acc, 8).total);
+        p.h_lane_units[3 * (size_t)u + 0] = unit_ptr[u + 1] - unit_ptr[u];
+        p.h_lane_units[3 * (size_t)u + 1] = ext_ptr[u + 1] - ext_ptr[u];
+        p.h_lane_units[3 * (size_t)u + 2] = acc;
     }
     BAAS_TRY(upload(p.lane_eoff, lane_eoff, st));
     BAAS_TRY(upload(p.lane_eptr, lane_eptr, st));
-    p.lane_lds[0] = lane4;
-    p.lane_lds[1] = lane8;
     BAAS_TRY(p.pcount.alloc(1));
     if (hipMemsetAsync(p.pcount.p, 0, sizeof(int), st) != hipSuccess) return BAAS_HIP_ERROR;
     p.xp_raw.free();
     p.xp_type = 0;
+    p.xp_nrhs = 0;
+    p.lane_lds_key = -1;
     BAAS_TRY(upload(p.scol, scol, st));
     BAAS_TRY(upload(p.ext_ptr, ext_ptr, st));
     BAAS_TRY(upload(p.ext_cols, ext_cols, st));
@@ -227,6 +227,17 @@ int make_plan(Plan &p, int n, const std::vector<int> &order, const std::vector<i
 }
 
 }  // namespace
+
+size_t Plan::lane_lds(int ts, int nrhs) {
+    const int key = nrhs * 16 + ts;
+    if (key == lane_lds_key) return lane_lds_val;
+    size_t m = 0;
+    for (size_t u = 0; u + 2 < h_lane_units.size(); u += 3)
+        m = std::max(m, lane_layout(h_lane_units[u], h_lane_units[u + 1], h_lane_units[u + 2], ts, nrhs).total);
+    lane_lds_key = key;
+    lane_lds_val = m;
+    return m;
+}
 
 // Strip the diagonal: parents of row r in L (lower CSR, diag last) and in
 // U = L^T (upper CSR, diag first).

@@ -144,6 +144,7 @@ __device__ __forceinline__ void st_lds(T *p, T v) {
 
 struct DfArgs {
     int n, nunits, keymode;
+    int nrhs;  // right-hand sides per row (1 for k_sweep_df)
     const int4 *meta;  // per position: row | ext flag, first off-diagonal entry, count, LDS slot
     const int *scol;
     const int *unit_ptr, *key_begin, *ext_ptr, *ext_cols;
@@ -207,17 +208,20 @@ __device__ __forceinline__ void df_load_row(const DfArgs &a, const T *__restrict
 template <class T>
 __device__ __forceinline__ void df_incoming(const DfArgs &a, const T *pub, T *sl, const int *cols, int nk, int ne,
                                             int lane) {
-    for (int e = lane; e < ne; e += DF_PF * 64) {
+    // with k right-hand sides, item i is parent i / k, column i % k; its
+    // published value sits at parent * k + i % k and its slot at nk * k + i
+    const int nr = a.nrhs, items = ne * nr;
+    for (int e = lane; e < items; e += DF_PF * 64) {
         int c[DF_PF];
 #pragma unroll
         for (int k = 0; k < DF_PF; k++) {
-            const int ek = e + k * 64;
-            c[k] = cols[ek < ne ? ek : ne - 1];
+            const int ek = e + k * 64 < items ? e + k * 64 : items - 1;
+            c[k] = nr == 1 ? cols[ek] : cols[ek / nr] * nr + ek % nr;
         }
         unsigned todo = 0;
 #pragma unroll
         for (int k = 0; k < DF_PF; k++)
-            if (e + k * 64 < ne) todo |= 1u << k;
+            if (e + k * 64 < items) todo |= 1u << k;
         long long spins = 0;
         while (todo) {
             T v[DF_PF];
@@ -226,7 +230,7 @@ __device__ __forceinline__ void df_incoming(const DfArgs &a, const T *pub, T *sl
 #pragma unroll
             for (int k = 0; k < DF_PF; k++) {
                 if (((todo >> k) & 1u) && !Sentinel<T>::is(v[k])) {
-                    st_lds(&sl[nk + e + k * 64], v[k]);
+                    st_lds(&sl[nk * nr + e + k * 64], v[k]);
                     todo &= ~(1u << k);
                 }
             }
@@ -236,7 +240,7 @@ __device__ __forceinline__ void df_incoming(const DfArgs &a, const T *pub, T *sl
                     raise_status(a.status, BAAS_TIMEOUT, c[__builtin_ctz(todo)]);
 #pragma unroll
                     for (int k = 0; k < DF_PF; k++)
-                        if ((todo >> k) & 1u) st_lds(&sl[nk + e + k * 64], T(0));
+                        if ((todo >> k) & 1u) st_lds(&sl[nk * nr + e + k * 64], T(0));
                     todo = 0;
                 } else {
                     __builtin_amdgcn_s_sleep(1);
@@ -458,44 +462,58 @@ __global__ __launch_bounds__(512) void k_sweep_df(DfArgs a, const T *__restrict_
 //    solved instead of waiting for the other lanes of its wavefront;
 //  * a lane publishes its own row for later units (the value is the flag,
 //    as in the outgoing exchange), so only wavefront 0 exchanges (incoming).
+// Load a chunk of a row's parents: their value slots (times nr, plus the
+// lane's right-hand side r) and coefficients; slots past the chunk repeat its
+// last entry, an empty chunk points at slot 0 with coefficient 0.
 template <class T, bool UPPER, int CH>
-__device__ __forceinline__ void lane_chunk(const T *lev, const int *les, int eo, int k, int nc, int (&cs)[CH],
-                                           T (&cv)[CH]) {
+__device__ __forceinline__ void lane_chunk(const T *lev, const int *les, int eo, int k, int nc, int nr, int r,
+                                           int (&cs)[CH], T (&cv)[CH]) {
     const int last = nc > 0 ? nc - 1 : 0;
 #pragma unroll
     for (int t = 0; t < CH; t++) {
         const int i = eo + k + (t < last ? t : last);
-        cs[t] = nc > 0 ? les[i] : 0;
+        cs[t] = nc > 0 ? les[i] * nr + r : 0;
         cv[t] = nc > 0 ? lev[i] : T(0);
     }
 }
 
+// One lane's items: item it is position it / nr of the unit, right-hand side
+// it % nr; the lane takes it, it + stride, ... in one flat loop.
 template <class T, bool UPPER, int CH>
 __device__ __forceinline__ void lane_solve(const DfArgs &a, T *sl, const T *lb, const T *lg, const int2 *lrec,
                                            const int *lslot, const int *lrow, const T *lev, const int *les, T *pub,
-                                           T *nxt, int p, int stride, int nk) {
-    bool have = p < nk;
-    int eo = 0, cnt = 0, k = 0, nc = 0, pc = 0, slot = 0, row = 0;
+                                           T *nxt, int it, int stride, int nk) {
+    const int nr = a.nrhs, items = nk * nr;
+    bool have = it < items;
+    int eo = 0, cnt = 0, k = 0, nc = 0, pc = 0, vslot = 0, vrow = 0, r = 0;
     bool ext = false;
     T s = T(0), bv = T(0), dv = T(1);
     int cs[CH];
     T cv[CH];
     long long spins = 0;
-    if (have) {
+    // start the item `it`: its row record, b and first chunk (LDS only)
+    auto start = [&]() {
+        const int p = nr == 1 ? it : it / nr;
+        r = it - p * nr;
         const int2 rc = lrec[p];
         eo = rc.x;
         cnt = rc.y & 0x7fffffff;
         ext = rc.y < 0;
-        slot = lslot[p];
-        row = lrow[p];
-        bv = lb[p];
+        vslot = lslot[p] * nr + r;
+        vrow = lrow[p] * nr + r;
+        bv = lb[p * nr + r];
         dv = lg[p];
+        k = 0;
+        pc = 0;
+        s = T(0);
+        spins = 0;
         nc = cnt < CH ? cnt : CH;
-        lane_chunk<T, UPPER, CH>(lev, les, eo, 0, nc, cs, cv);
-    }
+        lane_chunk<T, UPPER, CH>(lev, les, eo, 0, nc, nr, r, cs, cv);
+    };
+    if (have) start();
     // One loop whose exit is uniform across the wavefront and whose body is
     // a single divergent step, so no lane waits at an inner loop's exit for
-    // the others: a lane whose row is solved starts its next row in the
+    // the others: a lane whose row is solved starts its next item in the
     // same iteration in which the others keep polling.
     while (__ballot(have) != 0) {
         if (!have) continue;
@@ -516,35 +534,20 @@ __device__ __forceinline__ void lane_solve(const DfArgs &a, T *sl, const T *lb, 
             if (k < cnt) {  // rows longer than one chunk
                 nc = cnt - k < CH ? cnt - k : CH;
                 pc = 0;
-                lane_chunk<T, UPPER, CH>(lev, les, eo, k, nc, cs, cv);
+                lane_chunk<T, UPPER, CH>(lev, les, eo, k, nc, nr, r, cs, cv);
             } else {
                 const T xv = (bv - s) / dv;
-                st_lds(&sl[slot], xv);
+                st_lds(&sl[vslot], xv);
                 if (ext) {
-                    st_agent(&pub[row], xv);
-                    nxt[row] = Sentinel<T>::value();
+                    st_agent(&pub[vrow], xv);
+                    nxt[vrow] = Sentinel<T>::value();
                 }
-                p += stride;
-                have = p < nk;
-                if (have) {
-                    const int2 rc = lrec[p];
-                    eo = rc.x;
-                    cnt = rc.y & 0x7fffffff;
-                    ext = rc.y < 0;
-                    slot = lslot[p];
-                    row = lrow[p];
-                    bv = lb[p];
-                    dv = lg[p];
-                    k = 0;
-                    pc = 0;
-                    s = T(0);
-                    spins = 0;
-                    nc = cnt < CH ? cnt : CH;
-                    lane_chunk<T, UPPER, CH>(lev, les, eo, 0, nc, cs, cv);
-                }
+                it += stride;
+                have = it < items;
+                if (have) start();
             }
         } else if (++spins > a.spin_limit) {
-            raise_status(a.status, BAAS_TIMEOUT, row);
+            raise_status(a.status, BAAS_TIMEOUT, vrow);
             pc = nc;  // give up on this row's remaining parents
             cnt = k + nc;
         }
@@ -560,8 +563,9 @@ __global__ __launch_bounds__(1024) void k_sweep_lane(DfArgs a, const int *__rest
     if (a.skip && ld_agent(a.skip)) return;
     const int epoch = read_epoch(a.ctx);
     const int par = ld_agent(a.pcount) & 1;
-    T *pub = xp + (size_t)par * a.n;
-    T *nxt = xp + (size_t)(par ^ 1) * a.n;
+    const int nr = a.nrhs;
+    T *pub = xp + (size_t)par * a.n * nr;
+    T *nxt = xp + (size_t)(par ^ 1) * a.n * nr;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
     for (;;) {
         const int u = claim_unit(a.ctx.ticket, &s_unit);
@@ -569,7 +573,7 @@ __global__ __launch_bounds__(1024) void k_sweep_lane(DfArgs a, const int *__rest
         const int q0 = a.unit_ptr[u], q1 = a.unit_ptr[u + 1], nk = q1 - q0;
         const int e0 = a.ext_ptr[u], ne = a.ext_ptr[u + 1] - e0;
         const int kb = a.key_begin[u];
-        const LaneLayout lay = lane_layout(nk, ne, eptr[u + 1] - eptr[u], sizeof(T));
+        const LaneLayout lay = lane_layout(nk, ne, eptr[u + 1] - eptr[u], sizeof(T), nr);
         T *sl = reinterpret_cast<T *>(smem_raw + lay.sl);
         T *lb = reinterpret_cast<T *>(smem_raw + lay.bv);
         T *lg = reinterpret_cast<T *>(smem_raw + lay.dv);
@@ -578,15 +582,15 @@ __global__ __launch_bounds__(1024) void k_sweep_lane(DfArgs a, const int *__rest
         int *lrow = reinterpret_cast<int *>(smem_raw + lay.row);
         T *lev = reinterpret_cast<T *>(smem_raw + lay.ev);
         int *les = reinterpret_cast<int *>(smem_raw + lay.es);
-        for (int i = threadIdx.x; i < nk + ne; i += blockDim.x) st_lds(&sl[i], Sentinel<T>::value());
+        for (int i = threadIdx.x; i < (nk + ne) * nr; i += blockDim.x) st_lds(&sl[i], Sentinel<T>::value());
         for (int q = threadIdx.x; q < nk; q += blockDim.x) {
             const int4 m = a.meta[q0 + q];
-            const int cnt = m.z, eo = eoff[q0 + q];
-            lb[q] = b[m.x & 0x7fffffff];
+            const int cnt = m.z, eo = eoff[q0 + q], row = m.x & 0x7fffffff;
+            for (int r = 0; r < nr; r++) lb[q * nr + r] = b[(size_t)row * nr + r];
             lg[q] = vals[UPPER ? m.y - cnt : m.y + cnt];
             lrec[q] = make_int2(eo, cnt | (m.x & (int)0x80000000));
             lslot[q] = m.w;
-            lrow[q] = m.x & 0x7fffffff;
+            lrow[q] = row;
             for (int e = 0; e < cnt; e++) {
                 const int j = UPPER ? m.y - e : m.y + e;
                 lev[eo + e] = vals[j];
@@ -602,10 +606,10 @@ __global__ __launch_bounds__(1024) void k_sweep_lane(DfArgs a, const int *__rest
         }
         __syncthreads();
         // the unit's solution, slot i holding the row with key kb + i
-        for (int i = threadIdx.x; i < nk; i += blockDim.x) {
-            const int key = kb + i;
+        for (int i = threadIdx.x; i < nk * nr; i += blockDim.x) {
+            const int key = kb + (nr == 1 ? i : i / nr);
             const int row = a.keymode == 0 ? key : a.keymode == 1 ? a.n - 1 - key : a.key_row[key];
-            x[row] = sl[i];
+            x[(size_t)row * nr + (nr == 1 ? 0 : i % nr)] = sl[i];
         }
     }
     if (threadIdx.x == 0) {
@@ -916,19 +920,23 @@ int sweep(baas_handle *h, baas_schedule *s, int uplo, const T *vals, const T *b,
     if ((const void *)b == (const void *)x) return BAAS_INVALID_ARG;
     Plan &p = uplo ? s->bwd : s->fwd;
     if (p.mode == BAAS_PLAN_LEVELS) return launch_levels<T>(h, s, p, uplo, vals, b, x, nrhs, stream, skip);
-    const bool lane = nrhs == 1 && p.dataflow == 2 && p.block >= 128 && p.lane_lds[sizeof(T) == 8] <= lds_limit(h);
+    const bool lane = p.dataflow == 2 && p.block >= 128 && p.lane_lds((int)sizeof(T), nrhs) <= lds_limit(h);
     if (lane || (nrhs == 1 && p.dataflow == 1 && p.block >= 192 && p.block <= 512 &&
                  p.df_lds[sizeof(T) == 8] <= lds_limit(h))) {
-        // publication halves, armed with this value type's sentinel
-        if (p.xp_type != (int)sizeof(T)) {
-            const size_t cnt = 2 * (size_t)s->n;
+        // publication halves (one value per row and right-hand side), armed
+        // with this value type's sentinel; any change of layout re-arms them
+        const int xr = lane ? nrhs : 1;
+        if (p.xp_type != (int)sizeof(T) || p.xp_nrhs != xr) {
+            const size_t cnt = 2 * (size_t)s->n * xr;
             BAAS_TRY(p.xp_raw.alloc(cnt * sizeof(T)));
             hipLaunchKernelGGL(k_fill_sentinel<T>, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, stream, cnt,
                                reinterpret_cast<T *>(p.xp_raw.p));
             BAAS_HIP_TRY(hipGetLastError());
             p.xp_type = (int)sizeof(T);
+            p.xp_nrhs = xr;
         }
         DfArgs d;
+        d.nrhs = xr;
         d.pcount = p.pcount.p;
         d.n = s->n;
         d.nunits = p.nunits;
@@ -948,7 +956,7 @@ int sweep(baas_handle *h, baas_schedule *s, int uplo, const T *vals, const T *b,
         d.spin_limit = h->spin_limit;
         T *xp = reinterpret_cast<T *>(p.xp_raw.p);
         if (lane) {
-            const size_t lds = p.lane_lds[sizeof(T) == 8];
+            const size_t lds = p.lane_lds((int)sizeof(T), nrhs);
             return uplo ? launch_lane<T, true>(h, p, d, vals, b, x, xp, lds, stream)
                         : launch_lane<T, false>(h, p, d, vals, b, x, xp, lds, stream);
         }

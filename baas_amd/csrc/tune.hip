@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "blas.h"
+
 #include "internal.h"
 
 namespace baas {
@@ -65,7 +66,9 @@ int autotune(baas_handle *h, baas_schedule *s, const T *vals, int uplo, int nrhs
             if (rpu <= 16384 && !big) cands.push_back({BAAS_PLAN_RANGE, rpu, blk, 0, 0});
             if (nrhs == 1 && blk <= 512) cands.push_back({BAAS_PLAN_RANGE, rpu, blk, 1, 0});
         }
-    // lane kernel: units small enough for their rows to live in LDS
+    // lane kernel: units small enough for their rows to live in LDS (with
+    // several right-hand sides its units must shrink to one row and column
+    // per lane, and the unit and level-set kernels measured faster)
     if (nrhs == 1)
         for (int rpu : {384, 512, 768, 1024, 1536})
             for (int blk : {512, 768, 1024}) cands.push_back({BAAS_PLAN_RANGE, rpu, blk, 2, 0});
@@ -116,7 +119,7 @@ int autotune(baas_handle *h, baas_schedule *s, const T *vals, int uplo, int nrhs
         s->fwd.block = s->bwd.block = c.block;
         Plan &p = uplo ? s->bwd : s->fwd;
         if (c.df == 1 && p.df_lds[sizeof(T) == 8] > lds_limit(h)) continue;
-        if (c.df == 2 && p.lane_lds[sizeof(T) == 8] > lds_limit(h)) continue;
+        if (c.df == 2 && p.lane_lds((int)sizeof(T), nrhs) > lds_limit(h)) continue;
         s->fwd.dataflow = s->bwd.dataflow = c.df;
         float t_best = 1e30f;
         bool ok = true;

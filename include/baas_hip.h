@@ -235,11 +235,12 @@ int baas_replan_tiles(baas_schedule *s, int row_range, int band_levels, int bloc
 /* The plan in use (host int[5]): mode, rows per unit (TILES: row range),
  * threads per workgroup, dataflow, band levels (TILES, else 0). */
 int baas_schedule_plan(const baas_schedule *s, int *plan);
-/* Single-RHS sweeps of this schedule use a dataflow kernel when the unit's
- * data fits in LDS: enable 1 = rows in rounds of one per lane with prefetch
+/* Sweeps of this schedule use a dataflow kernel when the unit's data fits
+ * in LDS: enable 1 (single right-hand side) = rows in rounds of one per lane with prefetch
  * and exchange wavefronts for parents of other units; enable 2 = the unit's
- * rows staged in LDS, every lane advancing through its own rows on its own
- * and publishing them itself (workgroups of at least 128 threads); 0 = off. */
+ * rows staged in LDS, every lane advancing through its own rows (one row and
+ * right-hand side per item, any nrhs) on its own and publishing them itself
+ * (workgroups of at least 128 threads); 0 = off. */
 int baas_plan_dataflow(baas_schedule *s, int enable);
 /* Autotune the execution plan for sweeps with `uplo` and `nrhs` on these
  * values: times RANGE, SCHEDULE, LEVELS and TILES packings over their unit

@@ -118,9 +118,11 @@ def test_multi_rhs_bit_exact(handle, k):
         s = Schedule(handle, L.rowptr, L.colidx, kind=0)
         B = g.rhs_uniform(L.n, 3, k=k)
         X_ref = O.sptrsv_lower_multi(L, B)
-        for mode, rpu, blk in [(1, 0, 256), (1, 256, 256), (2, 0, 256), (3, 0, 512), (4, 16384, 256), (3, 1, 1024)]:
-            announce("multi", name, k, mode, rpu, blk)
-            s.replan(mode, rpu, blk)
+        for mode, rpu, blk, df in [(1, 0, 256, 0), (1, 256, 256, 0), (2, 0, 256, 0), (3, 0, 512, 0),
+                                   (4, 16384, 256, 0), (3, 1, 1024, 0), (1, 32, 1024, 2), (2, 16, 512, 2),
+                                   (1, 300, 128, 2)]:
+            announce("multi", name, k, mode, rpu, blk, df)
+            s.replan(mode, rpu, blk, dataflow=df)
             X = down(s.solve(up(L.values), up(B)))
             handle.check()
             np.testing.assert_array_equal(X, X_ref)
