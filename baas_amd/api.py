@@ -139,7 +139,8 @@ class Schedule:
         """Time the IC0/ILU0 factorisation over plans and keep the fastest."""
         c = (C.c_int * 5)()
         L.call("baas_autotune_factor", self.h.ptr, self._s, _ptr(a_vals), C.cast(c, C.c_void_p))
-        return {"mode": c[0], "rows_per_unit": c[1], "block": c[2], "us": c[4], "band": self.plan()["band"]}
+        return {"mode": c[0], "rows_per_unit": c[1], "block": c[2], "dataflow": c[3], "us": c[4],
+                "band": self.plan()["band"]}
 
     def upper_pattern(self):
         rp, ci = C.c_void_p(), C.c_void_p()

@@ -136,4 +136,78 @@ __device__ __forceinline__ void reset_barrier(int *bar) {
     for (int p = 0; p < 8; p++) atomicExch(&bar[32 * p], 0);
 }
 
+// Value-as-flag publication: a signalling-NaN bit pattern that arithmetic
+// never produces marks a slot whose value has not arrived yet.
+template <class T> struct Sentinel;
+template <> struct Sentinel<double> {
+    static __device__ __forceinline__ double value() { return __longlong_as_double(0x7ff4deadbeef0001LL); }
+    static __device__ __forceinline__ bool is(double v) { return __double_as_longlong(v) == 0x7ff4deadbeef0001LL; }
+};
+template <> struct Sentinel<float> {
+    static __device__ __forceinline__ float value() { return __int_as_float(0x7fa0dead); }
+    static __device__ __forceinline__ bool is(float v) { return __float_as_int(v) == 0x7fa0dead; }
+};
+
+// LDS accesses that other wavefronts of the workgroup poll
+template <class T>
+__device__ __forceinline__ T ld_lds(const T *p) {
+    return __hip_atomic_load(const_cast<T *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+template <class T>
+__device__ __forceinline__ void st_lds(T *p, T v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+constexpr int DF_PF = 8;  // entries an exchange lane polls at once
+
+// Exchange wavefront, incoming side: copy published values from other
+// workgroups into LDS slots as they appear.  With nr values per source,
+// item i is source cols[i / nr], value i % nr, read at cols[.] * nr + i % nr
+// and staged at slot base * nr + i.  Each lane walks its own stride of the
+// (first-use ordered) list, DF_PF items at a time: it polls all of them with
+// one load each per pass and stages each as soon as it is published.  A
+// wait beyond the spin limit reports BAAS_TIMEOUT and stages zeros, so the
+// workgroup still finishes.
+template <class T>
+__device__ __forceinline__ void stage_published(const T *pub, T *sl, const int *cols, int base, int ne, int nr,
+                                                int lane, long long limit, DevStatus *status) {
+    const int items = ne * nr;
+    for (int e = lane; e < items; e += DF_PF * 64) {
+        int c[DF_PF];
+#pragma unroll
+        for (int k = 0; k < DF_PF; k++) {
+            const int ek = e + k * 64 < items ? e + k * 64 : items - 1;
+            c[k] = nr == 1 ? cols[ek] : cols[ek / nr] * nr + ek % nr;
+        }
+        unsigned todo = 0;
+#pragma unroll
+        for (int k = 0; k < DF_PF; k++)
+            if (e + k * 64 < items) todo |= 1u << k;
+        long long spins = 0;
+        while (todo) {
+            T v[DF_PF];
+#pragma unroll
+            for (int k = 0; k < DF_PF; k++) v[k] = ld_agent(&pub[c[k]]);
+#pragma unroll
+            for (int k = 0; k < DF_PF; k++) {
+                if (((todo >> k) & 1u) && !Sentinel<T>::is(v[k])) {
+                    st_lds(&sl[base * nr + e + k * 64], v[k]);
+                    todo &= ~(1u << k);
+                }
+            }
+            if (todo) {
+                if (++spins > limit) {
+                    raise_status(status, BAAS_TIMEOUT, c[__builtin_ctz(todo)]);
+#pragma unroll
+                    for (int k = 0; k < DF_PF; k++)
+                        if ((todo >> k) & 1u) st_lds(&sl[base * nr + e + k * 64], T(0));
+                    todo = 0;
+                } else {
+                    __builtin_amdgcn_s_sleep(1);
+                }
+            }
+        }
+    }
+}
+
 }  // namespace baas

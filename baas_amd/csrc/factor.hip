@@ -21,6 +21,8 @@
 
 namespace baas {
 int build_ic0_plans(baas_schedule *s);
+int build_ic0_lane(baas_schedule *s);
+size_t lds_limit(baas_handle *h);
 
 struct FactorArgs {
     int n, nunits;
@@ -257,6 +259,216 @@ __global__ __launch_bounds__(1024) void k_factor_levels(FactorArgs a, int nlev, 
     }
 }
 
+// ---------------------------------------------------------------------------
+// IC0 on the lane kernel (dataflow plans).  Every lower entry and diagonal
+// that another unit reads is published into a double-buffered array indexed
+// by its lu position, the value being its own flag (as the sweep publishes
+// x).  A unit stages its rows' records, entries (A values, diagonal slots,
+// pair lists) and pairs in LDS; wavefront 0 stages the external entries the
+// unit reads as they are published; every other lane factors its own rows,
+// entry by entry, each dot over the precomputed pairs in the reference's
+// order (merge/IC.cpp:6-29, the products L_ic * L_jc ascending in c), then
+// the diagonal with the self dot in the reference build's order.  No level
+// barrier and no flag wait: an entry starts as soon as its own inputs are in
+// LDS.
+struct IclArgs {
+    int nunits;
+    size_t nnz;
+    const int *unit_ptr;  // the forward plan's units (positions)
+    const int4 *rec, *ent, *unit;
+    const int *pair, *ext;
+    const int *diagptr;
+    int *pcount;
+    Ctx ctx;
+    DevStatus *status;
+    long long spin_limit;
+};
+
+constexpr int ICL_CH = 4;  // pairs read per poll
+
+__device__ __forceinline__ void icl_factor_rows(const IclArgs &a, double *sl, const int4 *lrec, const double *laii,
+                                                const int *ldgp, const int4 *lent, const double *leav,
+                                                const int *lpair, double *lu, double *pub, double *nxt, int p,
+                                                int stride, int nk) {
+    bool have = p < nk;
+    int row = 0, eo = 0, ne = 0, base = 0, dgp = 0, e = 0, k = 0, djs = 0, ps = 0, np = 0, pc = 0;
+    bool dpub = false, epub = false, force = false;
+    double dp = 0.0, aii = 0.0;
+    long long spins = 0;
+    auto load_entry = [&]() {
+        if (e < ne) {
+            const int4 q = lent[eo + e];
+            k = q.x;
+            djs = q.y;
+            ps = q.z;
+            np = q.w & 0x7fffffff;
+            epub = q.w < 0;
+        }
+        pc = 0;
+        dp = 0.0;
+        spins = 0;
+        force = false;
+    };
+    auto start = [&]() {
+        const int4 r = lrec[p];
+        row = r.x;
+        eo = r.y;
+        ne = r.z & 0x7fffffff;
+        dpub = r.z < 0;
+        base = r.w;
+        aii = laii[p];
+        dgp = ldgp[p];
+        e = 0;
+        load_entry();
+    };
+    if (have) start();
+    // one loop with a wavefront-uniform exit and a single divergent step
+    while (__ballot(have) != 0) {
+        if (!have) continue;
+        if (e < ne) {
+            if (pc < np) {
+                double v[ICL_CH], o[ICL_CH];
+#pragma unroll
+                for (int t = 0; t < ICL_CH; t++) {
+                    const int i = ps + (pc + t < np ? pc + t : np - 1);
+                    const int w = lpair[i];
+                    v[t] = ld_lds(&sl[w >> 8]);
+                    o[t] = ld_lds(&sl[base + (w & 255)]);
+                }
+                int took = 0;
+#pragma unroll
+                for (int t = 0; t < ICL_CH; t++) {
+                    if (t == took && pc + t < np && (force || !Sentinel<double>::is(v[t]))) {
+                        dp = fma(o[t], Sentinel<double>::is(v[t]) ? 0.0 : v[t], dp);
+                        took++;
+                    }
+                }
+                pc += took;
+            }
+            bool done_entry = false;
+            if (pc == np) {
+                const double dj = ld_lds(&sl[djs]);
+                if (force || !Sentinel<double>::is(dj)) {
+                    const double L = (leav[eo + e] - dp) / (Sentinel<double>::is(dj) ? 1.0 : dj);
+                    st_lds(&sl[base + e], L);
+                    lu[k] = L;
+                    if (epub) {
+                        st_agent(&pub[k], L);
+                        nxt[k] = Sentinel<double>::value();
+                    }
+                    e++;
+                    load_entry();
+                    done_entry = true;
+                }
+            }
+            if (!done_entry && !force && ++spins > a.spin_limit) {
+                raise_status(a.status, BAAS_TIMEOUT, row);
+                force = true;  // finish this entry with what has arrived
+            }
+        } else {
+            // diagonal: the self dot in the reference build's order (blocks
+            // of four rounded squares added in order when there are more
+            // than three, FMA tail), then the pivot rule of merge/IC.cpp:82-87
+            double r = 0.0;
+            const int stop = ne > 3 ? (ne & ~3) : 0;
+            for (int t0 = 0; t0 < ne; t0 += 4) {
+                double v[4];
+#pragma unroll
+                for (int t = 0; t < 4; t++) v[t] = ld_lds(&sl[base + (t0 + t < ne ? t0 + t : ne - 1)]);
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    if (t0 + t < ne) {
+                        if (t0 + t < stop) {
+                            const double sq = v[t] * v[t];
+                            r = r + sq;
+                        } else {
+                            r = fma(v[t], v[t], r);
+                        }
+                    }
+                }
+            }
+            double d = aii;
+            if (aii - r > 0) {
+                d = sqrt(aii - r);
+            } else {
+                raise_status(a.status, BAAS_NONPOSITIVE_PIVOT, row);
+            }
+            st_lds(&sl[base + ne], d);
+            lu[dgp] = d;
+            if (dpub) {
+                st_agent(&pub[dgp], d);
+                nxt[dgp] = Sentinel<double>::value();
+            }
+            p += stride;
+            have = p < nk;
+            if (have) start();
+        }
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_factor_lane(IclArgs a, double *lu, double *xp) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ int s_unit;
+    const int epoch = read_epoch(a.ctx);
+    const int par = ld_agent(a.pcount) & 1;
+    double *pub = xp + (size_t)par * a.nnz;
+    double *nxt = xp + (size_t)(par ^ 1) * a.nnz;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    for (;;) {
+        const int u = claim_unit(a.ctx.ticket, &s_unit);
+        if (u >= a.nunits) break;
+        const int q0 = a.unit_ptr[u], nk = a.unit_ptr[u + 1] - q0;
+        const int4 U0 = a.unit[u], U1 = a.unit[u + 1];
+        const int nown = U0.x, nent = U1.y - U0.y, npair = U1.z - U0.z, next = U1.w - U0.w;
+        const IclLayout lay = icl_layout(nown + next, nk, nent, npair);
+        double *sl = reinterpret_cast<double *>(smem_raw + lay.vals);
+        int4 *lrec = reinterpret_cast<int4 *>(smem_raw + lay.rec);
+        double *laii = reinterpret_cast<double *>(smem_raw + lay.aii);
+        int *ldgp = reinterpret_cast<int *>(smem_raw + lay.dgp);
+        int4 *lent = reinterpret_cast<int4 *>(smem_raw + lay.ent);
+        double *leav = reinterpret_cast<double *>(smem_raw + lay.eav);
+        int *lpair = reinterpret_cast<int *>(smem_raw + lay.pair);
+        for (int i = threadIdx.x; i < nown + next; i += blockDim.x) st_lds(&sl[i], Sentinel<double>::value());
+        for (int q = threadIdx.x; q < nk; q += blockDim.x) {
+            const int4 r = a.rec[q0 + q];
+            const int dg = a.diagptr[r.x];
+            lrec[q] = r;
+            ldgp[q] = dg;
+            laii[q] = lu[dg];
+        }
+        for (int i = threadIdx.x; i < nent; i += blockDim.x) {
+            const int4 q = a.ent[U0.y + i];
+            lent[i] = q;
+            leav[i] = lu[q.x];
+        }
+        for (int i = threadIdx.x; i < npair; i += blockDim.x) lpair[i] = a.pair[U0.z + i];
+        __syncthreads();
+        if (wv == 0) {
+            stage_published<double>(pub, sl, a.ext + U0.w, nown, next, 1, lane, a.spin_limit, a.status);
+        } else {
+            icl_factor_rows(a, sl, lrec, laii, ldgp, lent, leav, lpair, lu, pub, nxt, (wv - 1) * 64 + lane,
+                            (nw - 1) * 64, nk);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const int d = atomicAdd(a.ctx.done, 1);
+        if (d == (int)gridDim.x - 1) {
+            atomicExch(a.ctx.ticket, 0);
+            atomicExch(a.ctx.epoch, epoch == 0x7fffffff ? 1 : epoch + 1);
+            atomicAdd(a.pcount, 1);
+            __threadfence();
+            atomicExch(a.ctx.done, 0);
+        }
+    }
+}
+
+__global__ void k_fill_sentinel_f64(size_t n, double *p) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = Sentinel<double>::value();
+}
+
 __global__ void k_copy_f64(int n, const double *__restrict__ in, double *__restrict__ out) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) out[i] = in[i];
@@ -319,6 +531,43 @@ static int factor(baas_handle *h, baas_schedule *s, const double *a_vals, double
     a.ctx = Ctx{h->counters, h->counters + 1, h->counters + 2};
     a.status = h->status;
     a.spin_limit = h->spin_limit;
+    if (KIND == BAAS_KIND_IC0 && p.dataflow == 2 && p.block >= 128) {
+        const int rc = build_ic0_lane(s);
+        if (rc != BAAS_OK && rc != BAAS_UNSUPPORTED) return rc;
+        if (rc == BAAS_OK && s->icl_lds <= lds_limit(h)) {
+            const size_t nnz = s->nnz_full;
+            if (!s->icl_armed) {
+                hipLaunchKernelGGL(k_fill_sentinel_f64, dim3((unsigned)((2 * nnz + 255) / 256)), dim3(256), 0,
+                                   h->stream, 2 * nnz, reinterpret_cast<double *>(s->icl_xp.p));
+                BAAS_HIP_TRY(hipGetLastError());
+                s->icl_armed = true;
+            }
+            IclArgs c;
+            c.nunits = p.nunits;
+            c.nnz = nnz;
+            c.unit_ptr = p.unit_ptr.p;
+            c.rec = reinterpret_cast<const int4 *>(s->icl_rec.p);
+            c.ent = reinterpret_cast<const int4 *>(s->icl_ent.p);
+            c.unit = reinterpret_cast<const int4 *>(s->icl_unit.p);
+            c.pair = s->icl_pair.p;
+            c.ext = s->icl_ext.p;
+            c.diagptr = s->a_diagptr.p;
+            c.pcount = s->icl_pcount.p;
+            c.ctx = Ctx{h->counters, h->counters + 1, h->counters + 2};
+            c.status = h->status;
+            c.spin_limit = h->spin_limit;
+            int per_cu = 0;
+            BAAS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_factor_lane, p.block, s->icl_lds));
+            if (per_cu > 0) {
+                long long grid = std::min<long long>((long long)per_cu * h->num_cus, p.nunits);
+                if (grid < 1) grid = 1;
+                hipLaunchKernelGGL(k_factor_lane, dim3((unsigned)grid), dim3(p.block), s->icl_lds, h->stream, c, lu,
+                                   reinterpret_cast<double *>(s->icl_xp.p));
+                BAAS_HIP_TRY(hipGetLastError());
+                return BAAS_OK;
+            }
+        }
+    }
     if (p.mode == BAAS_PLAN_LEVELS) {
         auto kern = k_factor_levels<KIND>;
         const size_t lds = KIND == BAAS_KIND_IC0 ? (size_t)IC_MAXE * p.block * sizeof(double) : 0;

@@ -145,6 +145,27 @@ constexpr int BAAS_COUNTER_INTS = BAAS_BARRIER_BASE + 8 * 32;
 // TILES plans: largest work unit (rows); its values fit the dataflow kernel's LDS
 constexpr int BAAS_TILE_MAX_ROWS = 6144;
 
+// IC0 lane kernel: LDS layout of one unit (byte offsets, 16-byte aligned
+// sections): value slots (the unit's lower entries and diagonals, then the
+// external entries it reads), per position record, A's diagonal value and
+// its lu position, per entry record and A value, per pair word.
+struct IclLayout {
+    size_t vals, rec, aii, dgp, ent, eav, pair, total;
+};
+__host__ __device__ inline size_t icl_align16(size_t v) { return (v + 15) & ~size_t(15); }
+__host__ __device__ inline IclLayout icl_layout(size_t nslots, size_t npos, size_t nent, size_t npair) {
+    IclLayout l;
+    l.vals = 0;
+    l.rec = icl_align16(nslots * 8);
+    l.aii = l.rec + npos * 16;
+    l.dgp = l.aii + icl_align16(npos * 8);
+    l.ent = l.dgp + icl_align16(npos * 4);
+    l.eav = l.ent + nent * 16;
+    l.pair = l.eav + nent * 8;
+    l.total = l.pair + icl_align16(npair * 4);
+    return l;
+}
+
 struct baas_handle {
     hipStream_t stream = nullptr;
     int device = 0;
@@ -188,6 +209,17 @@ struct baas_schedule {
     // summation order
     baas::DBuf<int> ic_djpos, ic_pptr, ic_pairs;
     bool ic_plans = false;
+    // IC0 on the lane kernel, built for the forward plan of generation
+    // icl_gen: per position {row, entry offset, count | publish flag, own
+    // slot base}; per entry {lu position, diagonal slot, pair offset, pair
+    // count | publish flag}; per pair slot << 8 | own index; per unit {own
+    // slots, entry start, pair start, external start}; external lu positions
+    baas::DBuf<int> icl_rec, icl_ent, icl_pair, icl_unit, icl_ext;
+    baas::DBuf<unsigned char> icl_xp;  // 2 x nnz(A) published values
+    baas::DBuf<int> icl_pcount;
+    int icl_gen = -1, icl_status = 0;
+    bool icl_armed = false;  // icl_xp holds sentinels
+    size_t icl_lds = 0;
     double t_gpu = 0, t_host = 0;
     baas_opts opts{};
 };

@@ -531,6 +531,127 @@ int build_ic0_plans(baas_schedule *s) {
     return BAAS_OK;
 }
 
+// IC0 on the lane kernel (dataflow plans): for the forward plan's units,
+// the value slots of every lower entry and diagonal a unit computes or
+// reads, the entry and pair records in the reference's order (pairs are the
+// products of the up-looking sparse dot, merge/IC.cpp:6-29, as in
+// build_ic0_plans), and the entries other units read, which are published.
+// Rows with more than 255 lower entries are not supported (their own-entry
+// index is a byte): BAAS_UNSUPPORTED, and the factorisation keeps the unit
+// kernels.
+int build_ic0_lane(baas_schedule *s) {
+    if (s->icl_gen == s->plan_gen) return s->icl_status;
+    s->icl_gen = s->plan_gen;
+    s->icl_status = BAAS_UNSUPPORTED;
+    const Plan &p = s->fwd;
+    if (p.mode == BAAS_PLAN_LEVELS || (int)p.h_rows.size() != s->n) return s->icl_status;
+    hipStream_t st = s->h->stream;
+    const int n = s->n;
+    HostCsr A;
+    BAAS_TRY(fetch(A, s->a_rowptr.p, s->a_colidx.p, n, st));
+    std::vector<int> dg(n);
+    if (hipMemcpy(dg.data(), s->a_diagptr.p, sizeof(int) * n, hipMemcpyDeviceToHost) != hipSuccess)
+        return BAAS_HIP_ERROR;
+    const size_t nnz = A.idx.size();
+    const int nunits = p.nunits;
+    std::vector<int> unit_of(n), pos_of(n), row_of_lu(nnz, -1);
+    for (int u = 0; u < nunits; u++)
+        for (int q = p.h_unit_ptr[u]; q < p.h_unit_ptr[u + 1]; q++) unit_of[p.h_rows[q]] = u, pos_of[p.h_rows[q]] = q;
+    for (int r = 0; r < n; r++) {
+        if (dg[r] - A.ptr[r] > 255) return s->icl_status;
+        for (int k = A.ptr[r]; k <= dg[r]; k++) row_of_lu[k] = r;
+    }
+    std::vector<int> rec(4 * (size_t)n), ent, pair, unit(4 * ((size_t)nunits + 1), 0), ext;
+    std::vector<char> pub(nnz, 0);
+    std::vector<int> own_base(n, 0), ext_slot(nnz, -1), touched;
+    size_t lds = 0;
+    for (int u = 0; u < nunits; u++) {
+        const int q0 = p.h_unit_ptr[u], q1 = p.h_unit_ptr[u + 1];
+        int nown = 0;
+        for (int q = q0; q < q1; q++) {
+            const int i = p.h_rows[q];
+            own_base[i] = nown;
+            nown += dg[i] - A.ptr[i] + 1;
+        }
+        const int ent0 = (int)ent.size() / 4, pair0 = (int)pair.size(), ext0 = (int)ext.size();
+        unit[4 * (size_t)u + 0] = nown;
+        unit[4 * (size_t)u + 1] = ent0;
+        unit[4 * (size_t)u + 2] = pair0;
+        unit[4 * (size_t)u + 3] = ext0;
+        // the value slot of lu position P for unit u
+        auto slot_of = [&](int P) {
+            const int r = row_of_lu[P];
+            if (unit_of[r] == u) return own_base[r] + (P - A.ptr[r]);
+            if (ext_slot[P] < 0) {
+                ext_slot[P] = nown + (int)(ext.size() - ext0);
+                ext.push_back(P);
+                touched.push_back(P);
+                pub[P] = 1;
+            }
+            return ext_slot[P];
+        };
+        for (int q = q0; q < q1; q++) {
+            const int i = p.h_rows[q], rb = A.ptr[i], di = dg[i], ne = di - rb;
+            rec[4 * (size_t)q + 0] = i;
+            rec[4 * (size_t)q + 1] = (int)ent.size() / 4 - ent0;
+            rec[4 * (size_t)q + 2] = ne;
+            rec[4 * (size_t)q + 3] = own_base[i];
+            for (int k = rb; k < di; k++) {
+                const int j = A.idx[k];
+                const int ps = (int)pair.size() - pair0;
+                int l1 = rb, l2 = A.ptr[j];
+                const int u1 = di, u2 = dg[j];
+                while (l1 < u1 && l2 < u2) {
+                    const int c1 = A.idx[l1], c2 = A.idx[l2];
+                    if (c1 == c2) {
+                        pair.push_back((slot_of(l2) << 8) | (l1 - rb));
+                        l1++;
+                        l2++;
+                    } else if (c1 < c2) {
+                        l1++;
+                    } else {
+                        l2++;
+                    }
+                }
+                ent.push_back(k);
+                ent.push_back(slot_of(dg[j]));
+                ent.push_back(ps);
+                ent.push_back((int)pair.size() - pair0 - ps);
+            }
+        }
+        const size_t nslots = nown + ext.size() - ext0;
+        if (nslots >= (1u << 23)) return s->icl_status;
+        lds = std::max(lds, icl_layout(nslots, q1 - q0, ent.size() / 4 - ent0, pair.size() - pair0).total);
+        for (int P : touched) ext_slot[P] = -1;
+        touched.clear();
+    }
+    unit[4 * (size_t)nunits + 0] = 0;
+    unit[4 * (size_t)nunits + 1] = (int)ent.size() / 4;
+    unit[4 * (size_t)nunits + 2] = (int)pair.size();
+    unit[4 * (size_t)nunits + 3] = (int)ext.size();
+    // publish flags: bit 31 of the count word
+    for (size_t e = 0; e < ent.size(); e += 4)
+        if (pub[ent[e]]) ent[e + 3] |= (int)0x80000000;
+    for (int q = 0; q < n; q++)
+        if (pub[dg[rec[4 * (size_t)q]]]) rec[4 * (size_t)q + 2] |= (int)0x80000000;
+    if (ent.empty()) ent.assign(4, 0);
+    if (pair.empty()) pair.push_back(0);
+    if (ext.empty()) ext.push_back(0);
+    BAAS_TRY(upload(s->icl_rec, rec, st));
+    BAAS_TRY(upload(s->icl_ent, ent, st));
+    BAAS_TRY(upload(s->icl_pair, pair, st));
+    BAAS_TRY(upload(s->icl_unit, unit, st));
+    BAAS_TRY(upload(s->icl_ext, ext, st));
+    BAAS_TRY(s->icl_pcount.alloc(1));
+    if (hipMemsetAsync(s->icl_pcount.p, 0, sizeof(int), st) != hipSuccess) return BAAS_HIP_ERROR;
+    BAAS_TRY(s->icl_xp.alloc(2 * nnz * sizeof(double)));
+    s->icl_armed = false;
+    if (hipStreamSynchronize(st) != hipSuccess) return BAAS_HIP_ERROR;
+    s->icl_lds = lds;
+    s->icl_status = BAAS_OK;
+    return BAAS_OK;
+}
+
 }  // namespace baas
 
 extern "C" int baas_replan(baas_schedule *s, int mode, int rows_per_unit, int block_threads) {

@@ -123,25 +123,6 @@ __global__ __launch_bounds__(1024) void k_sweep(SweepArgs a, const T *__restrict
 // move values between LDS and the publication array (one stages parents from
 // earlier units, one publishes rows later units read), so the compute waves
 // touch global memory only to load matrix data, a round ahead of its use.
-template <class T> struct Sentinel;
-template <> struct Sentinel<double> {
-    static __device__ __forceinline__ double value() { return __longlong_as_double(0x7ff4deadbeef0001LL); }
-    static __device__ __forceinline__ bool is(double v) { return __double_as_longlong(v) == 0x7ff4deadbeef0001LL; }
-};
-template <> struct Sentinel<float> {
-    static __device__ __forceinline__ float value() { return __int_as_float(0x7fa0dead); }
-    static __device__ __forceinline__ bool is(float v) { return __float_as_int(v) == 0x7fa0dead; }
-};
-
-template <class T>
-__device__ __forceinline__ T ld_lds(const T *p) {
-    return __hip_atomic_load(const_cast<T *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-template <class T>
-__device__ __forceinline__ void st_lds(T *p, T v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
-}
-
 struct DfArgs {
     int n, nunits, keymode;
     int nrhs;  // right-hand sides per row (1 for k_sweep_df)
@@ -157,7 +138,6 @@ struct DfArgs {
     long long spin_limit;
 };
 
-constexpr int DF_PF = 8;  // entries an exchange lane polls at once
 
 // one row's data, loaded a round ahead of its turn; CH parents per chunk
 template <class T, int CH>
@@ -201,53 +181,11 @@ __device__ __forceinline__ void df_load_row(const DfArgs &a, const T *__restrict
 }
 
 // Exchange wave, incoming side: stage the unit's parents in other units into
-// LDS slots [nk, nk + ne).  Each lane walks its own stride of the
-// first-use-ordered list (cols), DF_PF entries at a time: it loads their
-// columns, then polls all of them with one load each per pass and stages
-// each as soon as it is published.
+// LDS slots [nk * nrhs, ...) (stage_published).
 template <class T>
 __device__ __forceinline__ void df_incoming(const DfArgs &a, const T *pub, T *sl, const int *cols, int nk, int ne,
                                             int lane) {
-    // with k right-hand sides, item i is parent i / k, column i % k; its
-    // published value sits at parent * k + i % k and its slot at nk * k + i
-    const int nr = a.nrhs, items = ne * nr;
-    for (int e = lane; e < items; e += DF_PF * 64) {
-        int c[DF_PF];
-#pragma unroll
-        for (int k = 0; k < DF_PF; k++) {
-            const int ek = e + k * 64 < items ? e + k * 64 : items - 1;
-            c[k] = nr == 1 ? cols[ek] : cols[ek / nr] * nr + ek % nr;
-        }
-        unsigned todo = 0;
-#pragma unroll
-        for (int k = 0; k < DF_PF; k++)
-            if (e + k * 64 < items) todo |= 1u << k;
-        long long spins = 0;
-        while (todo) {
-            T v[DF_PF];
-#pragma unroll
-            for (int k = 0; k < DF_PF; k++) v[k] = ld_agent(&pub[c[k]]);
-#pragma unroll
-            for (int k = 0; k < DF_PF; k++) {
-                if (((todo >> k) & 1u) && !Sentinel<T>::is(v[k])) {
-                    st_lds(&sl[nk * nr + e + k * 64], v[k]);
-                    todo &= ~(1u << k);
-                }
-            }
-            if (todo) {
-                if (++spins > a.spin_limit) {
-                    // give up: report, and let the unit finish on zeros
-                    raise_status(a.status, BAAS_TIMEOUT, c[__builtin_ctz(todo)]);
-#pragma unroll
-                    for (int k = 0; k < DF_PF; k++)
-                        if ((todo >> k) & 1u) st_lds(&sl[nk * nr + e + k * 64], T(0));
-                    todo = 0;
-                } else {
-                    __builtin_amdgcn_s_sleep(1);
-                }
-            }
-        }
-    }
+    stage_published<T>(pub, sl, cols, nk, ne, a.nrhs, lane, a.spin_limit, a.status);
 }
 
 // Exchange wave, outgoing side: publish the unit's rows that later units read

@@ -176,6 +176,13 @@ static int autotune_factor(baas_handle *h, baas_schedule *s, const double *a_val
         for (int blk : {256, 512}) cands.push_back({BAAS_PLAN_TILES, 16384, blk, 0, hb});
     for (int R : {512, 1024})
         for (int blk : {256, 512}) cands.push_back({BAAS_PLAN_BOXES, R, blk, 0, 0});
+    if (s->kind == BAAS_KIND_IC0) {  // IC0 on the lane kernel
+        for (int rpu : {256, 512, 1024})
+            for (int blk : {512, 1024}) cands.push_back({BAAS_PLAN_RANGE, rpu, blk, 2, 0});
+        for (int rpu : {16, 32, 64})
+            for (int blk : {512, 1024}) cands.push_back({BAAS_PLAN_SCHEDULE, rpu, blk, 2, 0});
+        for (int R : {512, 1000}) cands.push_back({BAAS_PLAN_BOXES, R, 1024, 2, 0});
+    }
     hipEvent_t e0, e1;
     BAAS_HIP_TRY(hipEventCreate(&e0));
     BAAS_HIP_TRY(hipEventCreate(&e1));
@@ -194,6 +201,7 @@ static int autotune_factor(baas_handle *h, baas_schedule *s, const double *a_val
             built = c;
         }
         s->fwd.block = s->bwd.block = c.block;
+        s->fwd.dataflow = s->bwd.dataflow = c.df;
         float t_best = 1e30f;
         bool ok = true;
         for (int rep = 0; rep < 3 && ok; rep++) {
@@ -217,11 +225,12 @@ static int autotune_factor(baas_handle *h, baas_schedule *s, const double *a_val
     s->opts.rows_per_unit = win.rpu;
     s->tile_band = win.mode == BAAS_PLAN_TILES ? win.band : band0;
     BAAS_TRY(build_plans_mode(s, win.mode, win.rpu));
+    s->fwd.dataflow = s->bwd.dataflow = win.df;
     if (chosen) {
         chosen[0] = win.mode;
         chosen[1] = win.rpu;
         chosen[2] = win.block;
-        chosen[3] = 0;
+        chosen[3] = win.df;
         chosen[4] = (int)(best * 1000.0f);
     }
     // pivot reports of the trial runs are the caller's to see on the real call

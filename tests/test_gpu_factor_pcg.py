@@ -39,13 +39,18 @@ def test_ic0_bit_exact(handle, name):
     s = Schedule(handle, A.rowptr, A.colidx, kind=_lib.KIND_IC0)
     av = up(A.values)
     grid = name in ("C2_lap3d_64", "C5_diff3d_48")
-    for mode, rpu, blk in [(1, 0, 256), (2, 0, 256), (2, 64, 64), (1, 1024, 512), (3, 0, 256), (4, 16384, 256),
-                           (3, 1, 1024)] + ([(5, 512, 256)] if grid else []):
-        announce("ic0", name, mode, rpu, blk)
-        s.replan(mode, rpu, blk)
-        lu = s.ic0(av)
-        _status_ok(handle, (_lib.NONPOSITIVE_PIVOT,))
-        np.testing.assert_array_equal(down(lu), want, err_msg=f"{name} {mode} {rpu} {blk}")
+    plans = [(1, 0, 256, 0), (2, 0, 256, 0), (2, 64, 64, 0), (1, 1024, 512, 0), (3, 0, 256, 0), (4, 16384, 256, 0),
+             (3, 1, 1024, 0), (1, 512, 1024, 2), (2, 32, 512, 2), (1, 4096, 128, 2), (2, 16, 1024, 2)]
+    if grid:
+        plans += [(5, 512, 256, 0), (5, 1000, 1024, 2)]
+    for mode, rpu, blk, df in plans:
+        announce("ic0", name, mode, rpu, blk, df)
+        s.replan(mode, rpu, blk, dataflow=df)
+        for rep in range(2):  # both publication halves of the lane kernel
+            lu = s.ic0(av)
+            _status_ok(handle, (_lib.NONPOSITIVE_PIVOT,))
+            np.testing.assert_array_equal(down(lu), want, err_msg=f"{name} {mode} {rpu} {blk} {df} {rep}")
+    s.replan(1, 1024, 512, dataflow=0)
     # split into L and U = L^T, FP64 and FP32
     Lf, Uf = O.ic0_split(A, want)
     lv, uv = s.ic0_split(lu)
