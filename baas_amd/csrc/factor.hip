@@ -21,7 +21,7 @@
 
 namespace baas {
 int build_ic0_plans(baas_schedule *s);
-int build_ic0_lane(baas_schedule *s);
+int build_factor_lane(baas_schedule *s);
 size_t lds_limit(baas_handle *h);
 
 struct FactorArgs {
@@ -260,17 +260,21 @@ __global__ __launch_bounds__(1024) void k_factor_levels(FactorArgs a, int nlev, 
 }
 
 // ---------------------------------------------------------------------------
-// IC0 on the lane kernel (dataflow plans).  Every lower entry and diagonal
-// that another unit reads is published into a double-buffered array indexed
-// by its lu position, the value being its own flag (as the sweep publishes
-// x).  A unit stages its rows' records, entries (A values, diagonal slots,
-// pair lists) and pairs in LDS; wavefront 0 stages the external entries the
-// unit reads as they are published; every other lane factors its own rows,
-// entry by entry, each dot over the precomputed pairs in the reference's
-// order (merge/IC.cpp:6-29, the products L_ic * L_jc ascending in c), then
-// the diagonal with the self dot in the reference build's order.  No level
-// barrier and no flag wait: an entry starts as soon as its own inputs are in
-// LDS.
+// IC0 and ILU0 on the lane kernel (dataflow plans).  Every entry that
+// another unit reads is published into a double-buffered array indexed by
+// its lu position, the value being its own flag (as the sweep publishes x).
+// A unit stages its rows' records, entries (A values, divisor slots, pair
+// lists) and pairs in LDS; wavefront 0 stages the external entries the unit
+// reads as they are published; every other lane factors its own rows, entry
+// by entry, over the precomputed pairs in the reference's order:
+//  IC0 (merge/IC.cpp:6-89): L_ik = (A_ik - sum_c L_ic L_jc) / L_jj with the
+//  products ascending in c, then the diagonal with the self dot in the
+//  reference build's order;
+//  ILU0 (merge/ILU.cpp:350-386): lu_ik1 = A_ik1 updated by
+//  fma(-L_ij, U_{col(j),c1}, .) for the row's lower entries j in ascending
+//  order, then divided by U_{c1,c1} when k1 is a lower entry.
+// No level barrier and no flag wait: an entry starts as soon as its own
+// inputs are in LDS.
 struct IclArgs {
     int nunits;
     size_t nnz;
@@ -286,6 +290,7 @@ struct IclArgs {
 
 constexpr int ICL_CH = 4;  // pairs read per poll
 
+template <int KIND>
 __device__ __forceinline__ void icl_factor_rows(const IclArgs &a, double *sl, const int4 *lrec, const double *laii,
                                                 const int *ldgp, const int4 *lent, const double *leav,
                                                 const int *lpair, double *lu, double *pub, double *nxt, int p,
@@ -296,6 +301,7 @@ __device__ __forceinline__ void icl_factor_rows(const IclArgs &a, double *sl, co
     double dp = 0.0, aii = 0.0;
     long long spins = 0;
     auto load_entry = [&]() {
+        dp = 0.0;
         if (e < ne) {
             const int4 q = lent[eo + e];
             k = q.x;
@@ -303,9 +309,9 @@ __device__ __forceinline__ void icl_factor_rows(const IclArgs &a, double *sl, co
             ps = q.z;
             np = q.w & 0x7fffffff;
             epub = q.w < 0;
+            if (KIND == BAAS_KIND_ILU0) dp = leav[eo + e];  // ILU0 updates A's value in place
         }
         pc = 0;
-        dp = 0.0;
         spins = 0;
         force = false;
     };
@@ -339,7 +345,8 @@ __device__ __forceinline__ void icl_factor_rows(const IclArgs &a, double *sl, co
 #pragma unroll
                 for (int t = 0; t < ICL_CH; t++) {
                     if (t == took && pc + t < np && (force || !Sentinel<double>::is(v[t]))) {
-                        dp = fma(o[t], Sentinel<double>::is(v[t]) ? 0.0 : v[t], dp);
+                        const double pv = Sentinel<double>::is(v[t]) ? 0.0 : v[t];
+                        dp = KIND == BAAS_KIND_IC0 ? fma(o[t], pv, dp) : fma(-o[t], pv, dp);
                         took++;
                     }
                 }
@@ -347,9 +354,16 @@ __device__ __forceinline__ void icl_factor_rows(const IclArgs &a, double *sl, co
             }
             bool done_entry = false;
             if (pc == np) {
-                const double dj = ld_lds(&sl[djs]);
+                const double dj = djs >= 0 ? ld_lds(&sl[djs]) : 1.0;
                 if (force || !Sentinel<double>::is(dj)) {
-                    const double L = (leav[eo + e] - dp) / (Sentinel<double>::is(dj) ? 1.0 : dj);
+                    const double d = Sentinel<double>::is(dj) ? 1.0 : dj;
+                    double L;
+                    if (KIND == BAAS_KIND_IC0) {
+                        L = (leav[eo + e] - dp) / d;
+                    } else {
+                        if (djs >= 0 && !(fabs(d) > 1.0e-16)) raise_status(a.status, BAAS_ZERO_PIVOT, row);
+                        L = djs >= 0 ? dp / d : dp;
+                    }
                     st_lds(&sl[base + e], L);
                     lu[k] = L;
                     if (epub) {
@@ -365,6 +379,10 @@ __device__ __forceinline__ void icl_factor_rows(const IclArgs &a, double *sl, co
                 raise_status(a.status, BAAS_TIMEOUT, row);
                 force = true;  // finish this entry with what has arrived
             }
+        } else if (KIND == BAAS_KIND_ILU0) {
+            p += stride;  // every entry of the row, diagonal included, is done
+            have = p < nk;
+            if (have) start();
         } else {
             // diagonal: the self dot in the reference build's order (blocks
             // of four rounded squares added in order when there are more
@@ -406,6 +424,7 @@ __device__ __forceinline__ void icl_factor_rows(const IclArgs &a, double *sl, co
     }
 }
 
+template <int KIND>
 __global__ __launch_bounds__(1024) void k_factor_lane(IclArgs a, double *lu, double *xp) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int s_unit;
@@ -446,8 +465,8 @@ __global__ __launch_bounds__(1024) void k_factor_lane(IclArgs a, double *lu, dou
         if (wv == 0) {
             stage_published<double>(pub, sl, a.ext + U0.w, nown, next, 1, lane, a.spin_limit, a.status);
         } else {
-            icl_factor_rows(a, sl, lrec, laii, ldgp, lent, leav, lpair, lu, pub, nxt, (wv - 1) * 64 + lane,
-                            (nw - 1) * 64, nk);
+            icl_factor_rows<KIND>(a, sl, lrec, laii, ldgp, lent, leav, lpair, lu, pub, nxt, (wv - 1) * 64 + lane,
+                                  (nw - 1) * 64, nk);
         }
         __syncthreads();
     }
@@ -531,8 +550,8 @@ static int factor(baas_handle *h, baas_schedule *s, const double *a_vals, double
     a.ctx = Ctx{h->counters, h->counters + 1, h->counters + 2};
     a.status = h->status;
     a.spin_limit = h->spin_limit;
-    if (KIND == BAAS_KIND_IC0 && p.dataflow == 2 && p.block >= 128) {
-        const int rc = build_ic0_lane(s);
+    if (p.dataflow == 2 && p.block >= 128) {
+        const int rc = build_factor_lane(s);
         if (rc != BAAS_OK && rc != BAAS_UNSUPPORTED) return rc;
         if (rc == BAAS_OK && s->icl_lds <= lds_limit(h)) {
             const size_t nnz = s->nnz_full;
@@ -557,12 +576,13 @@ static int factor(baas_handle *h, baas_schedule *s, const double *a_vals, double
             c.status = h->status;
             c.spin_limit = h->spin_limit;
             int per_cu = 0;
-            BAAS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_factor_lane, p.block, s->icl_lds));
+            BAAS_HIP_TRY(
+                hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_factor_lane<KIND>, p.block, s->icl_lds));
             if (per_cu > 0) {
                 long long grid = std::min<long long>((long long)per_cu * h->num_cus, p.nunits);
                 if (grid < 1) grid = 1;
-                hipLaunchKernelGGL(k_factor_lane, dim3((unsigned)grid), dim3(p.block), s->icl_lds, h->stream, c, lu,
-                                   reinterpret_cast<double *>(s->icl_xp.p));
+                hipLaunchKernelGGL(k_factor_lane<KIND>, dim3((unsigned)grid), dim3(p.block), s->icl_lds, h->stream, c,
+                                   lu, reinterpret_cast<double *>(s->icl_xp.p));
                 BAAS_HIP_TRY(hipGetLastError());
                 return BAAS_OK;
             }

@@ -531,18 +531,24 @@ int build_ic0_plans(baas_schedule *s) {
     return BAAS_OK;
 }
 
-// IC0 on the lane kernel (dataflow plans): for the forward plan's units,
-// the value slots of every lower entry and diagonal a unit computes or
-// reads, the entry and pair records in the reference's order (pairs are the
-// products of the up-looking sparse dot, merge/IC.cpp:6-29, as in
-// build_ic0_plans), and the entries other units read, which are published.
-// Rows with more than 255 lower entries are not supported (their own-entry
-// index is a byte): BAAS_UNSUPPORTED, and the factorisation keeps the unit
-// kernels.
-int build_ic0_lane(baas_schedule *s) {
+// IC0 and ILU0 on the lane kernel (dataflow plans): for the forward plan's
+// units, the value slots of every entry a unit computes or reads, the entry
+// and pair records in the reference's order, and the entries other units
+// read, which are published.
+//  IC0 (merge/IC.cpp:6-89): a row's entries are its lower part; entry k
+//  (column j) has the pairs (L_jc, own L_ic) of the up-looking sparse dot,
+//  c ascending, and divides by L_jj; the diagonal follows (self dot).
+//  ILU0 (merge/ILU.cpp:350-386): a row's entries are all of it; entry k1
+//  (column c1) is updated by every lower entry j < k1 whose row col(j) has
+//  c1 in its upper part, in ascending j (pairs (U_{col(j),c1}, own L_ij)),
+//  and a lower entry then divides by U_{c1,c1}.
+// The own-entry index is a byte: rows with more than 255 lower entries get
+// BAAS_UNSUPPORTED, and the factorisation keeps the unit kernels.
+int build_factor_lane(baas_schedule *s) {
     if (s->icl_gen == s->plan_gen) return s->icl_status;
     s->icl_gen = s->plan_gen;
     s->icl_status = BAAS_UNSUPPORTED;
+    const bool ilu = s->kind == BAAS_KIND_ILU0;
     const Plan &p = s->fwd;
     if (p.mode == BAAS_PLAN_LEVELS || (int)p.h_rows.size() != s->n) return s->icl_status;
     hipStream_t st = s->h->stream;
@@ -554,16 +560,19 @@ int build_ic0_lane(baas_schedule *s) {
         return BAAS_HIP_ERROR;
     const size_t nnz = A.idx.size();
     const int nunits = p.nunits;
-    std::vector<int> unit_of(n), pos_of(n), row_of_lu(nnz, -1);
+    // a row's entries: IC0 its lower part and diagonal, ILU0 all of it
+    auto row_end = [&](int r) { return ilu ? A.ptr[r + 1] : dg[r] + 1; };
+    std::vector<int> unit_of(n), row_of_lu(nnz, -1);
     for (int u = 0; u < nunits; u++)
-        for (int q = p.h_unit_ptr[u]; q < p.h_unit_ptr[u + 1]; q++) unit_of[p.h_rows[q]] = u, pos_of[p.h_rows[q]] = q;
+        for (int q = p.h_unit_ptr[u]; q < p.h_unit_ptr[u + 1]; q++) unit_of[p.h_rows[q]] = u;
     for (int r = 0; r < n; r++) {
         if (dg[r] - A.ptr[r] > 255) return s->icl_status;
-        for (int k = A.ptr[r]; k <= dg[r]; k++) row_of_lu[k] = r;
+        for (int k = A.ptr[r]; k < row_end(r); k++) row_of_lu[k] = r;
     }
     std::vector<int> rec(4 * (size_t)n), ent, pair, unit(4 * ((size_t)nunits + 1), 0), ext;
     std::vector<char> pub(nnz, 0);
     std::vector<int> own_base(n, 0), ext_slot(nnz, -1), touched;
+    std::vector<std::pair<int, int>> upd;  // ILU0: (k1, pair word), j ascending
     size_t lds = 0;
     for (int u = 0; u < nunits; u++) {
         const int q0 = p.h_unit_ptr[u], q1 = p.h_unit_ptr[u + 1];
@@ -571,7 +580,7 @@ int build_ic0_lane(baas_schedule *s) {
         for (int q = q0; q < q1; q++) {
             const int i = p.h_rows[q];
             own_base[i] = nown;
-            nown += dg[i] - A.ptr[i] + 1;
+            nown += row_end(i) - A.ptr[i];
         }
         const int ent0 = (int)ent.size() / 4, pair0 = (int)pair.size(), ext0 = (int)ext.size();
         unit[4 * (size_t)u + 0] = nown;
@@ -594,29 +603,65 @@ int build_ic0_lane(baas_schedule *s) {
             const int i = p.h_rows[q], rb = A.ptr[i], di = dg[i], ne = di - rb;
             rec[4 * (size_t)q + 0] = i;
             rec[4 * (size_t)q + 1] = (int)ent.size() / 4 - ent0;
-            rec[4 * (size_t)q + 2] = ne;
+            rec[4 * (size_t)q + 2] = ilu ? A.ptr[i + 1] - rb : ne;
             rec[4 * (size_t)q + 3] = own_base[i];
-            for (int k = rb; k < di; k++) {
-                const int j = A.idx[k];
-                const int ps = (int)pair.size() - pair0;
-                int l1 = rb, l2 = A.ptr[j];
-                const int u1 = di, u2 = dg[j];
-                while (l1 < u1 && l2 < u2) {
-                    const int c1 = A.idx[l1], c2 = A.idx[l2];
-                    if (c1 == c2) {
-                        pair.push_back((slot_of(l2) << 8) | (l1 - rb));
-                        l1++;
-                        l2++;
-                    } else if (c1 < c2) {
-                        l1++;
-                    } else {
-                        l2++;
+            if (!ilu) {
+                for (int k = rb; k < di; k++) {
+                    const int j = A.idx[k];
+                    const int ps = (int)pair.size() - pair0;
+                    int l1 = rb, l2 = A.ptr[j];
+                    const int u1 = di, u2 = dg[j];
+                    while (l1 < u1 && l2 < u2) {
+                        const int c1 = A.idx[l1], c2 = A.idx[l2];
+                        if (c1 == c2) {
+                            pair.push_back((slot_of(l2) << 8) | (l1 - rb));
+                            l1++;
+                            l2++;
+                        } else if (c1 < c2) {
+                            l1++;
+                        } else {
+                            l2++;
+                        }
+                    }
+                    ent.push_back(k);
+                    ent.push_back(slot_of(dg[j]));
+                    ent.push_back(ps);
+                    ent.push_back((int)pair.size() - pair0 - ps);
+                }
+            } else {
+                // the reference's update loop, recorded per target entry k1
+                upd.clear();
+                const int re = A.ptr[i + 1];
+                for (int j = rb; j < di; j++) {
+                    const int col = A.idx[j];
+                    int k1 = j + 1, k2 = dg[col] + 1;
+                    const int e2 = A.ptr[col + 1];
+                    while (k1 < re && k2 < e2) {
+                        const int c1 = A.idx[k1], c2 = A.idx[k2];
+                        if (c1 < c2) {
+                            k1++;
+                        } else if (c1 > c2) {
+                            k2++;
+                        } else {
+                            upd.push_back({k1, (slot_of(k2) << 8) | (j - rb)});
+                            k1++;
+                            k2++;
+                        }
                     }
                 }
-                ent.push_back(k);
-                ent.push_back(slot_of(dg[j]));
-                ent.push_back(ps);
-                ent.push_back((int)pair.size() - pair0 - ps);
+                std::stable_sort(upd.begin(), upd.end(),
+                                 [](const std::pair<int, int> &x, const std::pair<int, int> &y) {
+                                     return x.first < y.first;
+                                 });
+                size_t t = 0;
+                for (int k1 = rb; k1 < re; k1++) {
+                    const int ps = (int)pair.size() - pair0;
+                    for (; t < upd.size() && upd[t].first == k1; t++) pair.push_back(upd[t].second);
+                    ent.push_back(k1);
+                    ent.push_back(k1 < di ? slot_of(dg[A.idx[k1]]) : -1);  // lower entries divide by U_cc
+                    ent.push_back(ps);
+                    ent.push_back((int)pair.size() - pair0 - ps);
+                }
             }
         }
         const size_t nslots = nown + ext.size() - ext0;
@@ -629,11 +674,12 @@ int build_ic0_lane(baas_schedule *s) {
     unit[4 * (size_t)nunits + 1] = (int)ent.size() / 4;
     unit[4 * (size_t)nunits + 2] = (int)pair.size();
     unit[4 * (size_t)nunits + 3] = (int)ext.size();
-    // publish flags: bit 31 of the count word
+    // publish flags: bit 31 of the count words (IC0: the diagonal's on the row record)
     for (size_t e = 0; e < ent.size(); e += 4)
         if (pub[ent[e]]) ent[e + 3] |= (int)0x80000000;
-    for (int q = 0; q < n; q++)
-        if (pub[dg[rec[4 * (size_t)q]]]) rec[4 * (size_t)q + 2] |= (int)0x80000000;
+    if (!ilu)
+        for (int q = 0; q < n; q++)
+            if (pub[dg[rec[4 * (size_t)q]]]) rec[4 * (size_t)q + 2] |= (int)0x80000000;
     if (ent.empty()) ent.assign(4, 0);
     if (pair.empty()) pair.push_back(0);
     if (ext.empty()) ext.push_back(0);

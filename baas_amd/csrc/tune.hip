@@ -176,7 +176,7 @@ static int autotune_factor(baas_handle *h, baas_schedule *s, const double *a_val
         for (int blk : {256, 512}) cands.push_back({BAAS_PLAN_TILES, 16384, blk, 0, hb});
     for (int R : {512, 1024})
         for (int blk : {256, 512}) cands.push_back({BAAS_PLAN_BOXES, R, blk, 0, 0});
-    if (s->kind == BAAS_KIND_IC0) {  // IC0 on the lane kernel
+    {  // IC0 and ILU0 on the lane kernel
         for (int rpu : {256, 512, 1024})
             for (int blk : {512, 1024}) cands.push_back({BAAS_PLAN_RANGE, rpu, blk, 2, 0});
         for (int rpu : {16, 32, 64})

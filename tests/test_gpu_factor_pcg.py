@@ -77,12 +77,17 @@ def test_ilu0_bit_exact(handle, name):
     _, _, A = workload(name)
     want = O.ilu0(A)
     s = Schedule(handle, A.rowptr, A.colidx, kind=_lib.KIND_ILU0)
-    for mode, rpu in ((1, 0), (2, 0), (3, 0), (4, 0), (3, 1)) + (((5, 512),) if name != "bcsstk18" else ()):
-        announce("ilu0", name, mode, rpu)
-        s.replan(mode, rpu, 256)
-        lu = s.ilu0(up(A.values))
-        _status_ok(handle)
-        np.testing.assert_array_equal(down(lu), want)
+    plans = [(1, 0, 256, 0), (2, 0, 256, 0), (3, 0, 256, 0), (4, 0, 256, 0), (3, 1, 256, 0), (1, 512, 1024, 2),
+             (2, 32, 512, 2), (1, 4096, 128, 2)]
+    if name != "bcsstk18":
+        plans += [(5, 512, 256, 0), (5, 1000, 1024, 2)]
+    for mode, rpu, blk, df in plans:
+        announce("ilu0", name, mode, rpu, blk, df)
+        s.replan(mode, rpu, blk, dataflow=df)
+        for rep in range(2):  # both publication halves of the lane kernel
+            lu = s.ilu0(up(A.values))
+            _status_ok(handle)
+            np.testing.assert_array_equal(down(lu), want, err_msg=f"{name} {mode} {rpu} {blk} {df} {rep}")
 
 
 def test_ilu0_zero_pivot_reported(handle):
@@ -227,7 +232,7 @@ def test_factor_autotune_keeps_bits(handle, kind, name):
     av = up(A.values)
     chosen = s.autotune_factor(av)
     print("[factor autotune]", name, chosen)
-    assert chosen["mode"] in (1, 2, 3) and chosen["us"] > 0
+    assert chosen["mode"] in (1, 2, 3, 4, 5) and chosen["us"] > 0
     lu = s.ic0(av) if kind == _lib.KIND_IC0 else s.ilu0(av)
     _status_ok(handle)
     np.testing.assert_array_equal(down(lu), want)
