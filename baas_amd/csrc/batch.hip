@@ -38,8 +38,21 @@ __global__ void k_spmv_multi(int N, int k, const int *__restrict__ rp, const int
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long long)N * k) return;
     int i = (int)(t / k), r = (int)(t - (long long)i * k);
+    // ascending FMA sum, the x gathers of up to eight entries issued at once
+    const int b = rp[i], e = rp[i + 1];
     double s = 0.0;
-    for (int j = rp[i]; j < rp[i + 1]; j++) s = fma(va[j], x[(size_t)ci[j] * k + r], s);
+    for (int j0 = b; j0 < e; j0 += 8) {
+        double xv[8], av[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int j = j0 + q < e ? j0 + q : e - 1;
+            av[q] = va[j];
+            xv[q] = x[(size_t)ci[j] * k + r];
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+            if (j0 + q < e) s = fma(av[q], xv[q], s);
+    }
     y[t] = s;
 }
 

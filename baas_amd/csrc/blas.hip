@@ -15,9 +15,24 @@ namespace baas {
 constexpr int SPMV_ROWS = 256;
 constexpr int SPMV_LDS_NNZ = 4096;  // 4096 x 12 B = 48 KB
 
+// One row's ascending FMA sum, its x gathers issued eight at a time (slots
+// past the row's end re-read its last entry and are not summed), so a row
+// waits for one gather latency per eight entries instead of one per entry.
+constexpr int SPMV_CH = 8;
 __device__ __forceinline__ double spmv_row_from(const int *ci, const double *va, int b, int e, const double *x) {
     double s = 0.0;
-    for (int j = b; j < e; j++) s = fma(va[j], x[ci[j]], s);
+    for (int j0 = b; j0 < e; j0 += SPMV_CH) {
+        double xv[SPMV_CH], av[SPMV_CH];
+#pragma unroll
+        for (int t = 0; t < SPMV_CH; t++) {
+            const int j = j0 + t < e ? j0 + t : e - 1;
+            av[t] = va[j];
+            xv[t] = x[ci[j]];
+        }
+#pragma unroll
+        for (int t = 0; t < SPMV_CH; t++)
+            if (j0 + t < e) s = fma(av[t], xv[t], s);
+    }
     return s;
 }
 
