@@ -240,7 +240,9 @@ int baas_schedule_plan(const baas_schedule *s, int *plan);
  * and exchange wavefronts for parents of other units; enable 2 = the unit's
  * rows staged in LDS, every lane advancing through its own rows (one row and
  * right-hand side per item, any nrhs) on its own and publishing them itself
- * (workgroups of at least 128 threads); 0 = off. */
+ * (workgroups of at least 128 threads); 0 = off.  Enable 2 also runs IC0 and
+ * ILU0 on this schedule's units with the same design (rows with at most 255
+ * lower entries; others keep the unit kernels). */
 int baas_plan_dataflow(baas_schedule *s, int enable);
 /* Autotune the execution plan for sweeps with `uplo` and `nrhs` on these
  * values: times RANGE, SCHEDULE, LEVELS and TILES packings over their unit
