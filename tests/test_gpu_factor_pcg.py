@@ -61,6 +61,35 @@ def test_ic0_bit_exact(handle, name):
     np.testing.assert_array_equal(down(lv32), Lf.values.astype(np.float32))
 
 
+@pytest.mark.parametrize("kind,name", [(_lib.KIND_IC0, "C3_knn"), (_lib.KIND_ILU0, "C2_convdiff_64")])
+def test_factor_lane_watchdog(handle, kind, name):
+    """A one-poll spin limit makes every wait of the lane factorisation give
+    up: the kernel still finishes and reports TIMEOUT (the factor is then
+    meaningless); with the limit restored the next factorisations are exact
+    again on both publication halves."""
+    from baas_amd.api import Schedule
+    _, _, A = workload(name)
+    want = O.ic0(A) if kind == _lib.KIND_IC0 else O.ilu0(A)
+    s = Schedule(handle, A.rowptr, A.colidx, kind=kind)
+    av = up(A.values)
+    fac = s.ic0 if kind == _lib.KIND_IC0 else s.ilu0
+    for mode, rpu, blk in ((2, 16, 1024), (1, 512, 512)):
+        announce("factor watchdog", kind, name, mode, rpu, blk)
+        s.replan(mode, rpu, blk, dataflow=2)
+        handle.set_spin_limit(1)
+        try:
+            for _ in range(3):
+                fac(av)
+            st = handle.status()
+        finally:
+            handle.set_spin_limit(1 << 22)
+        assert st in (_lib.OK, _lib.TIMEOUT, _lib.NONPOSITIVE_PIVOT)
+        for _ in range(3):
+            lu = fac(av)
+            _status_ok(handle, (_lib.NONPOSITIVE_PIVOT,))
+            np.testing.assert_array_equal(down(lu), want)
+
+
 def test_ic0_nonpositive_pivot_semantics(handle):
     from baas_amd.api import Schedule
     announce("ic0 nonpositive pivot")
