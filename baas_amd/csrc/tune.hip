@@ -17,6 +17,7 @@
 
 namespace baas {
 int build_plans_mode(baas_schedule *s, int mode, int rows_per_unit);
+int build_factor_lane(baas_schedule *s);
 size_t lds_limit(baas_handle *h);
 }  // namespace baas
 
@@ -144,6 +145,9 @@ int autotune(baas_handle *h, baas_schedule *s, const T *vals, int uplo, int nrhs
     s->tile_band = win.mode == BAAS_PLAN_TILES ? win.band : band0;
     BAAS_TRY(build_plans_mode(s, win.mode, win.rpu));
     s->fwd.dataflow = s->bwd.dataflow = win.df;
+    // a factorisation on this schedule will use the lane kernel too: build
+    // its records now, as part of the analysis, not on the first call
+    if (win.df == 2 && s->kind != BAAS_KIND_TRSV) (void)build_factor_lane(s);
     if (chosen) {
         chosen[0] = win.mode;
         chosen[1] = win.rpu;
@@ -226,6 +230,7 @@ static int autotune_factor(baas_handle *h, baas_schedule *s, const double *a_val
     s->tile_band = win.mode == BAAS_PLAN_TILES ? win.band : band0;
     BAAS_TRY(build_plans_mode(s, win.mode, win.rpu));
     s->fwd.dataflow = s->bwd.dataflow = win.df;
+    if (win.df == 2) (void)build_factor_lane(s);  // the records, now rather than on the first call
     if (chosen) {
         chosen[0] = win.mode;
         chosen[1] = win.rpu;

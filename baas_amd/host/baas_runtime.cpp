@@ -193,6 +193,11 @@ void SpTRSV_BAAS::buildset() {
     dL.upload(*csrA);
     if (sched) baas_schedule_destroy(sched);
     baas_rt::check(baas_analyze(eng.h, n, dL.rowptr, dL.colidx, BAAS_KIND_TRSV, &opts, &sched), "baas_analyze");
+    // with the automatic plan mode the analysis also times the execution
+    // plans on these values and keeps the fastest (any plan gives the same
+    // bits)
+    if (opts.plan_mode == BAAS_PLAN_AUTO)
+        baas_rt::check(baas_autotune(eng.h, sched, dL.values, 0, 1, nullptr), "baas_autotune");
 }
 
 void SpTRSV_BAAS::setup() {
@@ -232,6 +237,7 @@ void IC_BAAS::buildset() {
     o.ncore = num_thread;
     if (sched) baas_schedule_destroy(sched);
     baas_rt::check(baas_analyze(eng.h, csrA->m, dA.rowptr, dA.colidx, BAAS_KIND_IC0, &o, &sched), "baas_analyze");
+    baas_rt::check(baas_autotune_factor(eng.h, sched, dA.values, nullptr), "baas_autotune_factor");
     if (!d_lu) d_lu = dalloc<double>(dA.nnz);
     analysis_time = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
@@ -287,6 +293,7 @@ void ILU_BAAS::buildset() {
     o.ncore = num_thread;
     if (sched) baas_schedule_destroy(sched);
     baas_rt::check(baas_analyze(eng.h, csrA->m, dA.rowptr, dA.colidx, BAAS_KIND_ILU0, &o, &sched), "baas_analyze");
+    baas_rt::check(baas_autotune_factor(eng.h, sched, dA.values, nullptr), "baas_autotune_factor");
     if (!d_lu) d_lu = dalloc<double>(dA.nnz);
     analysis_time = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
 }
