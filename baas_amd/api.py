@@ -127,6 +127,14 @@ class Schedule:
         L.call("baas_schedule_plan", self._s, C.cast(c, C.c_void_p))
         return {"mode": c[0], "rows_per_unit": c[1], "block": c[2], "dataflow": c[3], "band": c[4]}
 
+    def apply_plan(self, plan):
+        """Re-pack into a plan returned by plan() or autotune() (for example one
+        saved from an earlier run, so a new process skips the search)."""
+        if plan["mode"] == L.PLAN_TILES:
+            self.replan_tiles(plan["rows_per_unit"], plan.get("band") or 16, plan["block"], plan["dataflow"])
+        else:
+            self.replan(plan["mode"], plan["rows_per_unit"], plan["block"], plan["dataflow"])
+
     def autotune(self, vals, uplo=0, nrhs=1):
         """Time candidate plans on these values and keep the fastest."""
         c = (C.c_int * 5)()
