@@ -219,6 +219,77 @@ def test_box_plans_reject_unstructured(handle, name):
     np.testing.assert_array_equal(down(s.solve(up(L.values), up(b))), O.sptrsv_lower(L, b))
 
 
+def _tiny_lower(kind):
+    """Small lower triangles (diagonal last): one row, two rows, a dense 3 x 3,
+    and a 1000-row bidiagonal chain (1000 levels)."""
+    if kind == "one":
+        return g.Csr(1, np.array([0, 1], np.int32), np.array([0], np.int32), np.array([2.5]))
+    if kind == "two":
+        return g.Csr(2, np.array([0, 1, 3], np.int32), np.array([0, 0, 1], np.int32), np.array([2.0, -1.0, 3.0]))
+    if kind == "dense3":
+        return g.Csr(3, np.array([0, 1, 3, 6], np.int32), np.array([0, 0, 1, 0, 1, 2], np.int32),
+                     np.array([4.0, -1.0, 4.0, -0.5, -1.0, 4.0]))
+    n = 1000
+    rp = np.concatenate([[0, 1], 1 + 2 * np.arange(1, n)]).astype(np.int32)
+    ci = np.concatenate([[0], np.stack([np.arange(n - 1), np.arange(1, n)], 1).ravel()]).astype(np.int32)
+    va = np.concatenate([[3.0], np.tile([-1.0, 3.0], n - 1)])
+    return g.Csr(n, rp, ci, va)
+
+
+@pytest.mark.parametrize("kind", ["one", "two", "dense3", "chain"])
+def test_tiny_systems_every_plan(handle, kind):
+    from baas_amd.api import Schedule
+    L = _tiny_lower(kind)
+    s = Schedule(handle, L.rowptr, L.colidx, kind=0)
+    b = g.rhs_uniform(L.n, 9)
+    x_ref = O.sptrsv_lower(L, b)
+    U = g.Csr.from_scipy(L.to_scipy().T)
+    y_ref = O.sptrsv_upper(U, b)
+    lv, bd = up(L.values), up(b)
+    uv = s.transpose_values(lv)
+    plans = [(1, 0, 256, 0), (1, 64, 192, 1), (1, 64, 128, 2), (2, 0, 256, 0), (2, 16, 512, 2), (3, 0, 64, 0),
+             (3, 1, 256, 0), (4, 64, 256, 2)] + ([(5, 100, 256, 2)] if kind == "chain" else [])
+    for mode, rpu, blk, df in plans:
+        announce("tiny", kind, mode, rpu, blk, df)
+        if mode == 4:
+            s.replan_tiles(rpu, 4, blk, dataflow=df)
+        else:
+            s.replan(mode, rpu, blk, dataflow=df)
+        x = s.solve(lv, bd)
+        y = s.solve(uv, bd, uplo=1)
+        handle.check()
+        np.testing.assert_array_equal(down(x), x_ref, err_msg=f"fwd {kind} {mode} {rpu} {blk} {df}")
+        np.testing.assert_array_equal(down(y), y_ref, err_msg=f"bwd {kind} {mode} {rpu} {blk} {df}")
+
+
+@pytest.mark.parametrize("kind", ["one", "two", "dense3", "chain"])
+def test_tiny_factorisations(handle, kind):
+    from baas_amd import _lib
+    from baas_amd.api import Schedule
+    L = _tiny_lower(kind)
+    A = g.Csr.from_scipy((L.to_scipy() + L.to_scipy().T).tocsr())  # symmetric, diagonally dominant
+    want_ic, want_ilu = O.ic0(A), O.ilu0(A)
+    for fkind, want in ((_lib.KIND_IC0, want_ic), (_lib.KIND_ILU0, want_ilu)):
+        s = Schedule(handle, A.rowptr, A.colidx, kind=fkind)
+        av = up(A.values)
+        for mode, rpu, blk, df in ((1, 0, 256, 0), (2, 16, 512, 2), (1, 64, 1024, 2), (3, 0, 64, 0)):
+            announce("tiny factor", kind, fkind, mode, rpu, blk, df)
+            s.replan(mode, rpu, blk, dataflow=df)
+            lu = s.ic0(av) if fkind == _lib.KIND_IC0 else s.ilu0(av)
+            st = handle.status()
+            assert st in (_lib.OK, _lib.NONPOSITIVE_PIVOT)
+            np.testing.assert_array_equal(down(lu), want)
+
+
+def test_empty_system_rejected(handle):
+    from baas_amd import _lib
+    from baas_amd.api import Schedule
+    announce("empty")
+    with pytest.raises(_lib.BaasError) as e:
+        Schedule(handle, np.zeros(1, np.int32), np.zeros(0, np.int32), kind=0)
+    assert e.value.code == _lib.INVALID_ARG
+
+
 def test_plan_saved_and_reapplied(handle):
     """An autotuned plan, saved as its five numbers and applied to a fresh
     schedule of the same matrix, is the same plan and gives the same bits."""
