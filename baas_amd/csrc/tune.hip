@@ -80,7 +80,7 @@ int autotune(baas_handle *h, baas_schedule *s, const T *vals, int uplo, int nrhs
     if (nrhs == 1)
         for (int rpu : {16, 32, 64, 128})
             for (int blk : {512, 1024}) cands.push_back({BAAS_PLAN_SCHEDULE, rpu, blk, 2, 0});
-    for (int blk : {64, 128, 256, 512}) cands.push_back({BAAS_PLAN_LEVELS, 0, blk, 0, 0});
+    for (int blk : {64, 128, 256, 512, 1024}) cands.push_back({BAAS_PLAN_LEVELS, 0, blk, 0, 0});
     // grid boxes (matrices without stencil-grid structure fail to build and
     // are skipped)
     for (int R : {512, 1000, 1331}) {
