@@ -524,7 +524,7 @@ __global__ __launch_bounds__(1024) void k_sweep_lane(DfArgs a, const int *__rest
         for (int q = threadIdx.x; q < nk; q += blockDim.x) {
             const int4 m = a.meta[q0 + q];
             const int cnt = m.z, eo = eoff[q0 + q], row = m.x & 0x7fffffff;
-            for (int r = 0; r < nr; r++) lb[q * nr + r] = b[(size_t)row * nr + r];
+            if (nr == 1) lb[q] = b[row];
             lg[q] = vals[UPPER ? m.y - cnt : m.y + cnt];
             lrec[q] = make_int2(eo, cnt | (m.x & (int)0x80000000));
             lslot[q] = m.w;
@@ -533,6 +533,13 @@ __global__ __launch_bounds__(1024) void k_sweep_lane(DfArgs a, const int *__rest
                 const int j = UPPER ? m.y - e : m.y + e;
                 lev[eo + e] = vals[j];
                 les[eo + e] = a.scol[j];
+            }
+        }
+        if (nr > 1) {  // b of every (position, right-hand side), spread over all threads
+            __syncthreads();
+            for (int i = threadIdx.x; i < nk * nr; i += blockDim.x) {
+                const int q = i / nr;
+                lb[i] = b[(size_t)lrow[q] * nr + (i - q * nr)];
             }
         }
         __syncthreads();

@@ -67,9 +67,8 @@ int autotune(baas_handle *h, baas_schedule *s, const T *vals, int uplo, int nrhs
             if (rpu <= 16384 && !big) cands.push_back({BAAS_PLAN_RANGE, rpu, blk, 0, 0});
             if (nrhs == 1 && blk <= 512) cands.push_back({BAAS_PLAN_RANGE, rpu, blk, 1, 0});
         }
-    // lane kernel: units small enough for their rows to live in LDS (with
-    // several right-hand sides its units must shrink to one row and column
-    // per lane, and the unit and level-set kernels measured faster)
+    // lane kernel: units small enough for their rows (times the right-hand
+    // sides) to live in LDS
     if (nrhs == 1)
         for (int rpu : {384, 512, 768, 1024, 1536})
             for (int blk : {512, 768, 1024}) cands.push_back({BAAS_PLAN_RANGE, rpu, blk, 2, 0});
@@ -77,9 +76,12 @@ int autotune(baas_handle *h, baas_schedule *s, const T *vals, int uplo, int nrhs
         for (int blk : {256, 1024}) cands.push_back({BAAS_PLAN_SCHEDULE, rpu, blk, 0, 0});
     // small units of the merged levels on the lane kernel: every row starts
     // as soon as its own parents are published, with no level barrier
-    if (nrhs == 1)
+    if (nrhs == 1) {
         for (int rpu : {16, 32, 64, 128})
             for (int blk : {512, 1024}) cands.push_back({BAAS_PLAN_SCHEDULE, rpu, blk, 2, 0});
+    } else {
+        for (int rpu : {16, 32}) cands.push_back({BAAS_PLAN_SCHEDULE, rpu, 1024, 2, 0});
+    }
     for (int blk : {64, 128, 256, 512, 1024}) cands.push_back({BAAS_PLAN_LEVELS, 0, blk, 0, 0});
     // grid boxes (matrices without stencil-grid structure fail to build and
     // are skipped)
@@ -91,6 +93,8 @@ int autotune(baas_handle *h, baas_schedule *s, const T *vals, int uplo, int nrhs
         }
         cands.push_back({BAAS_PLAN_BOXES, R, 256, 0, 0});
     }
+    if (nrhs > 1)
+        for (int R : {64, 125, 216}) cands.push_back({BAAS_PLAN_BOXES, R, 1024, 2, 0});
     for (int S : TILE_RANGES) {
         if (S >= 2 * n && S != TILE_RANGES[0]) continue;
         for (int hb : TILE_BANDS)
