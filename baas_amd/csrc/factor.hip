@@ -297,9 +297,13 @@ __device__ __forceinline__ void icl_factor_rows(const IclArgs &a, double *sl, co
                                                 int stride, int nk) {
     bool have = p < nk;
     int row = 0, eo = 0, ne = 0, base = 0, dgp = 0, e = 0, k = 0, djs = 0, ps = 0, np = 0, pc = 0;
-    bool dpub = false, epub = false, force = false;
+    bool dpub = false, epub = false, force = false, fast = false;
     double dp = 0.0, aii = 0.0;
     long long spins = 0;
+    // IC0 rows without dot pairs: all entries (at most four) at once
+    int fk[4], fdj[4];
+    bool fpub[4];
+    double fav[4];
     auto load_entry = [&]() {
         dp = 0.0;
         if (e < ne) {
@@ -317,7 +321,8 @@ __device__ __forceinline__ void icl_factor_rows(const IclArgs &a, double *sl, co
     };
     auto start = [&]() {
         const int4 r = lrec[p];
-        row = r.x;
+        row = r.x & 0x3fffffff;
+        fast = KIND == BAAS_KIND_IC0 && (r.x & (1 << 30)) != 0;
         eo = r.y;
         ne = r.z & 0x7fffffff;
         dpub = r.z < 0;
@@ -326,12 +331,53 @@ __device__ __forceinline__ void icl_factor_rows(const IclArgs &a, double *sl, co
         dgp = ldgp[p];
         e = 0;
         load_entry();
+        if (fast) {
+#pragma unroll
+            for (int t = 0; t < 4; t++) {
+                const int4 q = lent[eo + (t < ne ? t : ne - 1)];  // marked rows have ne >= 1
+                fk[t] = q.x;
+                fdj[t] = q.y;
+                fpub[t] = q.w < 0;
+                fav[t] = leav[eo + (t < ne ? t : ne - 1)];
+            }
+        }
     };
     if (have) start();
-    // one loop with a wavefront-uniform exit and a single divergent step
+    // one loop with a wavefront-uniform exit and a single divergent step;
+    // every path of a step ends at the one back edge (a second one lets the
+    // compiler nest loops, and lanes parked outside an inner spin loop can
+    // starve the lanes spinning in it)
     while (__ballot(have) != 0) {
         if (!have) continue;
-        if (e < ne) {
+        if (fast && e < ne) {
+            // every entry needs only its parent's diagonal: read them all,
+            // and when all are there factor the entries in order
+            double d[4];
+#pragma unroll
+            for (int t = 0; t < 4; t++) d[t] = ld_lds(&sl[fdj[t]]);
+            bool ready = true;
+#pragma unroll
+            for (int t = 0; t < 4; t++)
+                if (t < ne && Sentinel<double>::is(d[t])) ready = false;
+            if (ready || force) {
+#pragma unroll
+                for (int t = 0; t < 4; t++) {
+                    if (t < ne) {
+                        const double L = (fav[t] - 0.0) / (Sentinel<double>::is(d[t]) ? 1.0 : d[t]);
+                        st_lds(&sl[base + t], L);
+                        lu[fk[t]] = L;
+                        if (fpub[t]) {
+                            st_agent(&pub[fk[t]], L);
+                            nxt[fk[t]] = Sentinel<double>::value();
+                        }
+                    }
+                }
+                e = ne;  // the diagonal follows in the next step
+            } else if (!force && ++spins > a.spin_limit) {
+                raise_status(a.status, BAAS_TIMEOUT, row);
+                force = true;
+            }
+        } else if (e < ne) {
             if (pc < np) {
                 double v[ICL_CH], o[ICL_CH];
 #pragma unroll
@@ -450,7 +496,7 @@ __global__ __launch_bounds__(1024) void k_factor_lane(IclArgs a, double *lu, dou
         for (int i = threadIdx.x; i < nown + next; i += blockDim.x) st_lds(&sl[i], Sentinel<double>::value());
         for (int q = threadIdx.x; q < nk; q += blockDim.x) {
             const int4 r = a.rec[q0 + q];
-            const int dg = a.diagptr[r.x];
+            const int dg = a.diagptr[r.x & 0x3fffffff];  // bit 30 marks rows without dot pairs
             lrec[q] = r;
             ldgp[q] = dg;
             laii[q] = lu[dg];

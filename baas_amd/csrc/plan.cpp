@@ -565,6 +565,7 @@ int build_factor_lane(baas_schedule *s) {
     std::vector<int> unit_of(n), row_of_lu(nnz, -1);
     for (int u = 0; u < nunits; u++)
         for (int q = p.h_unit_ptr[u]; q < p.h_unit_ptr[u + 1]; q++) unit_of[p.h_rows[q]] = u;
+    if (n >= (1 << 30)) return s->icl_status;  // bit 30 of the row word is a flag
     for (int r = 0; r < n; r++) {
         if (dg[r] - A.ptr[r] > 255) return s->icl_status;
         for (int k = A.ptr[r]; k < row_end(r); k++) row_of_lu[k] = r;
@@ -680,6 +681,18 @@ int build_factor_lane(baas_schedule *s) {
     if (!ilu)
         for (int q = 0; q < n; q++)
             if (pub[dg[rec[4 * (size_t)q]]]) rec[4 * (size_t)q + 2] |= (int)0x80000000;
+    // IC0 rows of one to four lower entries and no dot pairs (stencil rows)
+    // are marked in bit 30 of the row word: their entries need only the
+    // parents' diagonals and are factored together
+    if (!ilu)
+        for (int u = 0; u < nunits; u++)
+            for (int q = p.h_unit_ptr[u]; q < p.h_unit_ptr[u + 1]; q++) {
+                const int ne = rec[4 * (size_t)q + 2] & 0x7fffffff;
+                const int e0 = unit[4 * (size_t)u + 1] + rec[4 * (size_t)q + 1];
+                bool nopairs = ne >= 1 && ne <= 4;
+                for (int e = 0; e < ne && nopairs; e++) nopairs = (ent[4 * ((size_t)e0 + e) + 3] & 0x7fffffff) == 0;
+                if (nopairs) rec[4 * (size_t)q + 0] |= 1 << 30;
+            }
     if (ent.empty()) ent.assign(4, 0);
     if (pair.empty()) pair.push_back(0);
     if (ext.empty()) ext.push_back(0);
