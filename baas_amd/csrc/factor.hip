@@ -300,8 +300,10 @@ __device__ __forceinline__ void icl_factor_rows(const IclArgs &a, double *sl, co
     bool dpub = false, epub = false, force = false, fast = false;
     double dp = 0.0, aii = 0.0;
     long long spins = 0;
-    // IC0 rows without dot pairs: all entries (at most four) at once
-    int fk[4], fdj[4];
+    // rows marked by the host (stencil rows): their nfl lower entries (at
+    // most four) take no updates and are factored together; an ILU0 row's
+    // upper entries take none either, so they are A's values from the start
+    int nfl = 0, fk[4], fdj[4];
     bool fpub[4];
     double fav[4];
     auto load_entry = [&]() {
@@ -322,7 +324,7 @@ __device__ __forceinline__ void icl_factor_rows(const IclArgs &a, double *sl, co
     auto start = [&]() {
         const int4 r = lrec[p];
         row = r.x & 0x3fffffff;
-        fast = KIND == BAAS_KIND_IC0 && (r.x & (1 << 30)) != 0;
+        fast = (r.x & (1 << 30)) != 0;
         eo = r.y;
         ne = r.z & 0x7fffffff;
         dpub = r.z < 0;
@@ -332,13 +334,29 @@ __device__ __forceinline__ void icl_factor_rows(const IclArgs &a, double *sl, co
         e = 0;
         load_entry();
         if (fast) {
+            // IC0: every entry is lower; ILU0: the lower entries precede the
+            // diagonal, whose lu position is dgp (the first entry's is A's row start)
+            nfl = KIND == BAAS_KIND_IC0 ? ne : dgp - lent[eo].x;
+            const int last = nfl > 0 ? nfl - 1 : 0;  // a row has at least one entry
 #pragma unroll
             for (int t = 0; t < 4; t++) {
-                const int4 q = lent[eo + (t < ne ? t : ne - 1)];  // marked rows have ne >= 1
+                const int4 q = lent[eo + (t < nfl ? t : last)];
                 fk[t] = q.x;
                 fdj[t] = q.y;
                 fpub[t] = q.w < 0;
-                fav[t] = leav[eo + (t < ne ? t : ne - 1)];
+                fav[t] = leav[eo + (t < nfl ? t : last)];
+            }
+            if (KIND == BAAS_KIND_ILU0) {
+                for (int t = nfl + 1; t < ne; t++) {
+                    const int4 q = lent[eo + t];
+                    const double L = leav[eo + t];
+                    st_lds(&sl[base + t], L);
+                    lu[q.x] = L;
+                    if (q.w < 0) {
+                        st_agent(&pub[q.x], L);
+                        nxt[q.x] = Sentinel<double>::value();
+                    }
+                }
             }
         }
     };
@@ -349,21 +367,25 @@ __device__ __forceinline__ void icl_factor_rows(const IclArgs &a, double *sl, co
     // starve the lanes spinning in it)
     while (__ballot(have) != 0) {
         if (!have) continue;
-        if (fast && e < ne) {
-            // every entry needs only its parent's diagonal: read them all,
-            // and when all are there factor the entries in order
+        if (fast && e < nfl) {
+            // every lower entry needs only its pivot: read them all, and when
+            // all are there factor the entries in order
             double d[4];
 #pragma unroll
             for (int t = 0; t < 4; t++) d[t] = ld_lds(&sl[fdj[t]]);
             bool ready = true;
 #pragma unroll
             for (int t = 0; t < 4; t++)
-                if (t < ne && Sentinel<double>::is(d[t])) ready = false;
+                if (t < nfl && Sentinel<double>::is(d[t])) ready = false;
             if (ready || force) {
 #pragma unroll
                 for (int t = 0; t < 4; t++) {
-                    if (t < ne) {
-                        const double L = (fav[t] - 0.0) / (Sentinel<double>::is(d[t]) ? 1.0 : d[t]);
+                    if (t < nfl) {
+                        const double dd = Sentinel<double>::is(d[t]) ? 1.0 : d[t];
+                        if (KIND == BAAS_KIND_ILU0 && !(fabs(dd) > 1.0e-16))
+                            raise_status(a.status, BAAS_ZERO_PIVOT, row);
+                        // IC0 subtracts a zero dot, ILU0 divides A's value
+                        const double L = (KIND == BAAS_KIND_IC0 ? fav[t] - 0.0 : fav[t]) / dd;
                         st_lds(&sl[base + t], L);
                         lu[fk[t]] = L;
                         if (fpub[t]) {
@@ -372,7 +394,8 @@ __device__ __forceinline__ void icl_factor_rows(const IclArgs &a, double *sl, co
                         }
                     }
                 }
-                e = ne;  // the diagonal follows in the next step
+                e = nfl;  // the diagonal follows in the next step
+                load_entry();
             } else if (!force && ++spins > a.spin_limit) {
                 raise_status(a.status, BAAS_TIMEOUT, row);
                 force = true;
@@ -416,7 +439,8 @@ __device__ __forceinline__ void icl_factor_rows(const IclArgs &a, double *sl, co
                         st_agent(&pub[k], L);
                         nxt[k] = Sentinel<double>::value();
                     }
-                    e++;
+                    // a marked ILU0 row comes here for its diagonal only
+                    e = KIND == BAAS_KIND_ILU0 && fast ? ne : e + 1;
                     load_entry();
                     done_entry = true;
                 }
