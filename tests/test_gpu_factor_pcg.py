@@ -90,11 +90,19 @@ def test_factor_lane_watchdog(handle, kind, name):
             np.testing.assert_array_equal(down(lu), want)
 
 
-def test_ic0_nonpositive_pivot_semantics(handle):
+# the default plan, a unit kernel, and the lane kernel on ranges and on small
+# SCHEDULE units (rows of at most four lower entries take its one-step path)
+PIVOT_PLANS = [None, (1, 0, 256, 0), (1, 64, 1024, 2), (2, 16, 512, 2)]
+
+
+@pytest.mark.parametrize("plan", PIVOT_PLANS)
+def test_ic0_nonpositive_pivot_semantics(handle, plan):
     from baas_amd.api import Schedule
-    announce("ic0 nonpositive pivot")
+    announce("ic0 nonpositive pivot", plan)
     A = g.Csr(2, np.array([0, 2, 4], np.int32), np.array([0, 1, 0, 1], np.int32), np.array([1.0, 2.0, 2.0, 1.0]))
     s = Schedule(handle, A.rowptr, A.colidx, kind=_lib.KIND_IC0)
+    if plan:
+        s.replan(*plan[:3], dataflow=plan[3])
     lu = s.ic0(up(A.values))
     assert handle.status() == _lib.NONPOSITIVE_PIVOT
     np.testing.assert_array_equal(down(lu), O.ic0(A))
@@ -119,13 +127,40 @@ def test_ilu0_bit_exact(handle, name):
             np.testing.assert_array_equal(down(lu), want, err_msg=f"{name} {mode} {rpu} {blk} {df} {rep}")
 
 
-def test_ilu0_zero_pivot_reported(handle):
+@pytest.mark.parametrize("plan", PIVOT_PLANS)
+def test_ilu0_zero_pivot_reported(handle, plan):
     from baas_amd.api import Schedule
-    announce("ilu0 zero pivot")
+    announce("ilu0 zero pivot", plan)
     A = g.Csr(2, np.array([0, 2, 4], np.int32), np.array([0, 1, 0, 1], np.int32), np.array([0.0, 1.0, 1.0, 1.0]))
     s = Schedule(handle, A.rowptr, A.colidx, kind=_lib.KIND_ILU0)
-    s.ilu0(up(A.values))
+    if plan:
+        s.replan(*plan[:3], dataflow=plan[3])
+    lu = s.ilu0(up(A.values))
     assert handle.status() == _lib.ZERO_PIVOT
+    with np.errstate(all="ignore"):
+        np.testing.assert_array_equal(down(lu), O.ilu0(A))
+
+
+def test_ilu0_zero_pivot_on_stencil_rows(handle):
+    """A zero first pivot on a 12^3 convection-diffusion matrix: the rows
+    below it divide by zero in the lane kernel's one-step path for stencil
+    rows; the status says so and the infinities that follow are the
+    oracle's, bit for bit."""
+    from baas_amd.api import Schedule
+    A = g.convdiff3d(12)
+    v = A.values.copy()
+    v[[j for j in range(A.rowptr[0], A.rowptr[1]) if A.colidx[j] == 0][0]] = 0.0
+    A = g.Csr(A.n, A.rowptr, A.colidx, v)
+    with np.errstate(all="ignore"):
+        want = O.ilu0(A)
+    assert np.isinf(want).any()
+    s = Schedule(handle, A.rowptr, A.colidx, kind=_lib.KIND_ILU0)
+    for mode, rpu, blk, df in ((1, 0, 256, 0), (1, 512, 1024, 2), (5, 512, 1024, 2), (2, 16, 512, 2)):
+        announce("ilu0 zero pivot stencil", mode, rpu, blk, df)
+        s.replan(mode, rpu, blk, dataflow=df)
+        lu = s.ilu0(up(A.values))
+        assert handle.status() == _lib.ZERO_PIVOT
+        np.testing.assert_array_equal(down(lu), want, err_msg=f"{mode} {rpu} {blk} {df}")
 
 
 def test_spmv_and_blas(handle):
