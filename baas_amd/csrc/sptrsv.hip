@@ -674,11 +674,15 @@ static int launch_df(baas_handle *h, const Plan &p, const DfArgs &a, const T *va
 template <class T, bool UP>
 static int launch_lane(baas_handle *h, const Plan &p, const DfArgs &a, const T *vals, const T *b, T *x, T *xp,
                        size_t lds, hipStream_t stream) {
-    // Chunk width: one chunk per row while rows have at most eight parents
-    // (four or eight parents per poll); two for matrices with longer rows,
-    // whose lanes then make more but cheaper steps (measured faster than
-    // four, eight or twelve, and than one, on rows of up to 20-40 parents).
-    auto kern = p.max_deg <= 4   ? k_sweep_lane<T, UP, 4>
+    // Chunk width: one chunk per row, as narrow as the longest row allows,
+    // while rows have at most eight parents (every polled slot costs each
+    // step); two for matrices with longer rows, whose lanes then make more
+    // but cheaper steps (measured faster than four, eight or twelve, and
+    // than one, on rows of up to 20-40 parents).
+    auto kern = p.max_deg <= 2   ? k_sweep_lane<T, UP, 2>
+                : p.max_deg <= 3 ? k_sweep_lane<T, UP, 3>
+                : p.max_deg <= 4 ? k_sweep_lane<T, UP, 4>
+                : p.max_deg <= 6 ? k_sweep_lane<T, UP, 6>
                 : p.max_deg <= 8 ? k_sweep_lane<T, UP, 8>
                                  : k_sweep_lane<T, UP, 2>;
     int per_cu = 0;
