@@ -236,6 +236,54 @@ def _tiny_lower(kind):
     return g.Csr(n, rp, ci, va)
 
 
+def _banded_lower(n, width, full, seed):
+    """Lower triangle whose rows have up to `width` parents among the 2*width
+    rows above them: all of the nearest `width` (full) or a random subset
+    of each row's window (ragged, so chunks are partly padded)."""
+    rng = np.random.default_rng(seed)
+    rp, ci, va = [0], [], []
+    for i in range(n):
+        lo = max(0, i - 2 * width)
+        if full:
+            cols = list(range(max(0, i - width), i))
+        else:
+            pool = np.arange(lo, i)
+            k = min(len(pool), int(rng.integers(0, width + 1)))
+            cols = sorted(rng.choice(pool, size=k, replace=False).tolist()) if k else []
+        ci += cols + [i]
+        va += list(rng.uniform(-1.0, 1.0, len(cols))) + [float(width) + 1.0 + rng.uniform(0.0, 1.0)]
+        rp.append(len(ci))
+    return g.Csr(n, np.array(rp, np.int32), np.array(ci, np.int32), np.array(va))
+
+
+@pytest.mark.parametrize("width", [1, 2, 3, 4, 5, 6, 7, 8, 9, 13])
+def test_lane_chunk_widths(handle, width):
+    """The lane kernel polls a row's parents in chunks sized from the longest
+    row (2, 3, 4, 6 or 8 parents, two for longer rows): every width, full
+    and ragged rows, forward and backward, FP64 and FP32, bit-exact."""
+    import torch
+    from baas_amd.api import Schedule
+    for full in (True, False):
+        L = _banded_lower(3000, width, full, seed=width)
+        s = Schedule(handle, L.rowptr, L.colidx, kind=0)
+        b = g.rhs_uniform(L.n, 4)
+        U = g.Csr.from_scipy(L.to_scipy().T)
+        x_ref, y_ref = O.sptrsv_lower(L, b), O.sptrsv_upper(U, b)
+        lv, bd = up(L.values), up(b)
+        uv = s.transpose_values(lv)
+        for mode, rpu, blk in ((1, 512, 1024), (2, 32, 512)):
+            announce("chunk width", width, full, mode, rpu, blk)
+            s.replan(mode, rpu, blk, dataflow=2)
+            x = s.solve(lv, bd)
+            y = s.solve(uv, bd, uplo=1)
+            handle.check()
+            np.testing.assert_array_equal(down(x), x_ref, err_msg=f"fwd {width} {full} {mode}")
+            np.testing.assert_array_equal(down(y), y_ref, err_msg=f"bwd {width} {full} {mode}")
+        x32 = down(s.solve(up(L.values, torch.float32), up(b, torch.float32)))
+        handle.check()
+        np.testing.assert_array_equal(x32, O.sptrsv_lower_f32(L, b))
+
+
 @pytest.mark.parametrize("kind", ["one", "two", "dense3", "chain"])
 def test_tiny_systems_every_plan(handle, kind):
     from baas_amd.api import Schedule
