@@ -290,7 +290,7 @@ struct IclArgs {
 
 constexpr int ICL_CH = 4;  // pairs read per poll
 
-template <int KIND>
+template <int KIND, int FW>
 __device__ __forceinline__ void icl_factor_rows(const IclArgs &a, double *sl, const int4 *lrec, const double *laii,
                                                 const int *ldgp, const int4 *lent, const double *leav,
                                                 const int *lpair, double *lu, double *pub, double *nxt, int p,
@@ -303,9 +303,9 @@ __device__ __forceinline__ void icl_factor_rows(const IclArgs &a, double *sl, co
     // rows marked by the host (stencil rows): their nfl lower entries (at
     // most four) take no updates and are factored together; an ILU0 row's
     // upper entries take none either, so they are A's values from the start
-    int nfl = 0, fk[4], fdj[4];
-    bool fpub[4];
-    double fav[4];
+    int nfl = 0, fk[FW], fdj[FW];
+    bool fpub[FW];
+    double fav[FW];
     auto load_entry = [&]() {
         dp = 0.0;
         if (e < ne) {
@@ -339,7 +339,7 @@ __device__ __forceinline__ void icl_factor_rows(const IclArgs &a, double *sl, co
             nfl = KIND == BAAS_KIND_IC0 ? ne : dgp - lent[eo].x;
             const int last = nfl > 0 ? nfl - 1 : 0;  // a row has at least one entry
 #pragma unroll
-            for (int t = 0; t < 4; t++) {
+            for (int t = 0; t < FW; t++) {
                 const int4 q = lent[eo + (t < nfl ? t : last)];
                 fk[t] = q.x;
                 fdj[t] = q.y;
@@ -370,16 +370,16 @@ __device__ __forceinline__ void icl_factor_rows(const IclArgs &a, double *sl, co
         if (fast && e < nfl) {
             // every lower entry needs only its pivot: read them all, and when
             // all are there factor the entries in order
-            double d[4];
+            double d[FW];
 #pragma unroll
-            for (int t = 0; t < 4; t++) d[t] = ld_lds(&sl[fdj[t]]);
+            for (int t = 0; t < FW; t++) d[t] = ld_lds(&sl[fdj[t]]);
             bool ready = true;
 #pragma unroll
-            for (int t = 0; t < 4; t++)
+            for (int t = 0; t < FW; t++)
                 if (t < nfl && Sentinel<double>::is(d[t])) ready = false;
             if (ready || force) {
 #pragma unroll
-                for (int t = 0; t < 4; t++) {
+                for (int t = 0; t < FW; t++) {
                     if (t < nfl) {
                         const double dd = Sentinel<double>::is(d[t]) ? 1.0 : d[t];
                         if (KIND == BAAS_KIND_ILU0 && !(fabs(dd) > 1.0e-16))
@@ -494,7 +494,7 @@ __device__ __forceinline__ void icl_factor_rows(const IclArgs &a, double *sl, co
     }
 }
 
-template <int KIND>
+template <int KIND, int FW>
 __global__ __launch_bounds__(1024) void k_factor_lane(IclArgs a, double *lu, double *xp) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int s_unit;
@@ -535,7 +535,7 @@ __global__ __launch_bounds__(1024) void k_factor_lane(IclArgs a, double *lu, dou
         if (wv == 0) {
             stage_published<double>(pub, sl, a.ext + U0.w, nown, next, 1, lane, a.spin_limit, a.status);
         } else {
-            icl_factor_rows<KIND>(a, sl, lrec, laii, ldgp, lent, leav, lpair, lu, pub, nxt, (wv - 1) * 64 + lane,
+            icl_factor_rows<KIND, FW>(a, sl, lrec, laii, ldgp, lent, leav, lpair, lu, pub, nxt, (wv - 1) * 64 + lane,
                                   (nw - 1) * 64, nk);
         }
         __syncthreads();
@@ -645,13 +645,14 @@ static int factor(baas_handle *h, baas_schedule *s, const double *a_vals, double
             c.ctx = Ctx{h->counters, h->counters + 1, h->counters + 2};
             c.status = h->status;
             c.spin_limit = h->spin_limit;
+            // marked rows have at most icl_fw lower entries (2 or 3)
+            auto kern = s->icl_fw == 2 ? k_factor_lane<KIND, 2> : k_factor_lane<KIND, 3>;
             int per_cu = 0;
-            BAAS_HIP_TRY(
-                hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_factor_lane<KIND>, p.block, s->icl_lds));
+            BAAS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, p.block, s->icl_lds));
             if (per_cu > 0) {
                 long long grid = std::min<long long>((long long)per_cu * h->num_cus, p.nunits);
                 if (grid < 1) grid = 1;
-                hipLaunchKernelGGL(k_factor_lane<KIND>, dim3((unsigned)grid), dim3(p.block), s->icl_lds, h->stream, c,
+                hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(p.block), s->icl_lds, h->stream, c,
                                    lu, reinterpret_cast<double *>(s->icl_xp.p));
                 BAAS_HIP_TRY(hipGetLastError());
                 return BAAS_OK;

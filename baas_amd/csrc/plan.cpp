@@ -681,20 +681,30 @@ int build_factor_lane(baas_schedule *s) {
     if (!ilu)
         for (int q = 0; q < n; q++)
             if (pub[dg[rec[4 * (size_t)q]]]) rec[4 * (size_t)q + 2] |= (int)0x80000000;
-    // rows of at most four lower entries whose entries other than the
+    // rows of at most three lower entries whose entries other than the
     // diagonal take no pairs (stencil rows) are marked in bit 30 of the row
     // word: their lower entries need only their pivots and are factored
-    // together (IC0 also needs at least one entry: its row is all lower)
+    // together (IC0 also needs at least one entry: its row is all lower).
+    // The kernel reads icl_fw pivots per step: 2 when no marked row has
+    // three lower entries (5-point stencils), else 3.
+    std::vector<char> mark(n, 0);
+    int fw = 2;
     for (int u = 0; u < nunits; u++)
         for (int q = p.h_unit_ptr[u]; q < p.h_unit_ptr[u + 1]; q++) {
             const int i = rec[4 * (size_t)q + 0], ne = rec[4 * (size_t)q + 2] & 0x7fffffff;
             const int nl = dg[i] - A.ptr[i];
             const int e0 = unit[4 * (size_t)u + 1] + rec[4 * (size_t)q + 1];
-            bool nopairs = nl <= 4 && (ilu || ne >= 1);
+            bool nopairs = nl <= 3 && (ilu || ne >= 1);
             for (int e = 0; e < ne && nopairs; e++)
                 if (!ilu || e != nl) nopairs = (ent[4 * ((size_t)e0 + e) + 3] & 0x7fffffff) == 0;
-            if (nopairs) rec[4 * (size_t)q + 0] |= 1 << 30;
+            if (nopairs) {
+                mark[q] = 1;
+                fw = std::max(fw, nl);
+            }
         }
+    for (int q = 0; q < n; q++)
+        if (mark[q]) rec[4 * (size_t)q + 0] |= 1 << 30;
+    s->icl_fw = fw;
     if (ent.empty()) ent.assign(4, 0);
     if (pair.empty()) pair.push_back(0);
     if (ext.empty()) ext.push_back(0);

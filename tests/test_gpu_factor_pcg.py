@@ -90,6 +90,27 @@ def test_factor_lane_watchdog(handle, kind, name):
             np.testing.assert_array_equal(down(lu), want)
 
 
+def test_factor_five_point_rows(handle):
+    """IC0 and ILU0 of a 128^2 5-point Laplacian (and of a non-symmetric
+    5-point operator for ILU0) on lane plans: every row takes the one-step
+    path for stencil rows with two pivots per step; bit-exact."""
+    from baas_amd.api import Schedule
+    A = g.laplace2d(128)
+    rng = np.random.default_rng(11)
+    N = g.Csr(A.n, A.rowptr, A.colidx, A.values * (1.0 + 0.1 * rng.uniform(-1.0, 1.0, A.values.size)))
+    for kind, M in ((_lib.KIND_IC0, A), (_lib.KIND_ILU0, A), (_lib.KIND_ILU0, N)):
+        want = O.ic0(M) if kind == _lib.KIND_IC0 else O.ilu0(M)
+        s = Schedule(handle, M.rowptr, M.colidx, kind=kind)
+        fac = s.ic0 if kind == _lib.KIND_IC0 else s.ilu0
+        for mode, rpu, blk, df in ((1, 512, 1024, 2), (2, 16, 512, 2), (5, 256, 1024, 2), (1, 0, 256, 0)):
+            announce("five-point factor", kind, mode, rpu, blk, df)
+            s.replan(mode, rpu, blk, dataflow=df)
+            for rep in range(2):
+                lu = fac(up(M.values))
+                _status_ok(handle)
+                np.testing.assert_array_equal(down(lu), want, err_msg=f"{kind} {mode} {rpu} {blk} {df} {rep}")
+
+
 # the default plan, a unit kernel, and the lane kernel on ranges and on small
 # SCHEDULE units (rows of at most four lower entries take its one-step path)
 PIVOT_PLANS = [None, (1, 0, 256, 0), (1, 64, 1024, 2), (2, 16, 512, 2)]
