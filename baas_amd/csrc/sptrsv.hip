@@ -847,8 +847,12 @@ static int launch_levels(baas_handle *h, baas_schedule *s, const Plan &p, int up
     a.skip = skip;
     a.status = h->status;
     a.spin_limit = h->spin_limit;
-    auto kern = p.max_deg <= 4 ? (uplo ? k_sweep_levels<T, true, 4> : k_sweep_levels<T, false, 4>)
-                               : (uplo ? k_sweep_levels<T, true, 8> : k_sweep_levels<T, false, 8>);
+    // parents gathered per chunk: as many as the longest row has, up to
+    // four (padded slots would be loaded for nothing), else eight at a time
+    auto kern = p.max_deg <= 2   ? (uplo ? k_sweep_levels<T, true, 2> : k_sweep_levels<T, false, 2>)
+                : p.max_deg <= 3 ? (uplo ? k_sweep_levels<T, true, 3> : k_sweep_levels<T, false, 3>)
+                : p.max_deg <= 4 ? (uplo ? k_sweep_levels<T, true, 4> : k_sweep_levels<T, false, 4>)
+                                 : (uplo ? k_sweep_levels<T, true, 8> : k_sweep_levels<T, false, 8>);
     int per_cu = 0;
     BAAS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, p.block, 0));
     if (per_cu <= 0) return BAAS_UNSUPPORTED;
