@@ -30,7 +30,9 @@ struct Chunk {
 
 __device__ __forceinline__ bool stopped(const int *alldone) { return alldone && ld_agent(alldone); }
 
-// y[i,r] = (A x)[i,r] for all right-hand sides (ascending FMA sum per row)
+// y[i,r] = (A x)[i,r] for all right-hand sides (ascending FMA sum per row),
+// the x gathers of up to CH entries issued at once
+template <int CH>
 __global__ void k_spmv_multi(int N, int k, const int *__restrict__ rp, const int *__restrict__ ci,
                              const double *__restrict__ va, const double *__restrict__ x, double *__restrict__ y,
                              const int *alldone) {
@@ -38,22 +40,36 @@ __global__ void k_spmv_multi(int N, int k, const int *__restrict__ rp, const int
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long long)N * k) return;
     int i = (int)(t / k), r = (int)(t - (long long)i * k);
-    // ascending FMA sum, the x gathers of up to eight entries issued at once
     const int b = rp[i], e = rp[i + 1];
     double s = 0.0;
-    for (int j0 = b; j0 < e; j0 += 8) {
-        double xv[8], av[8];
+    for (int j0 = b; j0 < e; j0 += CH) {
+        double xv[CH], av[CH];
 #pragma unroll
-        for (int q = 0; q < 8; q++) {
+        for (int q = 0; q < CH; q++) {
             const int j = j0 + q < e ? j0 + q : e - 1;
             av[q] = va[j];
             xv[q] = x[(size_t)ci[j] * k + r];
         }
 #pragma unroll
-        for (int q = 0; q < 8; q++)
+        for (int q = 0; q < CH; q++)
             if (j0 + q < e) s = fma(av[q], xv[q], s);
     }
     y[t] = s;
+}
+
+// the symmetric pattern's longest row has 2 * (longest lower row) + 1
+// entries: 5 for a 5-point stencil, 7 for a 7-point one
+static void launch_spmv_multi(int max_lower, hipStream_t st, int N, int k, const int *rp, const int *ci,
+                              const double *va, const double *x, double *y, const int *alldone) {
+    const int w = 2 * max_lower + 1;
+    const long long M = (long long)N * k;
+    const dim3 grid((unsigned)((M + 255) / 256)), block(256);
+    if (w <= 5)
+        hipLaunchKernelGGL(k_spmv_multi<5>, grid, block, 0, st, N, k, rp, ci, va, x, y, alldone);
+    else if (w <= 7)
+        hipLaunchKernelGGL(k_spmv_multi<7>, grid, block, 0, st, N, k, rp, ci, va, x, y, alldone);
+    else
+        hipLaunchKernelGGL(k_spmv_multi<8>, grid, block, 0, st, N, k, rp, ci, va, x, y, alldone);
 }
 
 // per chunk and right-hand side: sum over the chunk's rows of u*v (fixed order)
@@ -308,8 +324,7 @@ static int batch_iteration(baas_batch *B, int cur) {
     const long long M = (long long)N * k;
     int *ctl = B->ctl.p;
     double *rz_old = cur ? B->rz1.p : B->rz0.p, *rz_new = cur ? B->rz0.p : B->rz1.p;
-    hipLaunchKernelGGL(k_spmv_multi, dim3(nblk(M)), dim3(256), 0, B->st, N, k, B->rowptr, B->colidx, B->a_vals,
-                       B->p.p, B->Ap.p, ctl);
+    launch_spmv_multi(B->s->fwd.max_deg, B->st, N, k, B->rowptr, B->colidx, B->a_vals, B->p.p, B->Ap.p, ctl);
     BAAS_TRY(chunk_dot_sys(B, B->p.p, B->Ap.p, B->pap.p, 0, ctl));
     hipLaunchKernelGGL(k_bupdate_xr, dim3(nblk(M)), dim3(256), 0, B->st, N, k, B->sys_of.p, B->x.p, B->r.p, B->p.p,
                        B->Ap.p, rz_old, B->pap.p, B->coldone.p, B->r32.p, ctl);
@@ -458,8 +473,7 @@ extern "C" int baas_batch_pcg(baas_batch *B, const double *Bm, double *X, double
     BAAS_HIP_TRY(hipMemcpyAsync(B->tol.p, &tol, sizeof(double), hipMemcpyHostToDevice, B->st));
     // r = b - A x, per-column ||r0||
     hipLaunchKernelGGL(k_bcopy, dim3(nblk(M)), dim3(256), 0, B->st, M, X, B->x.p);
-    hipLaunchKernelGGL(k_spmv_multi, dim3(nblk(M)), dim3(256), 0, B->st, N, k, B->rowptr, B->colidx, B->a_vals, B->x.p,
-                       B->Ap.p, nullptr);
+    launch_spmv_multi(B->s->fwd.max_deg, B->st, N, k, B->rowptr, B->colidx, B->a_vals, B->x.p, B->Ap.p, nullptr);
     hipLaunchKernelGGL(k_bres, dim3(nblk(M)), dim3(256), 0, B->st, M, Bm, B->Ap.p, B->r.p, B->r32.p);
     BAAS_TRY(chunk_dot_sys(B, B->r.p, B->r.p, B->norm0.p, 1, nullptr));
     hipLaunchKernelGGL(k_binit, dim3(nblk(ncol)), dim3(256), 0, B->st, ncol, maxiter, B->coldone.p, B->coliter.p,
