@@ -301,7 +301,7 @@ __device__ __forceinline__ void icl_factor_rows(const IclArgs &a, double *sl, co
     double dp = 0.0, aii = 0.0;
     long long spins = 0;
     // rows marked by the host (stencil rows): their nfl lower entries (at
-    // most four) take no updates and are factored together; an ILU0 row's
+    // most FW) take no updates and are factored together; an ILU0 row's
     // upper entries take none either, so they are A's values from the start
     int nfl = 0, fk[FW], fdj[FW];
     bool fpub[FW];
