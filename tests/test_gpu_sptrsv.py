@@ -259,8 +259,10 @@ def _banded_lower(n, width, full, seed):
 @pytest.mark.parametrize("width", [1, 2, 3, 4, 5, 6, 7, 8, 9, 13])
 def test_lane_chunk_widths(handle, width):
     """The lane kernel polls a row's parents in chunks sized from the longest
-    row (2, 3, 4, 6 or 8 parents, two for longer rows): every width, full
-    and ragged rows, forward and backward, FP64 and FP32, bit-exact."""
+    row (2, 3, 4, 6 or 8 parents, two for longer rows), the level-set kernel
+    gathers them in chunks of 2, 3, 4 or 8: every width, full and ragged
+    rows, forward and backward, FP64 and FP32, one and eight right-hand
+    sides, bit-exact."""
     import torch
     from baas_amd.api import Schedule
     for full in (True, False):
@@ -271,9 +273,9 @@ def test_lane_chunk_widths(handle, width):
         x_ref, y_ref = O.sptrsv_lower(L, b), O.sptrsv_upper(U, b)
         lv, bd = up(L.values), up(b)
         uv = s.transpose_values(lv)
-        for mode, rpu, blk in ((1, 512, 1024), (2, 32, 512)):
+        for mode, rpu, blk in ((1, 512, 1024), (2, 32, 512), (3, 0, 256)):
             announce("chunk width", width, full, mode, rpu, blk)
-            s.replan(mode, rpu, blk, dataflow=2)
+            s.replan(mode, rpu, blk, dataflow=2 if mode != 3 else 0)
             x = s.solve(lv, bd)
             y = s.solve(uv, bd, uplo=1)
             handle.check()
@@ -282,6 +284,12 @@ def test_lane_chunk_widths(handle, width):
         x32 = down(s.solve(up(L.values, torch.float32), up(b, torch.float32)))
         handle.check()
         np.testing.assert_array_equal(x32, O.sptrsv_lower_f32(L, b))
+        B = np.random.default_rng(width).uniform(-1.0, 1.0, (L.n, 8))
+        for mode, rpu, blk, df in ((3, 0, 256, 0), (2, 32, 1024, 2)):
+            s.replan(mode, rpu, blk, dataflow=df)
+            X = down(s.solve(lv, up(B)))
+            handle.check()
+            np.testing.assert_array_equal(X, O.sptrsv_lower_multi(L, B), err_msg=f"8 rhs {width} {full} {mode}")
 
 
 @pytest.mark.parametrize("kind", ["one", "two", "dense3", "chain"])
