@@ -649,6 +649,8 @@ static int factor(baas_handle *h, baas_schedule *s, const double *a_vals, double
             auto kern = s->icl_fw == 2 ? k_factor_lane<KIND, 2> : k_factor_lane<KIND, 3>;
             int per_cu = 0;
             BAAS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, p.block, s->icl_lds));
+            // at most 16 wavefronts per CU, as for the lane sweep
+            per_cu = std::min(per_cu, std::max(1, 1024 / p.block));
             if (per_cu > 0) {
                 long long grid = std::min<long long>((long long)per_cu * h->num_cus, p.nunits);
                 if (grid < 1) grid = 1;

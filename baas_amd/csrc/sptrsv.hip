@@ -416,8 +416,9 @@ __device__ __forceinline__ void lane_chunk(const T *lev, const int *les, int eo,
 }
 
 // One lane's items: item it is position it / nr of the unit, right-hand side
-// it % nr; the lane takes it, it + stride, ... in one flat loop.
-template <class T, bool UPPER, int CH>
+// it % nr; the lane takes it, it + stride, ... in one flat loop.  ONE: every
+// row's parents fit in one chunk (the multi-chunk steps are compiled out).
+template <class T, bool UPPER, int CH, bool ONE>
 __device__ __forceinline__ void lane_solve(const DfArgs &a, T *sl, const T *lb, const T *lg, const int2 *lrec,
                                            const int *lslot, const int *lrow, const T *lev, const int *les, T *pub,
                                            T *nxt, int it, int stride, int nk) {
@@ -469,7 +470,7 @@ __device__ __forceinline__ void lane_solve(const DfArgs &a, T *sl, const T *lb, 
         }
         if (pc == nc) {
             k += nc;
-            if (k < cnt) {  // rows longer than one chunk
+            if (!ONE && k < cnt) {  // rows longer than one chunk
                 nc = cnt - k < CH ? cnt - k : CH;
                 pc = 0;
                 lane_chunk<T, UPPER, CH>(lev, les, eo, k, nc, nr, r, cs, cv);
@@ -492,7 +493,7 @@ __device__ __forceinline__ void lane_solve(const DfArgs &a, T *sl, const T *lb, 
     }
 }
 
-template <class T, bool UPPER, int CH>
+template <class T, bool UPPER, int CH, bool ONE>
 __global__ __launch_bounds__(1024) void k_sweep_lane(DfArgs a, const int *__restrict__ eoff,
                                                     const int *__restrict__ eptr, const T *__restrict__ vals,
                                                     const T *__restrict__ b, T *x, T *xp) {
@@ -546,7 +547,7 @@ __global__ __launch_bounds__(1024) void k_sweep_lane(DfArgs a, const int *__rest
         if (wv == 0) {
             df_incoming<T>(a, pub, sl, a.ext_cols + e0, nk, ne, lane);
         } else {
-            lane_solve<T, UPPER, CH>(a, sl, lb, lg, lrec, lslot, lrow, lev, les, pub, nxt, (wv - 1) * 64 + lane,
+            lane_solve<T, UPPER, CH, ONE>(a, sl, lb, lg, lrec, lslot, lrow, lev, les, pub, nxt, (wv - 1) * 64 + lane,
                                      (nw - 1) * 64, nk);
         }
         __syncthreads();
@@ -679,15 +680,19 @@ static int launch_lane(baas_handle *h, const Plan &p, const DfArgs &a, const T *
     // step); two for matrices with longer rows, whose lanes then make more
     // but cheaper steps (measured faster than four, eight or twelve, and
     // than one, on rows of up to 20-40 parents).
-    auto kern = p.max_deg <= 2   ? k_sweep_lane<T, UP, 2>
-                : p.max_deg <= 3 ? k_sweep_lane<T, UP, 3>
-                : p.max_deg <= 4 ? k_sweep_lane<T, UP, 4>
-                : p.max_deg <= 6 ? k_sweep_lane<T, UP, 6>
-                : p.max_deg <= 8 ? k_sweep_lane<T, UP, 8>
-                                 : k_sweep_lane<T, UP, 2>;
+    auto kern = p.max_deg <= 2   ? k_sweep_lane<T, UP, 2, true>
+                : p.max_deg <= 3 ? k_sweep_lane<T, UP, 3, true>
+                : p.max_deg <= 4 ? k_sweep_lane<T, UP, 4, true>
+                : p.max_deg <= 6 ? k_sweep_lane<T, UP, 6, true>
+                : p.max_deg <= 8 ? k_sweep_lane<T, UP, 8, true>
+                                 : k_sweep_lane<T, UP, 2, false>;
     int per_cu = 0;
     BAAS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, p.block, lds));
     if (per_cu <= 0) return BAAS_UNSUPPORTED;
+    // at most 16 wavefronts per CU: more resident units share the SIMDs'
+    // issue slots and slow every poll loop (whatever the register count
+    // would allow)
+    per_cu = std::min(per_cu, std::max(1, 1024 / p.block));
     long long grid = (long long)per_cu * h->num_cus;
     if (grid > p.nunits) grid = p.nunits;
     if (grid < 1) grid = 1;
