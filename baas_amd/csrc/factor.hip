@@ -290,7 +290,9 @@ struct IclArgs {
 
 constexpr int ICL_CH = 4;  // pairs read per poll
 
-template <int KIND, int FW>
+// ALLF: every row with lower entries is marked (IC0 on a stencil), so the
+// entry-by-entry path with pair lists is compiled out.
+template <int KIND, int FW, bool ALLF>
 __device__ __forceinline__ void icl_factor_rows(const IclArgs &a, double *sl, const int4 *lrec, const double *laii,
                                                 const int *ldgp, const int4 *lent, const double *leav,
                                                 const int *lpair, double *lu, double *pub, double *nxt, int p,
@@ -400,7 +402,7 @@ __device__ __forceinline__ void icl_factor_rows(const IclArgs &a, double *sl, co
                 raise_status(a.status, BAAS_TIMEOUT, row);
                 force = true;
             }
-        } else if (e < ne) {
+        } else if (!ALLF && e < ne) {
             if (pc < np) {
                 double v[ICL_CH], o[ICL_CH];
 #pragma unroll
@@ -494,7 +496,7 @@ __device__ __forceinline__ void icl_factor_rows(const IclArgs &a, double *sl, co
     }
 }
 
-template <int KIND, int FW>
+template <int KIND, int FW, bool ALLF>
 __global__ __launch_bounds__(1024) void k_factor_lane(IclArgs a, double *lu, double *xp) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int s_unit;
@@ -535,7 +537,7 @@ __global__ __launch_bounds__(1024) void k_factor_lane(IclArgs a, double *lu, dou
         if (wv == 0) {
             stage_published<double>(pub, sl, a.ext + U0.w, nown, next, 1, lane, a.spin_limit, a.status);
         } else {
-            icl_factor_rows<KIND, FW>(a, sl, lrec, laii, ldgp, lent, leav, lpair, lu, pub, nxt, (wv - 1) * 64 + lane,
+            icl_factor_rows<KIND, FW, ALLF>(a, sl, lrec, laii, ldgp, lent, leav, lpair, lu, pub, nxt, (wv - 1) * 64 + lane,
                                   (nw - 1) * 64, nk);
         }
         __syncthreads();
@@ -646,7 +648,9 @@ static int factor(baas_handle *h, baas_schedule *s, const double *a_vals, double
             c.status = h->status;
             c.spin_limit = h->spin_limit;
             // marked rows have at most icl_fw lower entries (2 or 3)
-            auto kern = s->icl_fw == 2 ? k_factor_lane<KIND, 2> : k_factor_lane<KIND, 3>;
+            auto kern = s->icl_fw == 2 ? k_factor_lane<KIND, 2, false> : k_factor_lane<KIND, 3, false>;
+            if constexpr (KIND == BAAS_KIND_IC0)
+                if (s->icl_allfast) kern = s->icl_fw == 2 ? k_factor_lane<KIND, 2, true> : k_factor_lane<KIND, 3, true>;
             int per_cu = 0;
             BAAS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, p.block, s->icl_lds));
             // at most 16 wavefronts per CU, as for the lane sweep

@@ -689,6 +689,7 @@ int build_factor_lane(baas_schedule *s) {
     // three lower entries (5-point stencils), else 3.
     std::vector<char> mark(n, 0);
     int fw = 2;
+    bool allfast = !ilu;
     for (int u = 0; u < nunits; u++)
         for (int q = p.h_unit_ptr[u]; q < p.h_unit_ptr[u + 1]; q++) {
             const int i = rec[4 * (size_t)q + 0], ne = rec[4 * (size_t)q + 2] & 0x7fffffff;
@@ -700,11 +701,14 @@ int build_factor_lane(baas_schedule *s) {
             if (nopairs) {
                 mark[q] = 1;
                 fw = std::max(fw, nl);
+            } else if (ne > 0) {
+                allfast = false;
             }
         }
     for (int q = 0; q < n; q++)
         if (mark[q]) rec[4 * (size_t)q + 0] |= 1 << 30;
     s->icl_fw = fw;
+    s->icl_allfast = allfast;
     if (ent.empty()) ent.assign(4, 0);
     if (pair.empty()) pair.push_back(0);
     if (ext.empty()) ext.push_back(0);
