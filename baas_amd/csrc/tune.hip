@@ -84,8 +84,9 @@ int autotune(baas_handle *h, baas_schedule *s, const T *vals, int uplo, int nrhs
     }
     for (int blk : {64, 128, 256, 512, 1024}) cands.push_back({BAAS_PLAN_LEVELS, 0, blk, 0, 0});
     // grid boxes (matrices without stencil-grid structure fail to build and
-    // are skipped)
-    for (int R : {512, 1000, 1331}) {
+    // are skipped); 900 rows (10 x 10 x 9 on a 3D grid) leave no lane of a
+    // 1024-thread workgroup with a second row
+    for (int R : {512, 900, 1000, 1331}) {
         if (nrhs == 1) {
             cands.push_back({BAAS_PLAN_BOXES, R, 1024, 2, 0});
             cands.push_back({BAAS_PLAN_BOXES, R, 512, 2, 0});
