@@ -191,7 +191,8 @@ def test_box_plans_bit_exact(handle, name):
     y_ref = O.sptrsv_upper(U, b)
     lv, bd = up(L.values), up(b)
     uv = s.transpose_values(lv)
-    for rpu, blk, df in [(512, 1024, 2), (512, 512, 2), (1024, 384, 1), (512, 256, 0), (64, 256, 2), (100, 128, 2)]:
+    for rpu, blk, df in [(512, 1024, 2), (512, 512, 2), (1024, 384, 1), (512, 256, 0), (64, 256, 2), (100, 128, 2),
+                         (900, 1024, 2), (1000, 1024, 2)]:
         announce("boxes", name, rpu, blk, df)
         s.replan(5, rpu, blk, dataflow=df)
         x = s.solve(lv, bd)
