@@ -290,13 +290,16 @@ struct IclArgs {
 
 constexpr int ICL_CH = 4;  // pairs read per poll
 
-// ALLF: every row with lower entries is marked (IC0 on a stencil), so the
-// entry-by-entry path with pair lists is compiled out.
+// ALLF: every row with lower entries is marked (a stencil).  IC0 then has no
+// entry-by-entry path; ILU0 uses it for diagonals only (no pivot, at most FW
+// updates, one per lower entry), so its pair chunk is FW wide.
 template <int KIND, int FW, bool ALLF>
 __device__ __forceinline__ void icl_factor_rows(const IclArgs &a, double *sl, const int4 *lrec, const double *laii,
                                                 const int *ldgp, const int4 *lent, const double *leav,
                                                 const int *lpair, double *lu, double *pub, double *nxt, int p,
                                                 int stride, int nk) {
+    constexpr bool DIAG_ONLY = KIND == BAAS_KIND_ILU0 && ALLF;
+    constexpr int PCH = DIAG_ONLY ? FW : ICL_CH;
     bool have = p < nk;
     int row = 0, eo = 0, ne = 0, base = 0, dgp = 0, e = 0, k = 0, djs = 0, ps = 0, np = 0, pc = 0;
     bool dpub = false, epub = false, force = false, fast = false;
@@ -402,11 +405,11 @@ __device__ __forceinline__ void icl_factor_rows(const IclArgs &a, double *sl, co
                 raise_status(a.status, BAAS_TIMEOUT, row);
                 force = true;
             }
-        } else if (!ALLF && e < ne) {
+        } else if ((!ALLF || KIND == BAAS_KIND_ILU0) && e < ne) {
             if (pc < np) {
-                double v[ICL_CH], o[ICL_CH];
+                double v[PCH], o[PCH];
 #pragma unroll
-                for (int t = 0; t < ICL_CH; t++) {
+                for (int t = 0; t < PCH; t++) {
                     const int i = ps + (pc + t < np ? pc + t : np - 1);
                     const int w = lpair[i];
                     v[t] = ld_lds(&sl[w >> 8]);
@@ -414,7 +417,7 @@ __device__ __forceinline__ void icl_factor_rows(const IclArgs &a, double *sl, co
                 }
                 int took = 0;
 #pragma unroll
-                for (int t = 0; t < ICL_CH; t++) {
+                for (int t = 0; t < PCH; t++) {
                     if (t == took && pc + t < np && (force || !Sentinel<double>::is(v[t]))) {
                         const double pv = Sentinel<double>::is(v[t]) ? 0.0 : v[t];
                         dp = KIND == BAAS_KIND_IC0 ? fma(o[t], pv, dp) : fma(-o[t], pv, dp);
@@ -425,15 +428,16 @@ __device__ __forceinline__ void icl_factor_rows(const IclArgs &a, double *sl, co
             }
             bool done_entry = false;
             if (pc == np) {
-                const double dj = djs >= 0 ? ld_lds(&sl[djs]) : 1.0;
+                const double dj = !DIAG_ONLY && djs >= 0 ? ld_lds(&sl[djs]) : 1.0;
                 if (force || !Sentinel<double>::is(dj)) {
                     const double d = Sentinel<double>::is(dj) ? 1.0 : dj;
                     double L;
                     if (KIND == BAAS_KIND_IC0) {
                         L = (leav[eo + e] - dp) / d;
                     } else {
-                        if (djs >= 0 && !(fabs(d) > 1.0e-16)) raise_status(a.status, BAAS_ZERO_PIVOT, row);
-                        L = djs >= 0 ? dp / d : dp;
+                        if (!DIAG_ONLY && djs >= 0 && !(fabs(d) > 1.0e-16))
+                            raise_status(a.status, BAAS_ZERO_PIVOT, row);
+                        L = !DIAG_ONLY && djs >= 0 ? dp / d : dp;
                     }
                     st_lds(&sl[base + e], L);
                     lu[k] = L;
@@ -649,8 +653,7 @@ static int factor(baas_handle *h, baas_schedule *s, const double *a_vals, double
             c.spin_limit = h->spin_limit;
             // marked rows have at most icl_fw lower entries (2 or 3)
             auto kern = s->icl_fw == 2 ? k_factor_lane<KIND, 2, false> : k_factor_lane<KIND, 3, false>;
-            if constexpr (KIND == BAAS_KIND_IC0)
-                if (s->icl_allfast) kern = s->icl_fw == 2 ? k_factor_lane<KIND, 2, true> : k_factor_lane<KIND, 3, true>;
+            if (s->icl_allfast) kern = s->icl_fw == 2 ? k_factor_lane<KIND, 2, true> : k_factor_lane<KIND, 3, true>;
             int per_cu = 0;
             BAAS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, p.block, s->icl_lds));
             // at most 16 wavefronts per CU, as for the lane sweep

@@ -221,7 +221,7 @@ struct baas_schedule {
     bool icl_armed = false;  // icl_xp holds sentinels
     size_t icl_lds = 0;
     int icl_fw = 3;  // lower entries of the rows marked for the one-step path (2 or 3)
-    bool icl_allfast = false;  // IC0: every row with lower entries is marked
+    bool icl_allfast = false;  // every row with lower entries is marked
     double t_gpu = 0, t_host = 0;
     baas_opts opts{};
 };

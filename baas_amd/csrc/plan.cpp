@@ -689,7 +689,7 @@ int build_factor_lane(baas_schedule *s) {
     // three lower entries (5-point stencils), else 3.
     std::vector<char> mark(n, 0);
     int fw = 2;
-    bool allfast = !ilu;
+    bool allfast = true;  // IC0: unmarked rows have no lower entries; ILU0: none unmarked
     for (int u = 0; u < nunits; u++)
         for (int q = p.h_unit_ptr[u]; q < p.h_unit_ptr[u + 1]; q++) {
             const int i = rec[4 * (size_t)q + 0], ne = rec[4 * (size_t)q + 2] & 0x7fffffff;
