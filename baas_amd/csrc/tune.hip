@@ -77,7 +77,7 @@ int autotune(baas_handle *h, baas_schedule *s, const T *vals, int uplo, int nrhs
     // small units of the merged levels on the lane kernel: every row starts
     // as soon as its own parents are published, with no level barrier
     if (nrhs == 1) {
-        for (int rpu : {16, 32, 40, 64, 128})
+        for (int rpu : {16, 32, 40, 64, 96, 128})
             for (int blk : {512, 1024}) cands.push_back({BAAS_PLAN_SCHEDULE, rpu, blk, 2, 0});
     } else {
         for (int rpu : {16, 32}) cands.push_back({BAAS_PLAN_SCHEDULE, rpu, 1024, 2, 0});
