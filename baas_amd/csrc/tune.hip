@@ -190,7 +190,7 @@ static int autotune_factor(baas_handle *h, baas_schedule *s, const double *a_val
             for (int blk : {512, 1024}) cands.push_back({BAAS_PLAN_RANGE, rpu, blk, 2, 0});
         for (int rpu : {16, 32, 64})
             for (int blk : {512, 1024}) cands.push_back({BAAS_PLAN_SCHEDULE, rpu, blk, 2, 0});
-        for (int R : {512, 900, 1000}) cands.push_back({BAAS_PLAN_BOXES, R, 1024, 2, 0});
+        for (int R : {512, 576, 900, 1000}) cands.push_back({BAAS_PLAN_BOXES, R, 1024, 2, 0});
     }
     hipEvent_t e0, e1;
     BAAS_HIP_TRY(hipEventCreate(&e0));
