@@ -47,6 +47,14 @@ def trisolve(vals, b, sched: Schedule):
 class ICCGSolve(torch.autograd.Function):
     @staticmethod
     def forward(ctx, a_vals, b, solver: PCG, tol: float, maxiter: int):
+        # The PCG handle factorises and multiplies with the buffer it was
+        # created on; any other tensor would be ignored by the forward pass
+        # while still receiving the gradient, so it is refused.
+        sa = solver.a_vals
+        if (not isinstance(a_vals, torch.Tensor) or a_vals.dtype != sa.dtype or a_vals.numel() != sa.numel()
+                or a_vals.device != sa.device or a_vals.data_ptr() != sa.data_ptr()):
+            raise ValueError("iccg_solve: a_vals must be the tensor the PCG solver was created on "
+                             "(same device buffer); create a PCG on this tensor instead")
         x, k, rel = solver.solve(b, tol=tol, maxiter=maxiter, refactor=True)
         ctx.solver, ctx.tol, ctx.maxiter = solver, tol, maxiter
         ctx.save_for_backward(a_vals, x)

@@ -5,7 +5,9 @@ the ``cpu_baseline`` leg of ``bench.py`` -- never by the product package.
 
 * ``lib``  -- ``oracle/liboracle.so``, the C restatement (oracle.c).
 * ``ref``  -- ``oracle/_ref/libbaas_ref.so``, the reference's own sources
-  compiled in place (None where it was not built).
+  compiled in place (None where it was not built).  It is loaded on first
+  use only (``get_ref()`` or the attribute ``ref``), so the GPU-side tests,
+  which never ask for it, do not load it.
 """
 from __future__ import annotations
 
@@ -47,7 +49,8 @@ _lib_path = os.path.join(HERE, "liboracle.so")
 if not os.path.exists(_lib_path):
     build()
 lib = _load(_lib_path)
-ref = _load(os.path.join(HERE, "_ref", "libbaas_ref.so"))
+_ref_lib = None
+_ref_tried = False
 
 _sig(lib, "orc_two_hop_reduce", _i, [_i, _I, _I, _I, _I])
 _sig(lib, "orc_levelset_csc", _i, [_i, _I, _I, _I, _I, _I])
@@ -70,16 +73,33 @@ _sig(lib, "orc_dcsrmv", None, [_i, _I, _I, _D, _D, _D, _d, _d])
 _sig(lib, "orc_dot", _d, [_i, _D, _D])
 _sig(lib, "orc_waxpby", None, [_i, _D, _d, _D, _d, _D])
 _sig(lib, "orc_pcg", _i, [_i, _I, _I, _I, _D, _D, _D, _i, _d, _i, C.c_void_p, _i])
-if ref is not None:
-    _sig(ref, "ref_two_hop", _i, [_i, _I, _I, _I, _I])
-    _sig(ref, "ref_levelset_rows", _i, [_i, _I, _I, _I, _I, _I])
-    _sig(ref, "ref_baas_schedule", None, [_i, _i, _I, _I, _I, _i, _I, _I, _I, _I, _I, _I, _I, _I])
-    _sig(ref, "ref_ic0", None, [_i, _I, _I, _D, _D])
-    _sig(ref, "ref_ilu0", None, [_i, _I, _I, _D, _D])
-    _sig(ref, "ref_spmv", None, [_i, _I, _I, _D, _D, _D])
-    _sig(ref, "ref_dcsrmv", None, [_i, _I, _I, _D, _D, _D, _d, _d])
-    _sig(ref, "ref_dot", _d, [_i, _D, _D])
-    _sig(ref, "ref_waxpby", None, [_i, _D, _d, _D, _d, _D])
+
+
+def get_ref():
+    """The compiled reference (oracle/_ref/libbaas_ref.so), or None where it was not built."""
+    global _ref_lib, _ref_tried
+    if _ref_tried:
+        return _ref_lib
+    _ref_tried = True
+    r = _load(os.path.join(HERE, "_ref", "libbaas_ref.so"))
+    if r is not None:
+        _sig(r, "ref_two_hop", _i, [_i, _I, _I, _I, _I])
+        _sig(r, "ref_levelset_rows", _i, [_i, _I, _I, _I, _I, _I])
+        _sig(r, "ref_baas_schedule", None, [_i, _i, _I, _I, _I, _i, _I, _I, _I, _I, _I, _I, _I, _I])
+        _sig(r, "ref_ic0", None, [_i, _I, _I, _D, _D])
+        _sig(r, "ref_ilu0", None, [_i, _I, _I, _D, _D])
+        _sig(r, "ref_spmv", None, [_i, _I, _I, _D, _D, _D])
+        _sig(r, "ref_dcsrmv", None, [_i, _I, _I, _D, _D, _D, _d, _d])
+        _sig(r, "ref_dot", _d, [_i, _D, _D])
+        _sig(r, "ref_waxpby", None, [_i, _D, _d, _D, _d, _D])
+    _ref_lib = r
+    return r
+
+
+def __getattr__(name):
+    if name == "ref":
+        return get_ref()
+    raise AttributeError(name)
 
 
 def _i32(a):
@@ -102,7 +122,7 @@ def csc_of(n, rowptr, colidx):
 def two_hop(n, ptr, st, use_ref=False):
     op = np.zeros(n + 1, np.int32)
     os_ = np.zeros(max(int(ptr[-1]), 1), np.int32)
-    f = ref.ref_two_hop if use_ref else lib.orc_two_hop_reduce
+    f = get_ref().ref_two_hop if use_ref else lib.orc_two_hop_reduce
     e = f(n, _i32(ptr), _i32(st), op, os_)
     return op, os_[:e]
 
@@ -120,7 +140,7 @@ def levelset_rows(n, lrowptr, lcolidx, use_ref=False):
         lp = np.zeros(n + 1, np.int32)
         ntl = np.zeros(n, np.int32)
         pto = np.zeros(n, np.int32)
-        nl = ref.ref_levelset_rows(n, _i32(lrowptr), _i32(lcolidx), lp, ntl, pto)
+        nl = get_ref().ref_levelset_rows(n, _i32(lrowptr), _i32(lcolidx), lp, ntl, pto)
         return lp[: nl + 1], ntl, pto
     cp, ri = csc_of(n, lrowptr, lcolidx)
     return levelset_csc(n, cp, ri)
@@ -138,7 +158,7 @@ def baas_schedule(kind, n, lrowptr, lcolidx, thr_rowptr=None, ncore=8, use_ref=F
     nt = np.zeros(n, np.int32)
     mp = np.zeros(n + 1, np.int32)
     cnt = np.zeros(4, np.int32)
-    f = ref.ref_baas_schedule if use_ref else lib.orc_baas_schedule
+    f = get_ref().ref_baas_schedule if use_ref else lib.orc_baas_schedule
     f(kind, n, _i32(lrowptr), _i32(lcolidx), _i32(thr_rowptr), ncore, gp, gs, lp, pto, ntl, nt, mp, cnt)
     ng, nl, nm, red = (int(v) for v in cnt)
     return dict(ngroups=ng, nlevels=nl, nmerged=nm, reduced_edges=red,
@@ -196,7 +216,7 @@ def sptrsv_lower_levels_omp(L, b, level_ptr, perm_to_orig, nthreads=0):
 def ic0(A, use_ref=False):
     lu = A.values.copy()
     if use_ref:
-        ref.ref_ic0(A.n, _i32(A.rowptr), _i32(A.colidx), _f64(A.values), lu)
+        get_ref().ref_ic0(A.n, _i32(A.rowptr), _i32(A.colidx), _f64(A.values), lu)
     else:
         lib.orc_ic0(A.n, _i32(A.rowptr), _i32(A.colidx), _i32(A.diagptr()), _f64(A.values), lu)
     return lu
@@ -205,7 +225,7 @@ def ic0(A, use_ref=False):
 def ilu0(A, use_ref=False):
     lu = A.values.copy()
     if use_ref:
-        ref.ref_ilu0(A.n, _i32(A.rowptr), _i32(A.colidx), _f64(A.values), lu)
+        get_ref().ref_ilu0(A.n, _i32(A.rowptr), _i32(A.colidx), _f64(A.values), lu)
     else:
         lib.orc_ilu0(A.n, _i32(A.rowptr), _i32(A.colidx), _i32(A.diagptr()), lu)
     return lu
@@ -224,9 +244,17 @@ def ic0_split(A, lu):
 
 def spmv(A, x, use_ref=False):
     y = np.zeros(A.n)
-    f = ref.ref_spmv if use_ref else lib.orc_spmv
+    f = get_ref().ref_spmv if use_ref else lib.orc_spmv
     f(A.n, _i32(A.rowptr), _i32(A.colidx), _f64(A.values), _f64(x), y)
     return y
+
+
+def dcsrmv(A, x, y, alpha, beta, use_ref=False):
+    """alpha * (A x) + beta * y (y is not modified; the result is returned)."""
+    out = _f64(y).copy()
+    f = get_ref().ref_dcsrmv if use_ref else lib.orc_dcsrmv
+    f(A.n, _i32(A.rowptr), _i32(A.colidx), _f64(A.values), _f64(x), out, float(alpha), float(beta))
+    return out
 
 
 def dot(x, y):
@@ -235,7 +263,7 @@ def dot(x, y):
 
 def waxpby(alpha, x, beta, y, use_ref=False):
     w = np.zeros(len(x))
-    f = ref.ref_waxpby if use_ref else lib.orc_waxpby
+    f = get_ref().ref_waxpby if use_ref else lib.orc_waxpby
     f(len(x), w, alpha, _f64(x), beta, _f64(y))
     return w
 

@@ -425,7 +425,10 @@ void orc_sptrsv_upper_desc(int n, const int *rowptr, const int *colidx, const do
 }
 
 /* FP32 forms used by the mixed-precision preconditioner (no reference
- * counterpart; same operation order as the FP64 forms).  Pinned: M. */
+ * counterpart; same operation order as the FP64 forms).  Pinned: bit-equal
+ * to an independent numpy float32 sweep with an exactly rounded fma (itself
+ * checked against rational arithmetic), Higham's componentwise backward-error
+ * bound for substitution, and the x = 1 manufactured solution. */
 void orc_sptrsv_lower_f32(int n, const int *rowptr, const int *colidx, const float *val,
                           const float *b, float *x) {
     for (int i = 0; i < n; i++) {
@@ -547,7 +550,8 @@ void orc_ilu0(int n, const int *rowptr, const int *colidx, const int *diagptr, d
 
 /* Split an IC0 result into L (CSR lower, diag last) and U = L^T (CSR upper,
  * diag first, pattern of A's upper part): test/iccg_level_merge.cpp:809-891.
- * Pinned: M (U^T == L), R. */
+ * Pinned: M (U equals scipy's transpose of L in pattern and bits; L L^T = A
+ * on the pattern). */
 void orc_ic0_split(int n, const int *rowptr, const int *colidx, const int *diagptr,
                    const double *lu, int *Lp, int *Li, double *Lx, int *Up, int *Ui, double *Ux) {
     Lp[0] = 0; Up[0] = 0;
@@ -579,7 +583,8 @@ void orc_spmv(int m, const int *rowptr, const int *colidx, const double *val,
     }
 }
 /* y = alpha*(A x) + beta*y: blas/spmv.cpp:29-45 (contracted as fma(alpha,s,beta*y)).
- * Pinned: M, R. */
+ * Pinned: G (golden vectors from the compiled reference's dcsrmv), alpha=1,
+ * beta=0 equal to SpMV bit for bit, R (live, where oracle/_ref is built). */
 void orc_dcsrmv(int m, const int *rowptr, const int *colidx, const double *val,
                 const double *x, double *y, double alpha, double beta) {
     for (int i = 0; i < m; i++) {

@@ -13,7 +13,7 @@ Reference outputs recorded (all from oracle/_ref/libbaas_ref.so):
   level set, threads per level, level merge) for every workload and kind.
 * row-level level sets (DAG_levelSet_CSC).
 * IC0 (spic0_csr_uL_serial) and ILU0 (ilu0csr_uplooking_ref) factors.
-* SpMV / waxpby on small vectors.
+* SpMV / dcsrmv / waxpby on small vectors.
 * Matrix Market loader output for bcsstk18 (the reference's data file).
 The reference's triangular solves (merge/SpTRSV.cpp) do not build here
 (papi.h is missing), so no solve outputs are recorded.
@@ -41,8 +41,8 @@ def sha(a) -> str:
 
 
 def workloads():
-    from workloads import NAMES, workload
-    return {name: workload(name) for name in NAMES}
+    from workloads import NAMES, NONSYMMETRIC, workload
+    return {name: workload(name) for name in NAMES}, NONSYMMETRIC
 
 
 def main():
@@ -51,7 +51,7 @@ def main():
         raise SystemExit("oracle/_ref was not built (reference tree absent)")
     manifest = {"generator": "tests/golden/make_golden.py", "schedules": {}, "levels": {},
                 "ic0": {}, "ilu0": {}, "inputs": {}}
-    wl = workloads()
+    wl, nonsym = workloads()
     arrays = {}
     for name, (L, thr, A) in wl.items():
         manifest["inputs"][name] = {"n": L.n, "nnz_lower": L.nnz,
@@ -72,7 +72,7 @@ def main():
                             arrays[f"C1_sched_c{ncore}_{k}"] = v
         if name == "C1_lap2d_128":
             arrays["C1_level_ptr"], arrays["C1_node_to_level"], arrays["C1_perm_to_orig"] = lp, ntl, pto
-        if A is not None and name != "C2_convdiff_64":
+        if A is not None and name not in nonsym:
             lu = O.ic0(A, use_ref=True)
             rng = np.random.default_rng(123)
             lower_pos = np.flatnonzero(A.colidx <= np.repeat(np.arange(A.n), np.diff(A.rowptr)))
@@ -80,7 +80,7 @@ def main():
             manifest["ic0"][name] = {"sum_lower": float(lu[lower_pos].sum()),
                                      "norm_lower": float(np.linalg.norm(lu[lower_pos])),
                                      "pick": pick.tolist(), "values": lu[pick].tolist()}
-        if A is not None and name in ("C2_convdiff_64", "C2_lap3d_64", "bcsstk18"):
+        if A is not None and name in ("C2_convdiff_64", "C2_convdiff_64_perm1", "C2_lap3d_64", "bcsstk18"):
             lu = O.ilu0(A, use_ref=True)
             rng = np.random.default_rng(321)
             pick = np.sort(rng.choice(A.nnz, size=64, replace=False))
@@ -93,6 +93,8 @@ def main():
     y = rng.standard_normal(K.n)
     arrays["blas_x"], arrays["blas_y"] = x, y
     arrays["blas_spmv"] = O.spmv(K, x, use_ref=True)
+    for a, b in [(1.0, 0.0), (-1.0, 1.0), (0.5, -2.0), (1.25, 0.75)]:
+        arrays[f"blas_dcsrmv_{a}_{b}"] = O.dcsrmv(K, x, y, a, b, use_ref=True)
     for a, b in [(1.0, 1.0), (1.0, -1.0), (1.0, 0.375), (0.625, 1.0), (-1.0, 0.0), (-1.0, 0.5),
                  (0.25, 0.0), (0.3, 0.9)]:
         arrays[f"blas_waxpby_{a}_{b}"] = O.waxpby(a, x[:1000], b, y[:1000], use_ref=True)

@@ -7,10 +7,13 @@ matrix (48^3, lognormal conductivities, seed 4).
   central finite differences (step 1e-6) on 64 random entries of each to a
   relative error <= 1e-6.
 * x = A^-1 b by ICCG (FP64 and FP32-preconditioned): the GPU gradient equals
-  the CPU reference gradient (tight CPU PCG) to <= 1e-6 relative; finite
-  differences with direct solves on the 16^3 instance of the same generator.
+  the CPU reference gradient (tight CPU PCG) to <= 1e-6 relative; central
+  finite differences on 64 entries of b and 64 of A at 48^3, the perturbed
+  problems solved exactly through the Sherman-Morrison identity from CPU
+  solves at 1e-14, and with direct solves on the 16^3 instance.
 Batched: small instance of the batched workload, each (system, rhs) column
-against a direct solve and against the oracle's PCG iteration count.
+against a direct solve and against the oracle's PCG iteration count; and the
+full 256 systems x 32 right-hand sides with every column's true residual.
 """
 import numpy as np
 import pytest
@@ -108,6 +111,34 @@ def test_iccg_adjoint_vs_cpu_reference(handle, prec):
     np.testing.assert_allclose(ga[pick], ga_ref[pick], rtol=1e-6, atol=1e-6 * np.abs(ga_ref).max())
 
 
+@pytest.fixture(scope="module")
+def fd48():
+    from adjoint_fd import iccg_fd_48
+    return iccg_fd_48()
+
+
+@pytest.mark.parametrize("prec", [0, 1])
+def test_iccg_adjoint_finite_differences_48(handle, fd48, prec):
+    import torch
+    from baas_amd.api import PCG, Schedule
+    from baas_amd.autograd import iccg_solve
+    A, xs, b, rows_i, ents, fd_b, fd_a = fd48
+    announce("adjoint iccg fd 48^3", prec)
+    s = Schedule(handle, A.rowptr, A.colidx, kind=_lib.KIND_IC0)
+    av = up(A.values).requires_grad_(True)
+    solver = PCG(handle, s, av.detach(), precision=prec)
+    bd = up(b).requires_grad_(True)
+    x = iccg_solve(av, bd, solver, tol=1e-13, maxiter=20000)
+    (0.5 * torch.sum((x - up(xs)) ** 2)).backward()
+    handle.check()
+    gb, ga = down(bd.grad)[rows_i], down(av.grad)[ents]
+    err_b = np.abs(fd_b - gb) / np.abs(gb)
+    err_a = np.abs(fd_a - ga) / np.abs(ga)
+    print(f"[fd48] prec {prec}: max rel. error b {err_b.max():.2e}, A {err_a.max():.2e}")
+    assert np.all(np.abs(fd_b - gb) <= 1e-6 * np.abs(gb) + 1e-9), err_b.max()
+    assert np.all(np.abs(fd_a - ga) <= 1e-6 * np.abs(ga) + 1e-9), err_a.max()
+
+
 @pytest.mark.parametrize("prec", [0, 1])
 def test_iccg_adjoint_finite_differences(handle, prec):
     import scipy.sparse.linalg as spl
@@ -169,3 +200,41 @@ def test_batch_small(handle):
             assert abs(int(iters[si, r]) - k_ref) <= 2, (si, r, iters[si, r], k_ref)
         off += M.n
     print("[batch] tuned", bt.info())
+
+
+def test_batch_full_size(handle):
+    """The whole batched workload: 256 2D Laplacians with sides 16..256
+    (seed 3, 6.07 M rows) x 32 right-hand sides, FP32 factor and FP64 CG to
+    1e-8.  Every one of the 8192 columns must meet ||b - A x|| / ||b|| <=
+    1.1e-8 (true residual, FP64 host SpMV), and a fixed sample of 12 columns
+    must take within 2 iterations of the oracle's PCG in the same precision."""
+    import scipy.sparse as sp
+    import torch
+    from baas_amd.api import Batch
+    announce("batch full 256 x 32")
+    mats, rhs = g.batch_laplace2d()
+    assert len(mats) == 256 and all(R.shape[1] == 32 for R in rhs)
+    bt = Batch(handle, mats, 32)
+    B = up(np.concatenate(rhs, axis=0))
+    X, iters = bt.solve(B, tol=1e-8, maxiter=2000)
+    torch.cuda.synchronize()
+    handle.check()
+    Xh = down(X)
+    del X, B
+    S = sp.block_diag([m.to_scipy() for m in mats], format="csr")
+    Bh = np.concatenate(rhs, axis=0)
+    R = Bh - S @ Xh
+    off = 0
+    worst = 0.0
+    for si, M in enumerate(mats):
+        rel = np.linalg.norm(R[off:off + M.n], axis=0) / np.linalg.norm(Bh[off:off + M.n], axis=0)
+        worst = max(worst, float(rel.max()))
+        assert np.all(rel <= 1.1e-8), (si, M.n, float(rel.max()))
+        off += M.n
+    rng = np.random.default_rng(30)
+    sample = [(int(si), int(r)) for si, r in zip(rng.choice(256, 12, replace=False), rng.integers(0, 32, 12))]
+    for si, r in sample:
+        _, k_ref, _ = O.pcg(mats[si], rhs[si][:, r].copy(), maxiter=2000, tol=1e-8, precision=1)
+        assert abs(int(iters[si, r]) - k_ref) <= 2, (si, r, int(iters[si, r]), k_ref)
+    print(f"[batch full] worst residual {worst:.3e}, iterations {int(iters.min())}..{int(iters.max())}, "
+          f"info {bt.info()}")

@@ -15,7 +15,7 @@ from workloads import workload
 pytestmark = pytest.mark.gpu
 
 SCHED_CASES = [("C1_lap2d_128", 0), ("C2_lap3d_64", 0), ("C2_lap3d_64", 1), ("C2_lap3d_64_perm1", 1),
-               ("C2_convdiff_64", 2), ("C3_knn", 1), ("C5_diff3d_48", 0), ("bcsstk18", 1), ("bcsstk38", 0)]
+               ("C2_convdiff_64", 2), ("C2_convdiff_64_perm1", 2), ("C3_knn", 1), ("C5_diff3d_48", 0), ("bcsstk18", 1), ("bcsstk38", 0)]
 
 
 @pytest.fixture(scope="module")

@@ -20,6 +20,8 @@ def workload(name):
         A, _ = g.random_symmetric_permutation(g.laplace3d(64), 1)
     elif name == "C2_convdiff_64":
         A = g.convdiff3d(64)
+    elif name == "C2_convdiff_64_perm1":
+        A, _ = g.random_symmetric_permutation(g.convdiff3d(64), 1)
     elif name == "C3_knn":
         A, _ = g.knn_stiffness()
     elif name == "C5_diff3d_48":
@@ -31,5 +33,7 @@ def workload(name):
     return A.lower(), A.rowptr, A
 
 
-NAMES = ("C1_lap2d_128", "C2_lap3d_64", "C2_lap3d_64_perm1", "C2_convdiff_64", "C3_knn",
-         "C5_diff3d_48", "bcsstk18", "bcsstk38")
+NAMES = ("C1_lap2d_128", "C2_lap3d_64", "C2_lap3d_64_perm1", "C2_convdiff_64", "C2_convdiff_64_perm1",
+         "C3_knn", "C5_diff3d_48", "bcsstk18", "bcsstk38")
+# nonsymmetric matrices: ILU0 only (IC0 needs SPD)
+NONSYMMETRIC = ("C2_convdiff_64", "C2_convdiff_64_perm1")
