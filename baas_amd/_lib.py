@@ -12,7 +12,8 @@ import os
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libbaas_hip.so")
 
-OK, INVALID_ARG, ZERO_PIVOT, NONPOSITIVE_PIVOT, HIP_ERROR, TIMEOUT, NOT_CONVERGED, ALLOC, UNSUPPORTED = range(9)
+OK, INVALID_ARG, ZERO_PIVOT, NONPOSITIVE_PIVOT, HIP_ERROR, TIMEOUT, NOT_CONVERGED, ALLOC, UNSUPPORTED, TOO_LARGE = range(10)
+NOALLOC_MAX_VALUES = 4194304
 KIND_TRSV, KIND_IC0, KIND_ILU0 = 0, 1, 2
 PLAN_AUTO, PLAN_RANGE, PLAN_SCHEDULE, PLAN_LEVELS, PLAN_TILES, PLAN_BOXES = 0, 1, 2, 3, 4, 5
 EXPORTS = {
@@ -62,6 +63,7 @@ _SIGS = {
     "baas_pcg_autotune": (_i, [_P, _P]),
     "baas_sptrsv_f64": (_i, [_P, _P, _i, _P, _P, _P, _i]),
     "baas_sptrsv_f32": (_i, [_P, _P, _i, _P, _P, _P, _i]),
+    "baas_sptrsv_f64_noalloc": (_i, [_P, _P, _i, _P, _P, _P, _i]),
     "baas_sptrsv_levelsync_f64": (_i, [_P, _P, _P, _P, _P]),
     "baas_transpose_values_f64": (_i, [_P, _P, _P, _P]),
     "baas_cast_f64_f32": (_i, [_P, _i, _P, _P]),

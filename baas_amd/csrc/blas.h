@@ -34,9 +34,15 @@ int spmv_dev(int m, const int *rp, const int *ci, const double *va, const double
 int spmv_blocks(int m);
 int dot_dev(baas_handle *h, int n, const double *x, const double *y, double *result, int do_sqrt, hipStream_t st);
 
+// fixed_xp: device address of a caller-chosen publication area of
+// fixed_bytes (re-armed by this call) instead of the plan-owned one, which
+// the plan allocates on first use; nullptr = the plan-owned area.
 template <class T>
 int sweep(baas_handle *h, baas_schedule *s, int uplo, const T *vals, const T *b, T *x, int nrhs, hipStream_t st,
-          const int *skip);
+          const int *skip, void *fixed_xp = nullptr, size_t fixed_bytes = 0);
+// Look up this device's copy of the allocation-free entry point's scratch
+// array (once per device; the device must be current).
+int noalloc_scratch_init(int device);
 int ic0_split_dev(baas_handle *h, baas_schedule *s, const double *lu, double *lv, double *uv, hipStream_t st);
 int ic0_split_dev32(baas_handle *h, baas_schedule *s, const double *lu, float *lv, float *uv, hipStream_t st);
 

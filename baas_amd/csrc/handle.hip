@@ -1,6 +1,7 @@
 // handle.hip -- library handles, status words and the error reporting path.
 #include <cstring>
 
+#include "blas.h"
 #include "internal.h"
 
 namespace baas {
@@ -35,6 +36,12 @@ extern "C" int baas_create(baas_handle **out, void *stream) {
     }
     h->num_cus = prop.multiProcessorCount;
     h->stream = (hipStream_t)stream;
+    // this device's address of the allocation-free entry point's scratch
+    int rc = baas::noalloc_scratch_init(h->device);
+    if (rc != BAAS_OK) {
+        delete h;
+        return rc;
+    }
     if (hipMalloc(&h->status, sizeof(baas::DevStatus)) != hipSuccess ||
         hipMalloc(&h->counters, BAAS_COUNTER_INTS * sizeof(int)) != hipSuccess) {
         if (h->status) (void)hipFree(h->status);
@@ -98,6 +105,7 @@ extern "C" const char *baas_status_string(int s) {
         case BAAS_NOT_CONVERGED: return "not converged";
         case BAAS_ALLOC: return "allocation failed";
         case BAAS_UNSUPPORTED: return "unsupported";
+        case BAAS_TOO_LARGE: return "input exceeds the fixed scratch of the allocation-free entry point";
         default: return "unknown status";
     }
 }

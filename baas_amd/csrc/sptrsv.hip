@@ -13,7 +13,9 @@
 // staged in LDS; parents in earlier units are waited for on per-row ready
 // flags (epoch-stamped, so no reset between calls).
 #include <algorithm>
+#include <mutex>
 
+#include "blas.h"
 #include "device.h"
 #include "internal.h"
 
@@ -879,9 +881,11 @@ static int launch_levels(baas_handle *h, baas_schedule *s, const Plan &p, int up
 
 template <class T>
 int sweep(baas_handle *h, baas_schedule *s, int uplo, const T *vals, const T *b, T *x, int nrhs,
-          hipStream_t stream, const int *skip) {
+          hipStream_t stream, const int *skip, void *fixed_xp, size_t fixed_bytes) {
     if (!h || !s || !vals || !b || !x || nrhs < 1 || (uplo != 0 && uplo != 1)) return BAAS_INVALID_ARG;
     if ((const void *)b == (const void *)x) return BAAS_INVALID_ARG;
+    // a fixed publication area must hold both halves at any plan's layout
+    if (fixed_xp && 2 * (size_t)s->n * (size_t)nrhs * sizeof(T) > fixed_bytes) return BAAS_TOO_LARGE;
     Plan &p = uplo ? s->bwd : s->fwd;
     if (p.mode == BAAS_PLAN_LEVELS) return launch_levels<T>(h, s, p, uplo, vals, b, x, nrhs, stream, skip);
     const bool lane = p.dataflow == 2 && p.block >= 128 && p.lane_lds((int)sizeof(T), nrhs) <= lds_limit(h);
@@ -890,7 +894,16 @@ int sweep(baas_handle *h, baas_schedule *s, int uplo, const T *vals, const T *b,
         // publication halves (one value per row and right-hand side), armed
         // with this value type's sentinel; any change of layout re-arms them
         const int xr = lane ? nrhs : 1;
-        if (p.xp_type != (int)sizeof(T) || p.xp_nrhs != xr) {
+        T *xp = reinterpret_cast<T *>(p.xp_raw.p);
+        if (fixed_xp) {
+            // shared by every call on this GPU and left holding the last
+            // call's values: arm both halves (a fill kernel, no memset, so
+            // the call can be captured in a graph)
+            const size_t cnt = 2 * (size_t)s->n * xr;
+            xp = reinterpret_cast<T *>(fixed_xp);
+            hipLaunchKernelGGL(k_fill_sentinel<T>, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, stream, cnt, xp);
+            BAAS_HIP_TRY(hipGetLastError());
+        } else if (p.xp_type != (int)sizeof(T) || p.xp_nrhs != xr) {
             const size_t cnt = 2 * (size_t)s->n * xr;
             BAAS_TRY(p.xp_raw.alloc(cnt * sizeof(T)));
             hipLaunchKernelGGL(k_fill_sentinel<T>, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, stream, cnt,
@@ -898,6 +911,7 @@ int sweep(baas_handle *h, baas_schedule *s, int uplo, const T *vals, const T *b,
             BAAS_HIP_TRY(hipGetLastError());
             p.xp_type = (int)sizeof(T);
             p.xp_nrhs = xr;
+            xp = reinterpret_cast<T *>(p.xp_raw.p);
         }
         DfArgs d;
         d.nrhs = xr;
@@ -918,7 +932,6 @@ int sweep(baas_handle *h, baas_schedule *s, int uplo, const T *vals, const T *b,
         d.skip = skip;
         d.status = h->status;
         d.spin_limit = h->spin_limit;
-        T *xp = reinterpret_cast<T *>(p.xp_raw.p);
         if (lane) {
             const size_t lds = p.lane_lds((int)sizeof(T), nrhs);
             return uplo ? launch_lane<T, true>(h, p, d, vals, b, x, xp, lds, stream)
@@ -954,9 +967,40 @@ int sweep(baas_handle *h, baas_schedule *s, int uplo, const T *vals, const T *b,
 }
 
 template int sweep<double>(baas_handle *, baas_schedule *, int, const double *, const double *, double *, int,
-                           hipStream_t, const int *);
+                           hipStream_t, const int *, void *, size_t);
 template int sweep<float>(baas_handle *, baas_schedule *, int, const float *, const float *, float *, int,
-                          hipStream_t, const int *);
+                          hipStream_t, const int *, void *, size_t);
+
+// ---------------------------------------------------------------------------
+// Scratch of the allocation-free entry point (baas_sptrsv_f64_noalloc): the
+// two publication halves of the dataflow sweeps, 2 x n x nrhs doubles, in a
+// fixed array that each GPU holds from the library's load until the process
+// ends.  In host code `scratch` names a host-side stand-in, not device
+// memory: kernels get the device address found by hipGetSymbolAddress, one
+// per device.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)BAAS_NOALLOC_MAX_VALUES * sizeof(double);
+static __device__ __align__(256) double scratch[BAAS_NOALLOC_MAX_VALUES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static void *g_scratch_dev[NOALLOC_MAX_DEVICES];
+static std::mutex g_scratch_mu;
+
+int noalloc_scratch_init(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return BAAS_UNSUPPORTED;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_dev[device]) return BAAS_OK;
+    void *ptr = nullptr;
+    BAAS_HIP_TRY(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return BAAS_HIP_ERROR;
+    g_scratch_dev[device] = ptr;
+    return BAAS_OK;
+}
+
+static void *noalloc_scratch(int device) {
+    if (device < 0 || device >= NOALLOC_MAX_DEVICES) return nullptr;
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return g_scratch_dev[device];
+}
 
 }  // namespace baas
 
@@ -970,6 +1014,14 @@ extern "C" int baas_sptrsv_f64(baas_handle *h, baas_schedule *s, int uplo, const
 extern "C" int baas_sptrsv_f32(baas_handle *h, baas_schedule *s, int uplo, const float *vals, const float *b,
                                float *x, int nrhs) {
     return sweep<float>(h, s, uplo, vals, b, x, nrhs, h ? h->stream : nullptr, nullptr);
+}
+
+extern "C" int baas_sptrsv_f64_noalloc(baas_handle *h, baas_schedule *s, int uplo, const double *vals,
+                                       const double *b, double *x, int nrhs) {
+    if (!h) return BAAS_INVALID_ARG;
+    void *ptr = noalloc_scratch(h->device);  // set up by baas_create on this device
+    if (!ptr) return BAAS_HIP_ERROR;
+    return sweep<double>(h, s, uplo, vals, b, x, nrhs, h->stream, nullptr, ptr, NOALLOC_SCRATCH_BYTES);
 }
 
 extern "C" int baas_sptrsv_levelsync_f64(baas_handle *h, baas_schedule *s, const double *vals, const double *b,

@@ -38,7 +38,8 @@ enum baas_status {
     BAAS_TIMEOUT = 5,          /* a ready-flag wait exceeded the watchdog spin limit */
     BAAS_NOT_CONVERGED = 6,
     BAAS_ALLOC = 7,
-    BAAS_UNSUPPORTED = 8
+    BAAS_UNSUPPORTED = 8,
+    BAAS_TOO_LARGE = 9         /* beyond the fixed scratch of baas_sptrsv_f64_noalloc */
 };
 
 /* Which numeric kernel a schedule is analysed for; selects the threads-per-
@@ -142,6 +143,21 @@ int baas_sptrsv_f64(baas_handle *h, baas_schedule *s, int uplo, const double *va
                     const double *b, double *x, int nrhs);
 int baas_sptrsv_f32(baas_handle *h, baas_schedule *s, int uplo, const float *vals,
                     const float *b, float *x, int nrhs);
+/* No counterpart in the reference.  For callers that cannot allocate (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): the solve of baas_sptrsv_f64, same arguments and the same
+ * bits, but it calls no hipMalloc/hipFree.  baas_sptrsv_f64 allocates a
+ * plan's publication area (2 x n x nrhs values) on its first dataflow call;
+ * this entry point uses a fixed device array of BAAS_NOALLOC_MAX_VALUES
+ * doubles (32 MiB, held by each GPU the library runs on until the process
+ * ends) that every call on that GPU shares, re-armed at the start of each
+ * call by a fill kernel.  Solves with 2 x n x nrhs > BAAS_NOALLOC_MAX_VALUES
+ * return BAAS_TOO_LARGE and launch nothing.  Calls that overlap on one GPU
+ * (several streams or threads) race on the array: serialise them.  The
+ * schedule (baas_analyze) must exist before; it is not allocated here. */
+#define BAAS_NOALLOC_MAX_VALUES 4194304
+int baas_sptrsv_f64_noalloc(baas_handle *h, baas_schedule *s, int uplo, const double *vals,
+                            const double *b, double *x, int nrhs);
 /* Level-synchronous solve, one launch per row level (sptrsv_level_csr_no_perm,
  * merge/SpTRSV.cpp:41-80).  Same bits as baas_sptrsv_f64 with uplo 0. */
 int baas_sptrsv_levelsync_f64(baas_handle *h, baas_schedule *s, const double *vals,
