@@ -105,6 +105,9 @@ struct Plan {
     size_t lane_lds(int ts, int nrhs);  // LDS bytes per workgroup (cached per ts, nrhs)
     int lane_lds_key = -1;
     size_t lane_lds_val = 0;
+    size_t mr_lds(int ts, int nrhs, bool stage_b);  // the same for the multi-RHS lane kernel
+    int mr_lds_key = -1;
+    size_t mr_lds_val = 0;
     DBuf<unsigned char> xp_raw;  // 2 x n x nrhs published values (dataflow kernels)
     int xp_type = 0;             // sizeof the value type the halves are armed for
     int xp_nrhs = 0;             // ... and their right-hand sides per row
@@ -125,6 +128,27 @@ struct LaneLayout {
     size_t sl, bv, dv, rec, slot, row, ev, es, total;
 };
 __host__ __device__ inline size_t lane_align8(size_t v) { return (v + 7) & ~size_t(7); }
+// Multi-RHS lane kernel (dataflow 3): value slots of the unit's own rows
+// only (parents in other units are read from the publication half), no
+// staged b; per row the diagonal, record, LDS slot and row id; per entry the
+// coefficient and its parent (LDS slot, or ~row for another unit's row).
+// With stage_b the unit's b values are copied into LDS too (nk x nrhs).
+struct MrLayout {
+    size_t sl, bv, dv, rec, slot, row, ev, es, total;
+};
+__host__ __device__ inline MrLayout mr_layout(size_t nk, size_t E, size_t ts, size_t nrhs, bool stage_b = false) {
+    MrLayout l;
+    l.sl = 0;
+    l.bv = lane_align8(nk * nrhs * ts);
+    l.dv = l.bv + (stage_b ? lane_align8(nk * nrhs * ts) : 0);
+    l.rec = l.dv + lane_align8(nk * ts);
+    l.slot = l.rec + nk * 8;
+    l.row = l.slot + lane_align8(nk * 4);
+    l.ev = l.row + lane_align8(nk * 4);
+    l.es = l.ev + lane_align8(E * ts);
+    l.total = l.es + lane_align8(E * 4);
+    return l;
+}
 __host__ __device__ inline LaneLayout lane_layout(size_t nk, size_t ne, size_t E, size_t ts, size_t nrhs = 1) {
     LaneLayout l;
     l.sl = 0;

@@ -194,6 +194,7 @@ int make_plan(Plan &p, int n, const std::vector<int> &order, const std::vector<i
     p.xp_type = 0;
     p.xp_nrhs = 0;
     p.lane_lds_key = -1;
+    p.mr_lds_key = -1;
     BAAS_TRY(upload(p.scol, scol, st));
     BAAS_TRY(upload(p.ext_ptr, ext_ptr, st));
     BAAS_TRY(upload(p.ext_cols, ext_cols, st));
@@ -227,6 +228,17 @@ int make_plan(Plan &p, int n, const std::vector<int> &order, const std::vector<i
 }
 
 }  // namespace
+
+size_t Plan::mr_lds(int ts, int nrhs, bool stage_b) {
+    const int key = (nrhs * 16 + ts) * 2 + (stage_b ? 1 : 0);
+    if (key == mr_lds_key) return mr_lds_val;
+    size_t m = 0;
+    for (size_t u = 0; u + 2 < h_lane_units.size(); u += 3)
+        m = std::max(m, mr_layout(h_lane_units[u], h_lane_units[u + 2], ts, nrhs, stage_b).total);
+    mr_lds_key = key;
+    mr_lds_val = m;
+    return m;
+}
 
 size_t Plan::lane_lds(int ts, int nrhs) {
     const int key = nrhs * 16 + ts;

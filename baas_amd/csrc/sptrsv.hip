@@ -14,6 +14,7 @@
 // flags (epoch-stamped, so no reset between calls).
 #include <algorithm>
 #include <mutex>
+#include <type_traits>
 
 #include "blas.h"
 #include "device.h"
@@ -573,6 +574,242 @@ __global__ __launch_bounds__(1024) void k_sweep_lane(DfArgs a, const int *__rest
     }
 }
 
+// ---------------------------------------------------------------------------
+// Multi-RHS lane kernel (dataflow plans, variant 3).  The lane kernel's
+// (row, right-hand side) items, with the exchange moved into the compute
+// lanes: a lane reads a parent in another unit straight from the
+// publication half (the value is the flag, one load per poll) instead of
+// waiting for an exchange wavefront to stage it, reads b when its item
+// starts and writes x when the item is solved.  So LDS holds only the
+// unit's own values (nk x nrhs), every wavefront computes, and with k
+// right-hand sides the k lanes of a row load and store its k contiguous
+// values together.  Same summation order and division as every other sweep.
+template <class T, bool UPPER, int CH>
+__device__ __forceinline__ void mr_chunk(const T *lev, const int *les, int eo, int k, int nc, int nr, int r,
+                                         int (&cs)[CH], T (&cv)[CH]) {
+    const int last = nc > 0 ? nc - 1 : 0;
+#pragma unroll
+    for (int t = 0; t < CH; t++) {
+        const int i = eo + k + (t < last ? t : last);
+        const int e = les[i];
+        // LDS slot x nr + r for the unit's rows; ~(row x nr + r) for others'
+        cs[t] = nc > 0 ? (e >= 0 ? e * nr + r : ~((~e) * nr + r)) : 0;
+        cv[t] = nc > 0 ? lev[i] : T(0);
+    }
+}
+
+// V right-hand sides per lane: a lane's item is position p with the V
+// contiguous right-hand sides r .. r + V - 1 (V divides nrhs), so a row's
+// control flow is shared by V values.
+template <class T, int V>
+struct MrVec {
+    T e[V];
+};
+
+template <class T, int V>
+__device__ __forceinline__ MrVec<T, V> ld_lds_vec(const T *p) {
+    MrVec<T, V> w;
+    if constexpr (V == 1) {
+        w.e[0] = ld_lds(p);
+    } else {
+        // one LDS read of V values (polled: volatile, never hoisted); a torn
+        // read only shows a value as not yet there
+        typedef T vt __attribute__((ext_vector_type(V)));
+        const vt u = *reinterpret_cast<const volatile vt *>(p);
+#pragma unroll
+        for (int i = 0; i < V; i++) w.e[i] = u[i];
+    }
+    return w;
+}
+
+template <class T, bool UPPER, int CH, bool ONE, bool SB, int V>
+__device__ __forceinline__ void mr_solve(const DfArgs &a, T *sl, const T *lb, const T *lg, const int2 *lrec,
+                                         const int *lslot, const int *lrow, const T *lev, const int *les,
+                                         const T *__restrict__ b, T *x, const T *pub, T *nxt, int it,
+                                         int stride, int nk) {
+    const int nr = a.nrhs, ng = nr / V, items = nk * ng;
+    bool have = it < items;
+    int eo = 0, cnt = 0, k = 0, nc = 0, pc = 0, vslot = 0, vrow = 0, r = 0;
+    bool ext = false;
+    T s[V], bv[V], dv = T(1);
+    int cs[CH];
+    T cv[CH];
+    long long spins = 0;
+    auto start = [&]() {
+        const int p = ng == 1 ? it : it / ng;
+        r = (it - p * ng) * V;
+        const int2 rc = lrec[p];
+        eo = rc.x;
+        cnt = rc.y & 0x7fffffff;
+        ext = rc.y < 0;
+        vslot = lslot[p] * nr + r;
+        vrow = lrow[p] * nr + r;
+#pragma unroll
+        for (int i = 0; i < V; i++) {
+            bv[i] = SB ? lb[p * nr + r + i] : b[vrow + i];
+            s[i] = T(0);
+        }
+        dv = lg[p];
+        k = 0;
+        pc = 0;
+        spins = 0;
+        nc = cnt < CH ? cnt : CH;
+        mr_chunk<T, UPPER, CH>(lev, les, eo, 0, nc, nr, r, cs, cv);
+    };
+    if (have) start();
+    // The body runs in every lane of a wavefront that still has work, the
+    // lanes without any masked by predicates rather than branches: every
+    // divergent branch costs scalar instructions (exec-mask saves and
+    // restores), and the CU's one scalar unit issues them for all its
+    // wavefronts.
+    while (__ballot(have) != 0) {
+        const int pc0 = pc;
+        {
+            // LDS slots for every parent (other units' rows read slot 0 and
+            // are replaced below), then, only in lanes that still wait for a
+            // parent of another unit, those values from memory.  The memory
+            // wait stays inside that branch: a wait after it would make
+            // every wavefront of the unit wait for its lanes' outstanding
+            // stores (loads and stores share one counter) on every poll.
+            MrVec<T, V> v[CH];
+            bool need = false;
+#pragma unroll
+            for (int t = 0; t < CH; t++) {
+                v[t] = ld_lds_vec<T, V>(&sl[cs[t] >= 0 ? cs[t] : 0]);
+                need |= have && cs[t] < 0 && t >= pc && t < nc;
+            }
+            if (need) {
+                MrVec<T, V> gv[CH];
+#pragma unroll
+                for (int t = 0; t < CH; t++)
+#pragma unroll
+                    for (int i = 0; i < V; i++)
+                        gv[t].e[i] = (cs[t] < 0 && t >= pc && t < nc) ? ld_agent(&pub[~cs[t] + i])
+                                                                      : Sentinel<T>::value();
+                __builtin_amdgcn_s_waitcnt(0);
+#pragma unroll
+                for (int t = 0; t < CH; t++)
+                    if (cs[t] < 0) v[t] = gv[t];
+            }
+            // consume the ready prefix in order, by selects
+#pragma unroll
+            for (int t = 0; t < CH; t++) {
+                bool take = have && t == pc && t < nc;
+#pragma unroll
+                for (int i = 0; i < V; i++) take = take && !Sentinel<T>::is(v[t].e[i]);
+#pragma unroll
+                for (int i = 0; i < V; i++) {
+                    const T f = UPPER ? fma(v[t].e[i], cv[t], s[i]) : fma(cv[t], v[t].e[i], s[i]);
+                    s[i] = take ? f : s[i];
+                }
+                pc += take ? 1 : 0;
+            }
+        }
+        if (have && pc == nc) {
+            k += nc;
+            if (!ONE && k < cnt) {
+                nc = cnt - k < CH ? cnt - k : CH;
+                pc = 0;
+                mr_chunk<T, UPPER, CH>(lev, les, eo, k, nc, nr, r, cs, cv);
+            } else {
+                T xv[V];
+#pragma unroll
+                for (int i = 0; i < V; i++) {
+                    xv[i] = (bv[i] - s[i]) / dv;
+                    st_lds(&sl[vslot + i], xv[i]);
+                    x[vrow + i] = xv[i];
+                }
+                if (ext) {
+#pragma unroll
+                    for (int i = 0; i < V; i++) {
+                        st_agent(const_cast<T *>(&pub[vrow + i]), xv[i]);
+                        nxt[vrow + i] = Sentinel<T>::value();
+                    }
+                }
+                it += stride;
+                have = it < items;
+                if (have) start();
+            }
+        } else {
+            spins += have ? 1 : 0;
+            // a wavefront none of whose lanes moved yields its issue slots
+            // to the others of its SIMD for a moment
+            if (__ballot(pc != pc0) == 0) __builtin_amdgcn_s_sleep(1);
+            if (spins > a.spin_limit) {
+                raise_status(a.status, BAAS_TIMEOUT, vrow);
+                pc = nc;  // give up on this row's remaining parents
+                cnt = k + nc;
+            }
+        }
+    }
+}
+
+template <class T, bool UPPER, int CH, bool ONE, bool SB, int V>
+__global__ __launch_bounds__(1024) void k_sweep_mr(DfArgs a, const int *__restrict__ eoff,
+                                                  const int *__restrict__ eptr, const T *__restrict__ vals,
+                                                  const T *__restrict__ b, T *x, T *xp) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ int s_unit;
+    if (a.skip && ld_agent(a.skip)) return;
+    const int epoch = read_epoch(a.ctx);
+    const int par = ld_agent(a.pcount) & 1;
+    const int nr = a.nrhs;
+    T *pub = xp + (size_t)par * a.n * nr;
+    T *nxt = xp + (size_t)(par ^ 1) * a.n * nr;
+    for (;;) {
+        const int u = claim_unit(a.ctx.ticket, &s_unit);
+        if (u >= a.nunits) break;
+        const int q0 = a.unit_ptr[u], q1 = a.unit_ptr[u + 1], nk = q1 - q0;
+        const int e0 = a.ext_ptr[u];
+        const MrLayout lay = mr_layout(nk, eptr[u + 1] - eptr[u], sizeof(T), nr, SB);
+        T *sl = reinterpret_cast<T *>(smem_raw + lay.sl);
+        T *lb = reinterpret_cast<T *>(smem_raw + lay.bv);
+        T *lg = reinterpret_cast<T *>(smem_raw + lay.dv);
+        int2 *lrec = reinterpret_cast<int2 *>(smem_raw + lay.rec);
+        int *lslot = reinterpret_cast<int *>(smem_raw + lay.slot);
+        int *lrow = reinterpret_cast<int *>(smem_raw + lay.row);
+        T *lev = reinterpret_cast<T *>(smem_raw + lay.ev);
+        int *les = reinterpret_cast<int *>(smem_raw + lay.es);
+        for (int i = threadIdx.x; i < nk * nr; i += blockDim.x) st_lds(&sl[i], Sentinel<T>::value());
+        for (int q = threadIdx.x; q < nk; q += blockDim.x) {
+            const int4 m = a.meta[q0 + q];
+            const int cnt = m.z, eo = eoff[q0 + q];
+            lg[q] = vals[UPPER ? m.y - cnt : m.y + cnt];
+            lrec[q] = make_int2(eo, cnt | (m.x & (int)0x80000000));
+            lslot[q] = m.w;
+            lrow[q] = m.x & 0x7fffffff;
+            for (int e = 0; e < cnt; e++) {
+                const int j = UPPER ? m.y - e : m.y + e;
+                const int sc = a.scol[j];
+                lev[eo + e] = vals[j];
+                les[eo + e] = sc < nk ? sc : ~a.ext_cols[e0 + sc - nk];
+            }
+        }
+        if (SB) {  // b of every (position, right-hand side), contiguous per row
+            __syncthreads();
+            for (int i = threadIdx.x; i < nk * nr; i += blockDim.x) {
+                const int q = nr == 1 ? i : i / nr;
+                lb[i] = b[(size_t)lrow[q] * nr + (i - q * nr)];
+            }
+        }
+        __syncthreads();
+        mr_solve<T, UPPER, CH, ONE, SB, V>(a, sl, lb, lg, lrec, lslot, lrow, lev, les, b, x, pub, nxt, threadIdx.x,
+                                           blockDim.x, nk);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const int d = atomicAdd(a.ctx.done, 1);
+        if (d == (int)gridDim.x - 1) {
+            atomicExch(a.ctx.ticket, 0);
+            atomicExch(a.ctx.epoch, epoch == 0x7fffffff ? 1 : epoch + 1);
+            atomicAdd(a.pcount, 1);
+            __threadfence();
+            atomicExch(a.ctx.done, 0);
+        }
+    }
+}
+
 template <class T>
 __global__ void k_fill_sentinel(size_t n, T *p) {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -694,6 +931,36 @@ static int launch_lane(baas_handle *h, const Plan &p, const DfArgs &a, const T *
     // at most 16 wavefronts per CU: more resident units share the SIMDs'
     // issue slots and slow every poll loop (whatever the register count
     // would allow)
+    per_cu = std::min(per_cu, std::max(1, 1024 / p.block));
+    long long grid = (long long)per_cu * h->num_cus;
+    if (grid > p.nunits) grid = p.nunits;
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(p.block), lds, stream, a, p.lane_eoff.p, p.lane_eptr.p, vals,
+                       b, x, xp);
+    BAAS_HIP_TRY(hipGetLastError());
+    return BAAS_OK;
+}
+
+template <class T, bool UP>
+static int launch_mr(baas_handle *h, const Plan &p, const DfArgs &a, const T *vals, const T *b, T *x, T *xp,
+                     size_t lds, bool sb, hipStream_t stream) {
+    using K = void (*)(DfArgs, const int *, const int *, const T *, const T *, T *, T *);
+    // chunk width as in the lane kernel (rows of up to four parents, else
+    // eight or two-wide chunks).  One value per lane: several right-hand
+    // sides per lane chain their IEEE divisions (they share VCC) and
+    // measured slower on every case.
+    auto pick = [&](auto sbtag) -> K {
+        constexpr bool S = decltype(sbtag)::value;
+        return p.max_deg <= 2   ? k_sweep_mr<T, UP, 2, true, S, 1>
+               : p.max_deg <= 3 ? k_sweep_mr<T, UP, 3, true, S, 1>
+               : p.max_deg <= 4 ? k_sweep_mr<T, UP, 4, true, S, 1>
+               : p.max_deg <= 8 ? k_sweep_mr<T, UP, 8, true, S, 1>
+                                : k_sweep_mr<T, UP, 2, false, S, 1>;
+    };
+    K kern = sb ? pick(std::true_type{}) : pick(std::false_type{});
+    int per_cu = 0;
+    BAAS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, p.block, lds));
+    if (per_cu <= 0) return BAAS_UNSUPPORTED;
     per_cu = std::min(per_cu, std::max(1, 1024 / p.block));
     long long grid = (long long)per_cu * h->num_cus;
     if (grid > p.nunits) grid = p.nunits;
@@ -888,7 +1155,12 @@ int sweep(baas_handle *h, baas_schedule *s, int uplo, const T *vals, const T *b,
     if (fixed_xp && 2 * (size_t)s->n * (size_t)nrhs * sizeof(T) > fixed_bytes) return BAAS_TOO_LARGE;
     Plan &p = uplo ? s->bwd : s->fwd;
     if (p.mode == BAAS_PLAN_LEVELS) return launch_levels<T>(h, s, p, uplo, vals, b, x, nrhs, stream, skip);
-    const bool lane = p.dataflow == 2 && p.block >= 128 && p.lane_lds((int)sizeof(T), nrhs) <= lds_limit(h);
+    // multi-RHS lane kernel: b staged in LDS where it fits (dataflow 3), or
+    // always read from memory (dataflow 4)
+    const bool mr = (p.dataflow == 3 || p.dataflow == 4) && p.block >= 64 &&
+                    p.mr_lds((int)sizeof(T), nrhs, false) <= lds_limit(h);
+    const bool mr_sb = mr && p.dataflow == 3 && p.mr_lds((int)sizeof(T), nrhs, true) <= lds_limit(h);
+    const bool lane = mr || (p.dataflow == 2 && p.block >= 128 && p.lane_lds((int)sizeof(T), nrhs) <= lds_limit(h));
     if (lane || (nrhs == 1 && p.dataflow == 1 && p.block >= 192 && p.block <= 512 &&
                  p.df_lds[sizeof(T) == 8] <= lds_limit(h))) {
         // publication halves (one value per row and right-hand side), armed
@@ -932,6 +1204,11 @@ int sweep(baas_handle *h, baas_schedule *s, int uplo, const T *vals, const T *b,
         d.skip = skip;
         d.status = h->status;
         d.spin_limit = h->spin_limit;
+        if (mr) {
+            const size_t lds = p.mr_lds((int)sizeof(T), nrhs, mr_sb);
+            return uplo ? launch_mr<T, true>(h, p, d, vals, b, x, xp, lds, mr_sb, stream)
+                        : launch_mr<T, false>(h, p, d, vals, b, x, xp, lds, mr_sb, stream);
+        }
         if (lane) {
             const size_t lds = p.lane_lds((int)sizeof(T), nrhs);
             return uplo ? launch_lane<T, true>(h, p, d, vals, b, x, xp, lds, stream)
@@ -1079,7 +1356,7 @@ extern "C" int baas_csr_outer_grad_f64(baas_handle *h, int m, const int *rp, con
 
 extern "C" int baas_plan_dataflow(baas_schedule *s, int enable) {
     if (!s) return BAAS_INVALID_ARG;
-    if (enable < 0 || enable > 2) return BAAS_INVALID_ARG;
+    if (enable < 0 || enable > 4) return BAAS_INVALID_ARG;
     s->fwd.dataflow = s->bwd.dataflow = enable;
     s->plan_gen++;
     return BAAS_OK;

@@ -94,8 +94,17 @@ int autotune(baas_handle *h, baas_schedule *s, const T *vals, int uplo, int nrhs
         }
         cands.push_back({BAAS_PLAN_BOXES, R, 256, 0, 0});
     }
-    if (nrhs > 1)
+    if (nrhs > 1) {
         for (int R : {64, 125, 216}) cands.push_back({BAAS_PLAN_BOXES, R, 1024, 2, 0});
+        // the multi-RHS lane kernel (parents of other units read directly,
+        // b staged in LDS where it fits or read from memory)
+        for (int R : {216, 512})
+            for (int blk : {512, 1024})
+                for (int df : {3, 4}) cands.push_back({BAAS_PLAN_BOXES, R, blk, df, 0});
+        for (int rpu : {256, 512, 1024})
+            for (int df : {3, 4}) cands.push_back({BAAS_PLAN_RANGE, rpu, 1024, df, 0});
+        for (int rpu : {32, 64, 128}) cands.push_back({BAAS_PLAN_SCHEDULE, rpu, 1024, 4, 0});
+    }
     for (int S : TILE_RANGES) {
         if (S >= 2 * n && S != TILE_RANGES[0]) continue;
         for (int hb : TILE_BANDS)
@@ -126,6 +135,7 @@ int autotune(baas_handle *h, baas_schedule *s, const T *vals, int uplo, int nrhs
         Plan &p = uplo ? s->bwd : s->fwd;
         if (c.df == 1 && p.df_lds[sizeof(T) == 8] > lds_limit(h)) continue;
         if (c.df == 2 && p.lane_lds((int)sizeof(T), nrhs) > lds_limit(h)) continue;
+        if (c.df >= 3 && p.mr_lds((int)sizeof(T), nrhs, false) > lds_limit(h)) continue;
         s->fwd.dataflow = s->bwd.dataflow = c.df;
         float t_best = 1e30f;
         bool ok = true;

@@ -258,7 +258,13 @@ int baas_schedule_plan(const baas_schedule *s, int *plan);
  * right-hand side per item, any nrhs) on its own and publishing them itself
  * (workgroups of at least 128 threads); 0 = off.  Enable 2 also runs IC0 and
  * ILU0 on this schedule's units with the same design (rows with at most 255
- * lower entries; others keep the unit kernels). */
+ * lower entries; others keep the unit kernels).  Enable 3 and 4 = the
+ * multi-right-hand-side lane kernel: (row, right-hand side) items over every
+ * lane of the workgroup, parents in other units read straight from the
+ * publication array by the lane that needs them (no exchange wavefront),
+ * x written as each item is solved, so LDS holds only the unit's own
+ * values; 3 also stages the unit's b in LDS where it fits, 4 reads b from
+ * memory.  Sweeps only (factorisations keep the unit kernels). */
 int baas_plan_dataflow(baas_schedule *s, int enable);
 /* Autotune the execution plan for sweeps with `uplo` and `nrhs` on these
  * values: times RANGE, SCHEDULE, LEVELS and TILES packings over their unit

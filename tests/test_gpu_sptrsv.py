@@ -67,7 +67,8 @@ PLANS = [(1, 0, 256, False), (1, 1024, 256, False), (1, 16384, 512, False), (2, 
          (1, 2048, 192, True), (2, 256, 256, True), (2, 2048, 512, True), (1, 8192, 1024, True),
          (3, 0, 256, False), (3, 0, 1024, False), (4, 16384, 256, True), (4, 32768, 512, True),
          (4, 8192, 1024, False), (1, 1024, 1024, 2), (1, 512, 512, 2), (1, 4096, 128, 2), (2, 256, 512, 2),
-         (4, 16384, 1024, 2), (3, 1, 1024, False), (3, 4, 512, False), (2, 64, 1024, 2), (2, 32, 512, 2)]
+         (4, 16384, 1024, 2), (3, 1, 1024, False), (3, 4, 512, False), (2, 64, 1024, 2), (2, 32, 512, 2),
+         (1, 512, 1024, 3), (2, 32, 512, 4), (4, 16384, 1024, 4), (1, 256, 256, 3)]
 SOLVE_CASES = ["C1_lap2d_128", "C2_lap3d_64", "C2_lap3d_64_perm1", "C5_diff3d_48", "bcsstk18", "bcsstk38"]
 
 
@@ -120,7 +121,8 @@ def test_multi_rhs_bit_exact(handle, k):
         X_ref = O.sptrsv_lower_multi(L, B)
         for mode, rpu, blk, df in [(1, 0, 256, 0), (1, 256, 256, 0), (2, 0, 256, 0), (3, 0, 512, 0),
                                    (4, 16384, 256, 0), (3, 1, 1024, 0), (1, 32, 1024, 2), (2, 16, 512, 2),
-                                   (1, 300, 128, 2)]:
+                                   (1, 300, 128, 2), (1, 256, 1024, 3), (2, 32, 512, 4), (1, 512, 1024, 4),
+                                   (1, 100, 64, 3)]:
             announce("multi", name, k, mode, rpu, blk, df)
             s.replan(mode, rpu, blk, dataflow=df)
             X = down(s.solve(up(L.values), up(B)))
@@ -140,7 +142,8 @@ def test_f32_bit_exact(handle):
         L, _, _ = workload(name)
         s = Schedule(handle, L.rowptr, L.colidx, kind=0)
         b = g.rhs_uniform(L.n, 1)
-        for mode, rpu, df in ((1, 4096, False), (1, 4096, True), (3, 0, False), (4, 16384, True), (1, 1024, 2)):
+        for mode, rpu, df in ((1, 4096, False), (1, 4096, True), (3, 0, False), (4, 16384, True), (1, 1024, 2),
+                              (1, 1024, 3), (2, 32, 4)):
             s.replan(mode, rpu, 256, dataflow=df)
             x = down(s.solve(up(L.values, torch.float32), up(b, torch.float32)))
             handle.check()
