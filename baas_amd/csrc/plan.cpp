@@ -380,9 +380,24 @@ static int box_units(const baas_schedule *s, const HostCsr &L, int R, std::vecto
     const long long gx = (nx + bx - 1) / bx, gy = (ny + by - 1) / by, gz = (nz + bz - 1) / bz;
     const long long nbox = gx * gy * gz;
     if (nbox > (1LL << 26)) return BAAS_INVALID_ARG;
+    // boxes numbered in wavefront order: by the sum of their box
+    // coordinates (every parent box has a smaller sum), then z, y, x; the
+    // persistent grid claims units in this order, so the boxes it holds at
+    // any moment are those whose parents are done or running
+    std::vector<long long> rank((size_t)nbox);
+    {
+        std::vector<long long> ids((size_t)nbox);
+        for (long long t = 0; t < nbox; t++) ids[t] = t;
+        auto key = [&](long long t) {
+            const long long tx = t % gx, ty = (t / gx) % gy, tz = t / (gx * gy);
+            return std::make_pair(tx + ty + tz, t);
+        };
+        std::sort(ids.begin(), ids.end(), [&](long long a, long long b) { return key(a) < key(b); });
+        for (long long q = 0; q < nbox; q++) rank[ids[q]] = q;
+    }
     auto box_of = [&](int r) {
         const long long x = r % nx, y = (r / nx) % ny, z = r / (nx * ny);
-        return ((z / bz) * gy + y / by) * gx + x / bx;
+        return rank[((z / bz) * gy + y / by) * gx + x / bx];
     };
     std::vector<int> start((size_t)nbox + 1, 0);
     for (int r = 0; r < n; r++) start[box_of(r) + 1]++;

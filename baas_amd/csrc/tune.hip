@@ -97,14 +97,15 @@ int autotune(baas_handle *h, baas_schedule *s, const T *vals, int uplo, int nrhs
     if (nrhs > 1) {
         for (int R : {64, 125, 216}) cands.push_back({BAAS_PLAN_BOXES, R, 1024, 2, 0});
         // the multi-RHS lane kernel (parents of other units read directly,
-        // b staged in LDS where it fits or read from memory)
-        for (int R : {216, 512})
-            for (int blk : {512, 1024})
-                for (int df : {3, 4}) cands.push_back({BAAS_PLAN_BOXES, R, blk, df, 0});
-        for (int rpu : {256, 512, 1024})
-            for (int df : {3, 4}) cands.push_back({BAAS_PLAN_RANGE, rpu, 1024, df, 0});
-        for (int rpu : {32, 64, 128}) cands.push_back({BAAS_PLAN_SCHEDULE, rpu, 1024, 4, 0});
+        // b staged in LDS where it fits or read from memory) on small
+        // boxes, small units of the merged levels and thin level bands
+        for (int R : {64, 125, 216, 512})
+            for (int df : {3, 4}) cands.push_back({BAAS_PLAN_BOXES, R, 1024, df, 0});
+        for (int rpu : {16, 32, 64}) cands.push_back({BAAS_PLAN_SCHEDULE, rpu, 1024, 4, 0});
     }
+    if (nrhs > 1)
+        for (int S : {1024, 2048})
+            for (int hb : {2, 4}) cands.push_back({BAAS_PLAN_TILES, S, 1024, 4, hb});
     for (int S : TILE_RANGES) {
         if (S >= 2 * n && S != TILE_RANGES[0]) continue;
         for (int hb : TILE_BANDS)
