@@ -64,19 +64,42 @@ def build(verbose: bool = False) -> str:
 
 
 DEMO = os.path.join(HERE, "bin", "baas_demo")
+ROOT = os.path.dirname(HERE)
+CPP_MAINS = os.path.join(ROOT, "tests", "cpp")  # drivers shaped like the reference's mains
+
+
+def _run(cmd, what):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"{what} failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
 
 
 def build_demo():
-    """The C++ runtime classes (host/) and their driver, linked to the library."""
+    """The C++ runtime classes (host/), the driver baas_demo and the
+    reference-shaped mains under tests/cpp, linked to the library."""
     os.makedirs(os.path.dirname(DEMO), exist_ok=True)
     host = os.path.join(HERE, "host")
-    cmd = [_hipcc(), "-O2", "-std=c++17", "-Wno-unused-result", os.path.join(host, "baas_runtime.cpp"),
-           os.path.join(host, "baas_demo.cpp"), "-L" + HERE, "-lbaas_hip", "-Wl,-rpath,$ORIGIN/..",
-           "-o", DEMO + ".tmp"]
-    r = subprocess.run(cmd, capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"demo build failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
-    os.replace(DEMO + ".tmp", DEMO)
+    os.makedirs(BUILD, exist_ok=True)
+    cflags = ["-O2", "-std=c++17", "-fopenmp", "-Wno-unused-result", "-I" + host]
+    rt_obj = os.path.join(BUILD, "baas_runtime.o")
+    _run([_hipcc()] + cflags + ["-c", os.path.join(host, "baas_runtime.cpp"), "-o", rt_obj], "runtime build")
+    targets = [(os.path.join(host, "baas_demo.cpp"), DEMO)]
+    if os.path.isdir(CPP_MAINS):
+        os.makedirs(os.path.join(CPP_MAINS, "bin"), exist_ok=True)
+        for f in sorted(os.listdir(CPP_MAINS)):
+            if f.endswith(".cpp"):
+                targets.append((os.path.join(CPP_MAINS, f), os.path.join(CPP_MAINS, "bin", f[:-4])))
+
+    def link(t):
+        src, out = t
+        obj = os.path.join(BUILD, "host_" + os.path.basename(src) + ".o")
+        _run([_hipcc()] + cflags + ["-c", src, "-o", obj], "host build")
+        rpath = "$ORIGIN/" + os.path.relpath(HERE, os.path.dirname(out))
+        _run([_hipcc(), "-fopenmp", obj, rt_obj, "-L" + HERE, "-lbaas_hip", "-Wl,-rpath," + rpath, "-o", out + ".tmp"],
+             "host link")
+        os.replace(out + ".tmp", out)
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(targets))) as ex:
+        list(ex.map(link, targets))
 
 
 if __name__ == "__main__":

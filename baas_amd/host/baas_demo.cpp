@@ -200,29 +200,38 @@ int main(int argc, char **argv) {
                     correct[i] = (b[i] - s) / L->values[L->rowptr[i + 1] - 1];
                 }
             }
-            SpTRSVRT::SpTRSV_BAAS rt(L, correct.data(), n, nnz, 1.0, "BAAS_GPU", nthread);
-            rt.setNumTest(num_test);
-            double ms = rt.evaluate(&ok);
+            SPM::CSC cscL(L->m, L->n, nnz);
+            SPM::CSR::transpositionToCSC(L, &cscL);
+            SpTRSVRT::SpTRSV_DUAL_TREE_BFS_WF_Merge_P2P_MapFourRule rt(L, &cscL, correct.data(), n, nnz, 1.0,
+                                                                       "BAAS_GPU", nthread, false);
+            rt.set_num_test(num_test);
+            TimeMeasure t = rt.evaluate();
+            ok = rt.passed();
             int c[8];
             baas_schedule_info(rt.schedule(), c);
             printf("%s,%d,%d,%d,%s,%d,%.4f,%.5f,%d\n", src.c_str(), n, nnz, num_test, "BAAS_GPU", nthread,
-                   rt.getAnalysisTime(), ms, c[1]);
+                   rt.getAnalysisTime(), t.getTime(), c[1]);
             delete L;
         } else if (kind == "ic0" || kind == "ilu0") {
             SPM::CSR *A = is_file ? to_csr(read_mtx(src.c_str(), false)) : to_csr(grid7(side, kind == "ilu0"));
             std::vector<double> correct;
             if (kind == "ic0") ic0_serial(*A, correct);
             else ilu0_serial(*A, correct);
-            double ms;
-            double at;
+            double ms, at;
             if (kind == "ic0") {
-                ICRT::IC_BAAS rt(A, correct.data(), nthread);
-                ms = rt.evaluate(num_test, &ok);
-                at = rt.analysis_time;
+                ICRT::IC_DUAL_TREE_BFS_WF_Merge_P2P_MapFourRule rt(A, correct.data(), A->m, A->getNnz(),
+                                                                   "IC0_BAAS_GPU", nthread);
+                rt.set_num_test(num_test);
+                ms = rt.evaluate().getTime();
+                at = rt.getAnalysisTime();
+                ok = rt.passed();
             } else {
-                ILURT::ILU_BAAS rt(A, correct.data(), nthread);
-                ms = rt.evaluate(num_test, &ok);
-                at = rt.analysis_time;
+                ILURT::ILU_DUAL_TREE_BFS_WF_Merge_P2P_MapFourRule rt(A, correct.data(), A->m, A->getNnz(),
+                                                                     "ILU0_BAAS_GPU", nthread);
+                rt.set_num_test(num_test);
+                ms = rt.evaluate().getTime();
+                at = rt.getAnalysisTime();
+                ok = rt.passed();
             }
             printf("%s,%d,%d,%d,%s,%d,%.4f,%.5f,%d\n", src.c_str(), A->m, A->getNnz(), num_test,
                    kind == "ic0" ? "IC0_BAAS_GPU" : "ILU0_BAAS_GPU", nthread, at, ms, 0);
@@ -232,7 +241,8 @@ int main(int argc, char **argv) {
             SPM::CSR *A = is_file ? to_csr(read_mtx(src.c_str(), false)) : to_csr(grid7(side, false));
             std::vector<double> correct;
             ic0_serial(*A, correct);
-            Schedule::Scheduler sc(A, nthread);
+            A->constructDiagPtr();
+            Schedule::Scheduler sc(A, true, false, Schedule::P2P_RULE, nthread);
             sc.preprocessing();
             std::vector<double> lu(A->getNnz());
             sc.runIC0(A, lu.data());
@@ -246,7 +256,7 @@ int main(int argc, char **argv) {
             for (int i = 0; i < A->m; i++)
                 for (int j = A->rowptr[i]; j <= A->diagptr[i]; j++) Lc->values[p++] = lu[j];
             SPM::CSC Ct;
-            SPM::transpositionToCSC(Lc, &Ct);
+            SPM::CSR::transpositionToCSC(Lc, &Ct);
             SPM::CSR U(A->m, A->n, Ct.colptr, Ct.rowidx, Ct.values);
             std::vector<double> b(A->m, 1.0), y(A->m), z(A->m), t(A->m, 0.0), r(A->m, 0.0);
             sc.runForwardTRSVNoPerm(Lc, y.data(), b.data());

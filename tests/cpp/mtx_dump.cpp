@@ -1,0 +1,28 @@
+// Loads a Matrix Market file with the runtime's SPM::CSR loaders (full:
+// CSR(file); lower with a unit diagonal: CSR(file, true, 0)) and writes the
+// arrays as raw little-endian files <prefix>.{full,lower}.{rowptr,colidx,values}
+// so a test can compare them with the reference loader's digests.  No GPU.
+#include <cstdio>
+#include <fstream>
+#include <string>
+
+#include "baas_runtime.hpp"
+
+static void dump(const SPM::CSR &A, const std::string &p) {
+    std::ofstream(p + ".rowptr", std::ios::binary).write((const char *)A.rowptr, sizeof(int) * (A.m + 1));
+    std::ofstream(p + ".colidx", std::ios::binary).write((const char *)A.colidx, sizeof(int) * A.getNnz());
+    std::ofstream(p + ".values", std::ios::binary).write((const char *)A.values, sizeof(double) * A.getNnz());
+}
+
+int main(int argc, char *argv[]) {
+    if (argc < 3) {
+        fprintf(stderr, "usage: %s matrix.mtx out-prefix\n", argv[0]);
+        return 2;
+    }
+    SPM::CSR full(argv[1]);
+    dump(full, std::string(argv[2]) + ".full");
+    SPM::CSR lower(argv[1], true, 0);
+    dump(lower, std::string(argv[2]) + ".lower");
+    printf("full %d x %d, %d nnz; lower %d nnz\n", full.m, full.n, full.getNnz(), lower.getNnz());
+    return 0;
+}
