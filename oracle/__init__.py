@@ -65,6 +65,8 @@ _sig(lib, "orc_sptrsv_lower_f32", None, [_i, _I, _I, _F, _F, _F])
 _sig(lib, "orc_sptrsv_upper_desc_f32", None, [_i, _I, _I, _F, _F, _F])
 _sig(lib, "orc_sptrsv_lower_levels_omp", None, [_i, _I, _I, _D, _D, _D, _i, _I, _I, _i])
 _sig(lib, "orc_sptrsv_lower_multi", None, [_i, _I, _I, _D, _i, _D, _D])
+_sig(lib, "orc_sptrsv_lower_p2p_omp", None, [_i, _I, _I, _D, _D, _D, _i, _I, _I, _I, _I, _i, _I, _i])
+_sig(lib, "omp_max_threads", _i, [])
 _sig(lib, "orc_ic0", None, [_i, _I, _I, _I, _D, _D])
 _sig(lib, "orc_ilu0", None, [_i, _I, _I, _I, _D])
 _sig(lib, "orc_ic0_split", None, [_i, _I, _I, _I, _D, _I, _I, _D, _I, _I, _D])
@@ -196,6 +198,39 @@ def sptrsv_upper_f32(U, b):
                                   np.ascontiguousarray(U.values, np.float32),
                                   np.ascontiguousarray(b, np.float32), x)
     return x
+
+
+class P2PSolver:
+    """The point-to-point OpenMP lower solve with its ready flags kept
+    between calls (one epoch per call, so nothing is reset)."""
+
+    def __init__(self, L, nthreads):
+        self.L = L
+        self.lp, ntl, self.pto = levelset_rows(L.n, L.rowptr, L.colidx)
+        self.lvl = _i32(ntl - 1)
+        self.nthreads = max(1, int(nthreads))
+        # the thread whose chunk of its level holds each row
+        cnt = np.diff(self.lp)
+        chunk = (cnt + self.nthreads - 1) // self.nthreads
+        q = np.arange(L.n)
+        lq = np.repeat(np.arange(len(cnt)), cnt)
+        self.own = np.empty(L.n, np.int32)
+        self.own[self.pto] = ((q - self.lp[:-1][lq]) // chunk[lq]).astype(np.int32)
+        self.flags = np.zeros((len(self.lp) - 1) * self.nthreads, np.int32)
+        self.epoch = 0
+        self.rp, self.ci, self.va = _i32(L.rowptr), _i32(L.colidx), _f64(L.values)
+
+    def solve(self, b, x=None):
+        if x is None:
+            x = np.zeros(self.L.n)
+        self.epoch += 1
+        lib.orc_sptrsv_lower_p2p_omp(self.L.n, self.rp, self.ci, self.va, _f64(b), x, len(self.lp) - 1, self.lp,
+                                     self.pto, self.lvl, self.own, self.nthreads, self.flags, self.epoch)
+        return x
+
+
+def omp_max_threads():
+    return lib.omp_max_threads()
 
 
 def sptrsv_lower_multi(L, B):

@@ -470,6 +470,71 @@ void orc_sptrsv_lower_levels_omp(int n, const int *rowptr, const int *colidx, co
     }
 }
 
+/* OpenMP threads the next parallel region would use (the CPU baseline
+ * reports it). */
+int omp_max_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+/* Point-to-point OpenMP lower solve, the synchronisation of the BAAS P2P
+ * kernel (merge/SpTRSV.cpp:421-478): each thread walks its own tasks in
+ * schedule order, waits on the ready flags of the tasks its rows depend on,
+ * solves them and publishes its own flag; there is no barrier.  A task is a
+ * thread's contiguous chunk of one row level (the reference's FourRule
+ * mapping of coarsened groups to threads, merge/LevelTask.cpp:2295-2684, is
+ * not restated); flags[l * nthreads + t] marks task (l, t) done with value
+ * epoch, and a thread remembers the highest level it has seen done per
+ * thread, so most parents cost no atomic load.  lvl: 0-based row level,
+ * own: the thread whose task holds the row.  flags (nlevels * nthreads ints)
+ * stay with the caller, who passes a new epoch per call, so nothing is
+ * reset.  Same per-row arithmetic as orc_sptrsv_lower: results are
+ * bit-identical to it.  Pinned: M (equals the serial solve at every thread
+ * count). */
+void orc_sptrsv_lower_p2p_omp(int n, const int *rowptr, const int *colidx, const double *val,
+                              const double *b, double *x, int nlevels, const int *level_ptr,
+                              const int *perm_to_orig, const int *lvl, const int *own, int nthreads,
+                              int *flags, int epoch) {
+    (void)n;
+    if (nthreads < 1) nthreads = 1;
+#pragma omp parallel num_threads(nthreads)
+    {
+#ifdef _OPENMP
+        const int tid = omp_get_thread_num(), nt = omp_get_num_threads();
+#else
+        const int tid = 0, nt = 1;
+#endif
+        int known[256];
+        for (int t = 0; t < 256; t++) known[t] = -1;
+        if (nt <= 256) {
+            for (int l = 0; l < nlevels; l++) {
+                const int q0 = level_ptr[l], cnt = level_ptr[l + 1] - q0;
+                const int chunk = (cnt + nt - 1) / nt;
+                const int qa = q0 + tid * chunk < q0 + cnt ? q0 + tid * chunk : q0 + cnt;
+                const int qb = qa + chunk < q0 + cnt ? qa + chunk : q0 + cnt;
+                for (int q = qa; q < qb; q++) {
+                    const int i = perm_to_orig[q];
+                    double s = 0.0;
+                    for (int j = rowptr[i]; j < rowptr[i + 1] - 1; j++) {
+                        const int c = colidx[j], lc = lvl[c], owner = own[c];
+                        if (owner != tid && known[owner] < lc) {
+                            while (__atomic_load_n(&flags[lc * nt + owner], __ATOMIC_ACQUIRE) != epoch) {
+                            }
+                            known[owner] = lc;
+                        }
+                        s = fma(val[j], x[c], s);
+                    }
+                    x[i] = (b[i] - s) / val[rowptr[i + 1] - 1];
+                }
+                __atomic_store_n(&flags[l * nt + tid], epoch, __ATOMIC_RELEASE);
+            }
+        }
+    }
+}
+
 /* k right-hand sides, row-major b[i*k + r]; each column is the serial solve.
  * Pinned: M (column-by-column equals orc_sptrsv_lower). */
 void orc_sptrsv_lower_multi(int n, const int *rowptr, const int *colidx, const double *val,

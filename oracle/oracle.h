@@ -75,6 +75,11 @@ void orc_sptrsv_upper_desc_f32(int n, const int *rowptr, const int *colidx, cons
 void orc_sptrsv_lower_levels_omp(int n, const int *rowptr, const int *colidx, const double *val,
                                  const double *b, double *x, int nlevels, const int *level_ptr,
                                  const int *perm_to_orig, int nthreads);
+int omp_max_threads(void);
+void orc_sptrsv_lower_p2p_omp(int n, const int *rowptr, const int *colidx, const double *val,
+                              const double *b, double *x, int nlevels, const int *level_ptr,
+                              const int *perm_to_orig, const int *lvl, const int *own, int nthreads,
+                              int *flags, int epoch);
 void orc_sptrsv_lower_multi(int n, const int *rowptr, const int *colidx, const double *val,
                             int k, const double *b, double *x);
 void orc_ic0(int n, const int *rowptr, const int *colidx, const int *diagptr,

@@ -523,3 +523,16 @@ def test_iccg_adjoint_fd_reference_48():
     off = cols != rows
     wrong = -(lam[cols] * x[rows])
     assert np.any(np.abs(fd_a[off] - wrong[off]) > 1e-6 * np.abs(wrong[off]))
+
+
+@pytest.mark.parametrize("name", ["C1_lap2d_128", "C3_knn", "bcsstk38"])
+def test_sptrsv_p2p_omp_equals_serial_bitwise(name):
+    """The point-to-point OpenMP restatement (the CPU baseline's P2P leg)
+    gives the serial solve's bits at every thread count, call after call."""
+    L, _, _ = workload(name)
+    b = g.rhs_uniform(L.n, 2)
+    want = O.sptrsv_lower(L, b)
+    for nt in (1, 2, 3, 8):
+        P = O.P2PSolver(L, nt)
+        for _ in range(3):
+            np.testing.assert_array_equal(P.solve(b), want, err_msg=f"{name} {nt}")
