@@ -311,6 +311,12 @@ __device__ __forceinline__ void icl_factor_rows(const IclArgs &a, double *sl, co
     int nfl = 0, fk[FW], fdj[FW];
     bool fpub[FW];
     double fav[FW];
+    // DIAG_ONLY: the diagonal entry of a marked row (its lu position, A's
+    // value, publication mark) and its pairs (the LDS slot of the parent's
+    // upper value, the index of the row's own lower entry), at most FW
+    int dk = 0, dnp = 0, uslot[FW], uo[FW];
+    bool dpubd = false;
+    double dav = 0.0;
     auto load_entry = [&]() {
         dp = 0.0;
         if (e < ne) {
@@ -351,6 +357,19 @@ __device__ __forceinline__ void icl_factor_rows(const IclArgs &a, double *sl, co
                 fpub[t] = q.w < 0;
                 fav[t] = leav[eo + (t < nfl ? t : last)];
             }
+            if (DIAG_ONLY) {
+                const int4 q = lent[eo + nfl];
+                dk = q.x;
+                dnp = q.w & 0x7fffffff;
+                dpubd = q.w < 0;
+                dav = leav[eo + nfl];
+#pragma unroll
+                for (int t = 0; t < FW; t++) {
+                    const int w = t < dnp ? lpair[q.z + t] : 0;
+                    uslot[t] = t < dnp ? w >> 8 : base;  // any slot of the unit when unused
+                    uo[t] = w & 255;
+                }
+            }
             if (KIND == BAAS_KIND_ILU0) {
                 for (int t = nfl + 1; t < ne; t++) {
                     const int4 q = lent[eo + t];
@@ -365,6 +384,64 @@ __device__ __forceinline__ void icl_factor_rows(const IclArgs &a, double *sl, co
             }
         }
     };
+    // One entry's step: its pairs, then its pivot (an ILU0 row whose last
+    // entry this completes moves on to its next row in the same step).
+    auto pair_step = [&]() {
+        if (pc < np) {
+            double v[PCH], o[PCH];
+#pragma unroll
+            for (int t = 0; t < PCH; t++) {
+                const int i = ps + (pc + t < np ? pc + t : np - 1);
+                const int w = lpair[i];
+                v[t] = ld_lds(&sl[w >> 8]);
+                o[t] = ld_lds(&sl[base + (w & 255)]);
+            }
+            int took = 0;
+#pragma unroll
+            for (int t = 0; t < PCH; t++) {
+                if (t == took && pc + t < np && (force || !Sentinel<double>::is(v[t]))) {
+                    const double pv = Sentinel<double>::is(v[t]) ? 0.0 : v[t];
+                    dp = KIND == BAAS_KIND_IC0 ? fma(o[t], pv, dp) : fma(-o[t], pv, dp);
+                    took++;
+                }
+            }
+            pc += took;
+        }
+        bool done_entry = false;
+        if (pc == np) {
+            const double dj = !DIAG_ONLY && djs >= 0 ? ld_lds(&sl[djs]) : 1.0;
+            if (force || !Sentinel<double>::is(dj)) {
+                const double d = Sentinel<double>::is(dj) ? 1.0 : dj;
+                double L;
+                if (KIND == BAAS_KIND_IC0) {
+                    L = (leav[eo + e] - dp) / d;
+                } else {
+                    if (!DIAG_ONLY && djs >= 0 && !(fabs(d) > 1.0e-16))
+                        raise_status(a.status, BAAS_ZERO_PIVOT, row);
+                    L = !DIAG_ONLY && djs >= 0 ? dp / d : dp;
+                }
+                st_lds(&sl[base + e], L);
+                lu[k] = L;
+                if (epub) {
+                    st_agent(&pub[k], L);
+                    nxt[k] = Sentinel<double>::value();
+                }
+                // a marked ILU0 row comes here for its diagonal only
+                e = KIND == BAAS_KIND_ILU0 && fast ? ne : e + 1;
+                load_entry();
+                done_entry = true;
+                if (KIND == BAAS_KIND_ILU0 && e >= ne) {
+                    p += stride;  // every entry of the row, diagonal included, is done
+                    have = p < nk;
+                    if (have) start();
+                }
+            }
+        }
+        if (!done_entry && !force && ++spins > a.spin_limit) {
+            raise_status(a.status, BAAS_TIMEOUT, row);
+            force = true;  // finish this entry with what has arrived
+        }
+    };
     if (have) start();
     // one loop with a wavefront-uniform exit and a single divergent step;
     // every path of a step ends at the one back edge (a second one lets the
@@ -375,22 +452,34 @@ __device__ __forceinline__ void icl_factor_rows(const IclArgs &a, double *sl, co
         if (fast && e < nfl) {
             // every lower entry needs only its pivot: read them all, and when
             // all are there factor the entries in order
-            double d[FW];
+            double d[FW], uv[FW];
 #pragma unroll
             for (int t = 0; t < FW; t++) d[t] = ld_lds(&sl[fdj[t]]);
-            bool ready = true;
+            bool ready = true, uready = true;
 #pragma unroll
             for (int t = 0; t < FW; t++)
                 if (t < nfl && Sentinel<double>::is(d[t])) ready = false;
+            if (DIAG_ONLY) {
+                // the parents' upper values for the diagonal, published when
+                // those rows started: normally there before their pivots
+#pragma unroll
+                for (int t = 0; t < FW; t++) uv[t] = ld_lds(&sl[uslot[t]]);
+#pragma unroll
+                for (int t = 0; t < FW; t++)
+                    if (t < dnp && Sentinel<double>::is(uv[t])) uready = false;
+            }
             if (ready || force) {
+                double Lv[FW];
 #pragma unroll
                 for (int t = 0; t < FW; t++) {
+                    Lv[t] = 0.0;
                     if (t < nfl) {
                         const double dd = Sentinel<double>::is(d[t]) ? 1.0 : d[t];
                         if (KIND == BAAS_KIND_ILU0 && !(fabs(dd) > 1.0e-16))
                             raise_status(a.status, BAAS_ZERO_PIVOT, row);
                         // IC0 subtracts a zero dot, ILU0 divides A's value
                         const double L = (KIND == BAAS_KIND_IC0 ? fav[t] - 0.0 : fav[t]) / dd;
+                        Lv[t] = L;
                         st_lds(&sl[base + t], L);
                         lu[fk[t]] = L;
                         if (fpub[t]) {
@@ -399,62 +488,62 @@ __device__ __forceinline__ void icl_factor_rows(const IclArgs &a, double *sl, co
                         }
                     }
                 }
-                e = nfl;  // the diagonal follows in the next step
-                load_entry();
+                if (KIND == BAAS_KIND_IC0) {
+                    // the diagonal in the same step, from the entries just
+                    // computed: at most FW <= 3 of them, so the self dot is
+                    // its FMA tail alone, in order (as in the step below),
+                    // then the pivot rule of merge/IC.cpp:82-87
+                    double r = 0.0;
+#pragma unroll
+                    for (int t = 0; t < FW; t++)
+                        if (t < nfl) r = fma(Lv[t], Lv[t], r);
+                    double dg = aii;
+                    if (aii - r > 0) {
+                        dg = sqrt(aii - r);
+                    } else {
+                        raise_status(a.status, BAAS_NONPOSITIVE_PIVOT, row);
+                    }
+                    st_lds(&sl[base + ne], dg);
+                    lu[dgp] = dg;
+                    if (dpub) {
+                        st_agent(&pub[dgp], dg);
+                        nxt[dgp] = Sentinel<double>::value();
+                    }
+                    p += stride;
+                    have = p < nk;
+                    if (have) start();
+                } else if (DIAG_ONLY && (uready || force)) {
+                    // the diagonal in the same step: A's value less the
+                    // pairs' products in order (as in pair_step), no division
+                    double dd = dav;
+#pragma unroll
+                    for (int t = 0; t < FW; t++) {
+                        if (t < dnp) {
+                            const double o = uo[t] == 0 ? Lv[0] : (FW > 1 && uo[t] == 1) ? Lv[1 < FW ? 1 : 0]
+                                                                                        : Lv[FW - 1];
+                            const double pv = Sentinel<double>::is(uv[t]) ? 0.0 : uv[t];
+                            dd = fma(-o, pv, dd);
+                        }
+                    }
+                    st_lds(&sl[base + nfl], dd);
+                    lu[dk] = dd;
+                    if (dpubd) {
+                        st_agent(&pub[dk], dd);
+                        nxt[dk] = Sentinel<double>::value();
+                    }
+                    p += stride;
+                    have = p < nk;
+                    if (have) start();
+                } else {
+                    e = nfl;  // the diagonal follows in the next step
+                    load_entry();
+                }
             } else if (!force && ++spins > a.spin_limit) {
                 raise_status(a.status, BAAS_TIMEOUT, row);
                 force = true;
             }
         } else if ((!ALLF || KIND == BAAS_KIND_ILU0) && e < ne) {
-            if (pc < np) {
-                double v[PCH], o[PCH];
-#pragma unroll
-                for (int t = 0; t < PCH; t++) {
-                    const int i = ps + (pc + t < np ? pc + t : np - 1);
-                    const int w = lpair[i];
-                    v[t] = ld_lds(&sl[w >> 8]);
-                    o[t] = ld_lds(&sl[base + (w & 255)]);
-                }
-                int took = 0;
-#pragma unroll
-                for (int t = 0; t < PCH; t++) {
-                    if (t == took && pc + t < np && (force || !Sentinel<double>::is(v[t]))) {
-                        const double pv = Sentinel<double>::is(v[t]) ? 0.0 : v[t];
-                        dp = KIND == BAAS_KIND_IC0 ? fma(o[t], pv, dp) : fma(-o[t], pv, dp);
-                        took++;
-                    }
-                }
-                pc += took;
-            }
-            bool done_entry = false;
-            if (pc == np) {
-                const double dj = !DIAG_ONLY && djs >= 0 ? ld_lds(&sl[djs]) : 1.0;
-                if (force || !Sentinel<double>::is(dj)) {
-                    const double d = Sentinel<double>::is(dj) ? 1.0 : dj;
-                    double L;
-                    if (KIND == BAAS_KIND_IC0) {
-                        L = (leav[eo + e] - dp) / d;
-                    } else {
-                        if (!DIAG_ONLY && djs >= 0 && !(fabs(d) > 1.0e-16))
-                            raise_status(a.status, BAAS_ZERO_PIVOT, row);
-                        L = !DIAG_ONLY && djs >= 0 ? dp / d : dp;
-                    }
-                    st_lds(&sl[base + e], L);
-                    lu[k] = L;
-                    if (epub) {
-                        st_agent(&pub[k], L);
-                        nxt[k] = Sentinel<double>::value();
-                    }
-                    // a marked ILU0 row comes here for its diagonal only
-                    e = KIND == BAAS_KIND_ILU0 && fast ? ne : e + 1;
-                    load_entry();
-                    done_entry = true;
-                }
-            }
-            if (!done_entry && !force && ++spins > a.spin_limit) {
-                raise_status(a.status, BAAS_TIMEOUT, row);
-                force = true;  // finish this entry with what has arrived
-            }
+            pair_step();
         } else if (KIND == BAAS_KIND_ILU0) {
             p += stride;  // every entry of the row, diagonal included, is done
             have = p < nk;

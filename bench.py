@@ -1,6 +1,7 @@
 """Benchmark: FP64 SpTRSV on the 64^3 7-point Laplacian (one MI355X).
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--no-cpu] [--quick]
+                    [--plan MODE:ROWS:BLOCK:DATAFLOW[:BAND]] [--only WORKLOAD]
 
 A step is one forward triangular solve x = L^-1 b with the lower triangle L of
 the 3D 7-point Laplacian on a 64^3 grid in natural ordering (n = 262144,
@@ -15,8 +16,10 @@ Further fields (not the headline): the same solve on the seed-1 random
 symmetric permutation, 32 right-hand sides, IC0 and ILU0 (64^3), ICCG
 time to ||r||/||r0|| < 1e-8 on the kNN stiffness matrix (n = 36000) with
 the FP64 factor and with the FP32 factor, batched ICCG, and rocSPARSE's
-csrsv / csric0 / csrilu0 timed on the same inputs as a vendor-library
-comparator (in place of the reference's MKL baselines).
+csrsv / spsv / csric0 / csrilu0 timed on the same inputs as a vendor-library
+comparator (in place of the reference's MKL baselines).  --only runs one of
+them alone (under a profiler, with --plan fixing its plan so that no
+autotune launches mix into the kernel statistics).
 """
 from __future__ import annotations
 
@@ -140,6 +143,13 @@ def cpu_baseline(seconds):
             "runs": d["runs"]}
 
 
+def parse_plan(text):
+    """MODE:ROWS_PER_UNIT:BLOCK:DATAFLOW[:BAND] -> the plan dict autotune returns."""
+    f = [int(v) for v in text.split(":")]
+    return {"mode": f[0], "rows_per_unit": f[1], "block": f[2], "dataflow": f[3],
+            "band": f[4] if len(f) > 4 else 16, "us": None}
+
+
 def apply_plan(s, plan):
     """Re-pack schedule s into the plan a previous autotune (or --plan) chose."""
     if plan["mode"] == 4:
@@ -149,8 +159,8 @@ def apply_plan(s, plan):
 
 
 def rocsparse_comparators(L, b, x_ours, A, lu_ours, Cd, luc_ours, steps=50):
-    """rocSPARSE on the same inputs, timed as a black box: csrsv (FP64 lower
-    solve, analysis timed apart), csric0 on A and csrilu0 on the
+    """rocSPARSE on the same inputs, timed as a black box: csrsv and the
+    generic spsv (FP64 lower solve, analysis timed apart), csric0 on A and csrilu0 on the
     convection-diffusion matrix.  Its summation order is its own, so the
     largest relative difference from this library's (bit-exact) result is
     reported, not an equality.  Factorisations overwrite their input, so each
@@ -213,6 +223,34 @@ def rocsparse_comparators(L, b, x_ours, A, lu_ours, Cd, luc_ours, steps=50):
         by = bytes_sptrsv(L.n, L.nnz)
         out["csrsv_lap3d64"] = {"ms": round(ms, 4), "GB/s": round(by / (ms * 1e-3) / 1e9, 2),
                                 "analysis_s": round(t_an, 4), "max_rel_diff": reldiff(y, x_ours)}
+        # the generic interface (rocsparse_spsv, default algorithm) on the same solve
+        I64, F64, I32T, BASE0 = C.c_int64, 152, 2, 0
+        mat, vx, vy, bsz = P(), P(), P(), C.c_size_t()
+        chk(rs.rocsparse_create_csr_descr(C.byref(mat), I64(L.n), I64(L.n), I64(L.nnz), dptr(rp), dptr(ci),
+                                          dptr(lv), I(I32T), I(I32T), I(BASE0), I(F64)), "create_csr_descr")
+        objs.append(("rocsparse_destroy_spmat_descr", mat))
+        fill, diag = C.c_int(0), C.c_int(0)  # lower, non-unit
+        chk(rs.rocsparse_spmat_set_attribute(mat, I(0), C.byref(fill), C.c_size_t(4)), "fill attribute")
+        chk(rs.rocsparse_spmat_set_attribute(mat, I(1), C.byref(diag), C.c_size_t(4)), "diag attribute")
+        y2 = torch.empty_like(b)
+        chk(rs.rocsparse_create_dnvec_descr(C.byref(vx), I64(L.n), dptr(b), I(F64)), "dnvec x")
+        objs.append(("rocsparse_destroy_dnvec_descr", vx))
+        chk(rs.rocsparse_create_dnvec_descr(C.byref(vy), I64(L.n), dptr(y2), I(F64)), "dnvec y")
+        objs.append(("rocsparse_destroy_dnvec_descr", vy))
+
+        def spsv(stage, buf_ptr):
+            chk(rs.rocsparse_spsv(handle, I(NONE), C.byref(alpha), mat, vx, vy, I(F64), I(0), I(stage),
+                                  C.byref(bsz), buf_ptr), f"spsv stage {stage}")
+        spsv(1, None)
+        buf2 = torch.empty(max(bsz.value, 8), dtype=torch.uint8, device="cuda")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        spsv(2, dptr(buf2))
+        torch.cuda.synchronize()
+        t_an2 = time.perf_counter() - t0
+        _, ms2, _ = time_events(lambda: spsv(3, dptr(buf2)), steps, 5)
+        out["spsv_generic_lap3d64"] = {"ms": round(ms2, 4), "GB/s": round(by / (ms2 * 1e-3) / 1e9, 2),
+                                       "analysis_s": round(t_an2, 4), "max_rel_diff": reldiff(y2, x_ours)}
 
         def factor(name, M, ours, fn_pre):
             d2, inf2, sz2 = P(), P(), C.c_size_t()
@@ -268,6 +306,174 @@ def traffic_from_profiles():
     return d.get("sptrsv_lap3d64_bytes_per_launch"), d.get("plan")
 
 
+class Ctx:
+    """What the secondary measurements share: the handle, the 64^3 Laplacian
+    A, its lower triangle L with its analysed schedule, values, b and the
+    headline solve's x, and the headline plan.  `plan_override` (from --plan
+    with --only) replaces a workload's autotune with that plan."""
+
+    def __init__(self, h, A, L, s, lv, bd, x, plan, plan_override=None):
+        self.h, self.A, self.L, self.s, self.lv, self.bd, self.x, self.plan = h, A, L, s, lv, bd, x, plan
+        self.plan_override = plan_override
+        self.sa = self.av = self.lu = self.Cd = self.sc = self.cv = self.luc = None
+
+    def tune(self, sched, fn):
+        """autotune through fn() unless a plan was forced."""
+        if self.plan_override is None:
+            return fn()
+        apply_plan(sched, self.plan_override)
+        return dict(self.plan_override)
+
+
+def x_perm1(c):
+    """The same solve on the seed-1 random symmetric permutation of A."""
+    import torch
+    from baas_amd import matgen as g
+    from baas_amd.api import Schedule
+    P, _ = g.random_symmetric_permutation(c.A, 1)
+    LP = P.lower()
+    sp = Schedule(c.h, LP.rowptr, LP.colidx)
+    lpv = torch.as_tensor(LP.values, device="cuda")
+    pplan = c.tune(sp, lambda: sp.autotune(lpv))
+    _, m, _ = time_events(lambda: sp.solve(lpv, c.bd, c.x), 50, 5)
+    by = bytes_sptrsv(c.L.n, c.L.nnz)
+    return {"ms": round(m, 4), "GB/s": round(by / (m * 1e-3) / 1e9, 2), "plan": pplan}
+
+
+def x_rhs32(c):
+    """32 right-hand sides (row-major n x 32, U(-1,1) seed 1), natural ordering."""
+    import torch
+    from baas_amd import matgen as g
+    B = torch.as_tensor(g.rhs_uniform(c.L.n, 1, k=32), device="cuda")
+    X = torch.empty_like(B)
+    kplan = c.tune(c.s, lambda: c.s.autotune(c.lv, nrhs=32))
+    _, m, _ = time_events(lambda: c.s.solve(c.lv, B, X), 20, 3)
+    byk = bytes_sptrsv(c.L.n, c.L.nnz, k=32)
+    if c.plan_override is None and c.plan is not None:
+        apply_plan(c.s, c.plan)
+    return {"ms": round(m, 4), "GB/s": round(byk / (m * 1e-3) / 1e9, 2),
+            "frac_of_peak": round(byk / (m * 1e-3) / 1e9 / HBM_PEAK_GBS, 5), "bytes": byk, "plan": kplan}
+
+
+def x_ic0(c):
+    """IC0 of A (the 64^3 Laplacian); algorithmic bytes 20 nnz(L) + 8 n."""
+    import torch
+    from baas_amd import _lib
+    from baas_amd.api import Schedule
+    c.sa = Schedule(c.h, c.A.rowptr, c.A.colidx, kind=_lib.KIND_IC0)
+    c.av = torch.as_tensor(c.A.values, device="cuda")
+    c.lu = torch.empty_like(c.av)
+    fplan = c.tune(c.sa, lambda: c.sa.autotune_factor(c.av))
+    _, m, med = time_events(lambda: c.sa.ic0(c.av, c.lu), 20, 3)
+    c.h.check()
+    by = 20 * c.L.nnz + 8 * c.L.n
+    gbs = by / (m * 1e-3) / 1e9
+    return {"ms": round(m, 4), "median_ms": round(med, 4), "GB/s": round(gbs, 2),
+            "frac_of_peak": round(gbs / HBM_PEAK_GBS, 5), "bytes": by, "plan": fplan}
+
+
+def x_ilu0(c):
+    """ILU0 of the 64^3 convection-diffusion operator; bytes 20 nnz(A) + 8 n."""
+    import torch
+    from baas_amd import _lib
+    from baas_amd import matgen as g
+    from baas_amd.api import Schedule
+    c.Cd = g.convdiff3d(64)
+    c.sc = Schedule(c.h, c.Cd.rowptr, c.Cd.colidx, kind=_lib.KIND_ILU0)
+    c.cv = torch.as_tensor(c.Cd.values, device="cuda")
+    c.luc = torch.empty_like(c.cv)
+    fplan = c.tune(c.sc, lambda: c.sc.autotune_factor(c.cv))
+    _, m, med = time_events(lambda: c.sc.ilu0(c.cv, c.luc), 20, 3)
+    c.h.check()
+    by = 20 * c.Cd.nnz + 8 * c.Cd.n
+    gbs = by / (m * 1e-3) / 1e9
+    return {"ms": round(m, 4), "median_ms": round(med, 4), "GB/s": round(gbs, 2),
+            "frac_of_peak": round(gbs / HBM_PEAK_GBS, 5), "bytes": by, "plan": fplan}
+
+
+def x_rocsparse(c):
+    """The vendor library on the same inputs, checked against this library's results."""
+    if c.sa is None:
+        x_ic0(c)
+    if c.sc is None:
+        x_ilu0(c)
+    apply_plan(c.s, c.plan)
+    c.s.solve(c.lv, c.bd, c.x)
+    c.sa.ic0(c.av, c.lu)
+    c.sc.ilu0(c.cv, c.luc)
+    c.h.check()
+    return rocsparse_comparators(c.L, c.bd, c.x, c.A, c.lu, c.Cd, c.luc)
+
+
+def iccg(c, precisions):
+    """ICCG time to ||r||/||r0|| < 1e-8 on the kNN 3x3-block stiffness matrix
+    (n = 36000), per factor precision (0 FP64, 1 FP32); IC0 timed apart."""
+    import torch
+    from baas_amd import _lib
+    from baas_amd import matgen as g
+    from baas_amd.api import PCG, Schedule
+    K, kb = g.knn_stiffness()
+    sk = Schedule(c.h, K.rowptr, K.colidx, kind=_lib.KIND_IC0)
+    kv = torch.as_tensor(K.values, device="cuda")
+    kbd = torch.as_tensor(kb, device="cuda")
+    res = {}
+    for prec in precisions:
+        name = ("fp64", "fp32_factor")[prec]
+        solver = PCG(c.h, sk, kv, precision=prec)
+        pc_plan = c.tune(sk, solver.autotune)  # plans tuned for this precision's sweeps
+        solver.solve(kbd, tol=1e-8, maxiter=5000)  # warm-up: factor, capture the graph
+        ts, its = [], 0
+        for _ in range(5):
+            xk, its, rel = solver.solve(kbd, tol=1e-8, maxiter=5000)
+            ts.append(solver.times()[1])
+        # IC0 timed on its own: the first refactorisation on a new plan
+        # also builds that plan's factorisation records (host, once)
+        solver.solve(kbd, tol=1e-8, maxiter=5000, refactor=True)
+        ic_first = solver.times()[0]
+        solver.solve(kbd, tol=1e-8, maxiter=5000, refactor=True)
+        t = float(np.median(ts))
+        res[name] = {"iterations": its, "time_to_1e-8_s": round(t, 6), "ms_per_iteration": round(t / its * 1e3, 5),
+                     "ic0_s": round(solver.times()[0], 6), "ic0_first_s": round(ic_first, 6), "relres": rel,
+                     "plan": pc_plan}
+    if len(precisions) == 2:
+        res["speedup_fp32_factor"] = round(res["fp64"]["time_to_1e-8_s"] / res["fp32_factor"]["time_to_1e-8_s"], 3)
+    return res
+
+
+def x_batch(c):
+    """Batched ICCG: 256 2D Laplacians (sides 16..256, seed 3) x 32 RHS."""
+    import torch
+    from baas_amd import matgen as g
+    from baas_amd.api import Batch
+    mats, rhs = g.batch_laplace2d()
+    bt = Batch(c.h, mats, 32)
+    Bt = torch.as_tensor(np.concatenate(rhs, axis=0), device="cuda")
+    del rhs
+    bt.solve(Bt, tol=1e-8, maxiter=3000)  # first call: IC0, plan tuning, graph capture
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    _, iters = bt.solve(Bt, tol=1e-8, maxiter=3000)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    c.h.check()
+    r = {"systems": len(mats), "rows": int(sum(m.n for m in mats)), "rhs": 32, "time_to_1e-8_s": round(dt, 4),
+         "iterations_max": int(iters.max()), "iterations_mean": round(float(iters.mean()), 1), "info": bt.info()}
+    del bt, Bt
+    return r
+
+
+EXTRAS = {"sptrsv_perm1": x_perm1, "sptrsv_32rhs": x_rhs32, "ic0_lap3d64": x_ic0, "ilu0_convdiff64": x_ilu0,
+          "rocsparse": x_rocsparse, "iccg_knn36000": lambda c: iccg(c, (0, 1)),
+          "iccg_knn36000_fp64": lambda c: iccg(c, (0,)), "iccg_knn36000_fp32": lambda c: iccg(c, (1,)),
+          "batch_iccg_256x32": x_batch}
+DEFAULT_EXTRAS = ["sptrsv_perm1", "sptrsv_32rhs", "ic0_lap3d64", "ilu0_convdiff64", "rocsparse", "iccg_knn36000",
+                  "batch_iccg_256x32"]
+
+
+def run_extras(c, names):
+    return {n: EXTRAS[n](c) for n in names}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -278,7 +484,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--plan", default="auto",
                     help="auto (autotune) or MODE:ROWS_PER_UNIT:BLOCK:DATAFLOW[:BAND], MODE 1 = RANGE, "
-                         "2 = SCHEDULE, 3 = LEVELS, 4 = TILES (ROWS_PER_UNIT = row range, BAND = levels)")
+                         "2 = SCHEDULE, 3 = LEVELS, 4 = TILES (ROWS_PER_UNIT = row range, BAND = levels), "
+                         "5 = BOXES; with --only it applies to that workload")
+    ap.add_argument("--only", choices=sorted(EXTRAS), default=None,
+                    help="run one secondary measurement alone (no headline solve) and print its JSON")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
                     help="first run 3 plain copy kernels of this many MiB (HBM counter calibration)")
     ap.add_argument("--cpu-baseline-worker", type=float, default=None, help=argparse.SUPPRESS)
@@ -294,7 +503,7 @@ def main():
 
     from baas_amd import _lib
     from baas_amd import matgen as g
-    from baas_amd.api import PCG, Handle, Schedule
+    from baas_amd.api import Handle, Schedule
 
     torch.cuda.set_device(0)
     if args.calib_copy_mib:
@@ -308,19 +517,30 @@ def main():
     A = g.laplace3d(64)
     L = A.lower()
     b = g.rhs_uniform(L.n, 0)
+    bd = torch.as_tensor(b, device="cuda")
+    x = torch.empty_like(bd)
+    forced = None if args.plan == "auto" else parse_plan(args.plan)
+    if args.only:
+        # one secondary workload; the headline schedule only where it needs it
+        c = Ctx(h, A, L, None, None, bd, x, None, forced)
+        if args.only in ("sptrsv_32rhs", "rocsparse"):
+            c.s = Schedule(h, L.rowptr, L.colidx, kind=_lib.KIND_TRSV)
+            c.lv = torch.as_tensor(L.values, device="cuda")
+            if args.only == "rocsparse":
+                c.plan, c.plan_override = c.s.autotune(c.lv), None
+        r = EXTRAS[args.only](c)
+        h.check()
+        print(json.dumps({"only": args.only, args.only: r}))
+        return
     t0 = time.perf_counter()
     s = Schedule(h, L.rowptr, L.colidx, kind=_lib.KIND_TRSV)
     t_an = time.perf_counter() - t0
     lv = torch.as_tensor(L.values, device="cuda")
-    bd = torch.as_tensor(b, device="cuda")
-    x = torch.empty_like(bd)
     t0 = time.perf_counter()
-    if args.plan == "auto":
+    if forced is None:
         plan = s.autotune(lv)
     else:
-        f = [int(v) for v in args.plan.split(":")]
-        plan = {"mode": f[0], "rows_per_unit": f[1], "block": f[2], "dataflow": f[3],
-                "band": f[4] if len(f) > 4 else 16, "us": None}
+        plan = forced
         apply_plan(s, plan)
     t_tune = time.perf_counter() - t0
     print(f"[bench] analysis {t_an:.3f} s, autotune {t_tune:.2f} s, plan {plan}", file=sys.stderr, flush=True)
@@ -357,91 +577,7 @@ def main():
                      "median_launch_ms": round(med_ms, 5)},
     }
     if not args.quick:
-        extra = {}
-        # seed-1 random symmetric permutation of the same matrix
-        P, _ = g.random_symmetric_permutation(A, 1)
-        LP = P.lower()
-        sp = Schedule(h, LP.rowptr, LP.colidx)
-        lpv = torch.as_tensor(LP.values, device="cuda")
-        pplan = sp.autotune(lpv)
-        _, m, _ = time_events(lambda: sp.solve(lpv, bd, x), 50, 5)
-        extra["sptrsv_perm1"] = {"ms": round(m, 4), "GB/s": round(by / (m * 1e-3) / 1e9, 2), "plan": pplan}
-        # 32 right-hand sides, natural ordering
-        B = torch.as_tensor(g.rhs_uniform(L.n, 1, k=32), device="cuda")
-        X = torch.empty_like(B)
-        kplan = s.autotune(lv, nrhs=32)
-        _, m, _ = time_events(lambda: s.solve(lv, B, X), 20, 3)
-        byk = bytes_sptrsv(L.n, L.nnz, k=32)
-        extra["sptrsv_32rhs"] = {"ms": round(m, 4), "GB/s": round(byk / (m * 1e-3) / 1e9, 2),
-                                 "frac_of_peak": round(byk / (m * 1e-3) / 1e9 / HBM_PEAK_GBS, 5), "plan": kplan}
-        # IC0 (64^3 Laplacian) and ILU0 (64^3 convection-diffusion)
-        sa = Schedule(h, A.rowptr, A.colidx, kind=_lib.KIND_IC0)
-        av = torch.as_tensor(A.values, device="cuda")
-        lu = torch.empty_like(av)
-        fplan = sa.autotune_factor(av)
-        _, m, _ = time_events(lambda: sa.ic0(av, lu), 20, 3)
-        h.check()
-        nl = L.nnz
-        extra["ic0_lap3d64"] = {"ms": round(m, 4), "GB/s": round((20 * nl + 8 * L.n) / (m * 1e-3) / 1e9, 2),
-                                "plan": fplan}
-        Cd = g.convdiff3d(64)
-        sc = Schedule(h, Cd.rowptr, Cd.colidx, kind=_lib.KIND_ILU0)
-        cv = torch.as_tensor(Cd.values, device="cuda")
-        luc = torch.empty_like(cv)
-        fplan = sc.autotune_factor(cv)
-        _, m, _ = time_events(lambda: sc.ilu0(cv, luc), 20, 3)
-        h.check()
-        extra["ilu0_convdiff64"] = {"ms": round(m, 4),
-                                    "GB/s": round((20 * Cd.nnz + 8 * Cd.n) / (m * 1e-3) / 1e9, 2), "plan": fplan}
-        # the vendor library on the same inputs (headline plan restored first)
-        apply_plan(s, plan)
-        s.solve(lv, bd, x)
-        sa.ic0(av, lu)
-        sc.ilu0(cv, luc)
-        h.check()
-        extra["rocsparse"] = rocsparse_comparators(L, bd, x, A, lu, Cd, luc)
-        # ICCG time to 1e-8, FP64 factor vs FP32 factor (kNN 3x3-block stiffness, n = 36000)
-        K, kb = g.knn_stiffness()
-        sk = Schedule(h, K.rowptr, K.colidx, kind=_lib.KIND_IC0)
-        kv = torch.as_tensor(K.values, device="cuda")
-        kbd = torch.as_tensor(kb, device="cuda")
-        res = {}
-        for prec, name in ((0, "fp64"), (1, "fp32_factor")):
-            solver = PCG(h, sk, kv, precision=prec)
-            pc_plan = solver.autotune()  # plans tuned for this precision's sweeps
-            solver.solve(kbd, tol=1e-8, maxiter=5000)  # warm-up: factor, capture the graph
-            ts, its = [], 0
-            for _ in range(5):
-                xk, its, rel = solver.solve(kbd, tol=1e-8, maxiter=5000)
-                ts.append(solver.times()[1])
-            # IC0 timed on its own: the first refactorisation on a new plan
-            # also builds that plan's factorisation records (host, once)
-            solver.solve(kbd, tol=1e-8, maxiter=5000, refactor=True)
-            ic_first = solver.times()[0]
-            solver.solve(kbd, tol=1e-8, maxiter=5000, refactor=True)
-            res[name] = {"iterations": its, "time_to_1e-8_s": round(float(np.median(ts)), 6),
-                         "ic0_s": round(solver.times()[0], 6), "ic0_first_s": round(ic_first, 6), "relres": rel,
-                         "plan": pc_plan}
-        res["speedup_fp32_factor"] = round(res["fp64"]["time_to_1e-8_s"] / res["fp32_factor"]["time_to_1e-8_s"], 3)
-        extra["iccg_knn36000"] = res
-        # batched ICCG: 256 2D Laplacians (sides 16..256, seed 3) x 32 RHS
-        from baas_amd.api import Batch
-        mats, rhs = g.batch_laplace2d()
-        bt = Batch(h, mats, 32)
-        Bt = torch.as_tensor(np.concatenate(rhs, axis=0), device="cuda")
-        del rhs
-        bt.solve(Bt, tol=1e-8, maxiter=3000)  # first call: IC0, plan tuning, graph capture
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        _, iters = bt.solve(Bt, tol=1e-8, maxiter=3000)
-        torch.cuda.synchronize()
-        dt = time.perf_counter() - t0
-        h.check()
-        extra["batch_iccg_256x32"] = {"systems": len(mats), "rows": int(sum(m.n for m in mats)), "rhs": 32,
-                                      "time_to_1e-8_s": round(dt, 4), "iterations_max": int(iters.max()),
-                                      "iterations_mean": round(float(iters.mean()), 1), "info": bt.info()}
-        del bt, Bt
-        out["extra"] = extra
+        out["extra"] = run_extras(Ctx(h, A, L, s, lv, bd, x, plan), DEFAULT_EXTRAS)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(out))
