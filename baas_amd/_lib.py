@@ -19,7 +19,7 @@ PLAN_AUTO, PLAN_RANGE, PLAN_SCHEDULE, PLAN_LEVELS, PLAN_TILES, PLAN_BOXES = 0, 1
 EXPORTS = {
     "group_ptr": 0, "group_set": 1, "level_ptr": 2, "perm_to_orig": 3, "node_to_level": 4,
     "nthreads": 5, "merged_ptr": 6, "row_level_ptr": 7, "row_perm": 8, "row_level": 9,
-    "reduced_ptr": 10, "reduced_set": 11, "unit_ptr": 12, "unit_rows": 13,
+    "reduced_ptr": 10, "reduced_set": 11, "unit_ptr": 12, "unit_rows": 13, "group_cost": 14,
 }
 
 

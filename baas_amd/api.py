@@ -132,6 +132,14 @@ class Schedule:
         L.call("baas_schedule_export", self._s, code, C.c_void_p(out.ctypes.data), C.byref(n))
         return out
 
+    def unit_order(self) -> np.ndarray:
+        """The forward execution plan's row order (work units in claim
+        order, each unit's rows by sub-level): a topological order in which
+        renumbering the matrix (matgen.symmetric_permute) makes every unit's
+        rows consecutive -- the GPU counterpart of the reference's
+        thread-contiguous renumbering (test/SpTRSVRuntime.hpp:751-772)."""
+        return self.export("unit_rows")
+
     def schedule_dict(self):
         i = self.info()
         d = {k: self.export(k) for k in ("group_ptr", "group_set", "level_ptr", "perm_to_orig",

@@ -206,26 +206,44 @@ __global__ void k_group_of_slot(int ng, const int *gptr, int *gid) {
     for (int q = gptr[g]; q < gptr[g + 1]; q++) gid[q] = g;
 }
 
-// threads per level, merge/LevelMerge.cpp:118-262
-__global__ void k_nthreads(int kind, int nlevels, const int *lptr, const int *pto, const int *gptr,
-                           const int *gset, const int *rowptr, int ncore, int *nt) {
-    int l = blockIdx.x * blockDim.x + threadIdx.x;
+// threads per level, merge/LevelMerge.cpp:118-262: one workgroup per level
+// sums its groups' nodes (and, for TRSV, their rows' nonzeros), then applies
+// the rule
+constexpr int NT_BLOCK = 256;
+__global__ __launch_bounds__(NT_BLOCK) void k_nthreads(int kind, int nlevels, const int *lptr, const int *pto,
+                                                       const int *gptr, const int *gset, const int *rowptr,
+                                                       int ncore, int *nt) {
+    const int l = blockIdx.x;
     if (l >= nlevels) return;
-    int groups = lptr[l + 1] - lptr[l];
+    const int groups = lptr[l + 1] - lptr[l];
     if (nlevels == 1) {
-        nt[0] = groups < ncore ? groups : ncore;
+        if (threadIdx.x == 0) nt[0] = groups < ncore ? groups : ncore;
         return;
     }
     long long nodes = 0, nnz = 0;
-    for (int q = lptr[l]; q < lptr[l + 1]; q++) {
-        int g = pto[q];
+    for (int q = lptr[l] + threadIdx.x; q < lptr[l + 1]; q += NT_BLOCK) {
+        const int g = pto[q];
         nodes += gptr[g + 1] - gptr[g];
         if (kind == BAAS_KIND_TRSV)
             for (int k = gptr[g]; k < gptr[g + 1]; k++) {
-                int v = gset[k];
+                const int v = gset[k];
                 nnz += rowptr[v + 1] - rowptr[v];
             }
     }
+    __shared__ long long s_nodes[NT_BLOCK], s_nnz[NT_BLOCK];
+    s_nodes[threadIdx.x] = nodes;
+    s_nnz[threadIdx.x] = nnz;
+    __syncthreads();
+    for (int w = NT_BLOCK / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            s_nodes[threadIdx.x] += s_nodes[threadIdx.x + w];
+            s_nnz[threadIdx.x] += s_nnz[threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    nodes = s_nodes[0];
+    nnz = s_nnz[0];
     int t;
     if (kind == BAAS_KIND_TRSV) {
         if (nodes < 16 || groups <= 4 || nnz < 32) {
@@ -248,22 +266,43 @@ __global__ void k_nthreads(int kind, int nlevels, const int *lptr, const int *pt
     nt[l] = t;
 }
 
-// merge runs of 1-thread levels (one thread: the loop is a sequential scan)
-__global__ void k_level_merge(int nlevels, const int *lptr, const int *nt_in, int *tmp, int *mptr,
-                              int *nmerged) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    for (int i = 0; i < nlevels; i++) tmp[i] = nt_in[i];
-    for (int i = 0; i <= nlevels; i++) mptr[i] = 0;
-    int m = 0;
-    for (int i = 1; i < nlevels; i++) {
-        if (tmp[i] == 1 && tmp[i] == tmp[m]) continue;
-        m++;
-        tmp[m] = tmp[i];
-        mptr[m] = lptr[i];
+// Merge of runs of 1-thread levels (merge/LevelMerge.cpp:236-262).  Level i
+// >= 1 starts a merged level unless it and level i-1 both have one thread
+// (a 1-thread level is absorbed exactly when the last kept level has one
+// thread, which is the case exactly when level i-1 has one); a scan of
+// these marks gives each kept level its slot.
+__global__ void k_merge_marks(int nlevels, const int *nt, int *mark) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nlevels) return;
+    mark[i] = i == 0 ? 0 : (nt[i] == 1 && nt[i - 1] == 1 ? 0 : 1);
+}
+// pos = inclusive scan of the marks: kept level i goes to slot pos[i]; the
+// merged count is pos[nlevels-1] + 1 and its end is lptr[nlevels]
+__global__ void k_merge_scatter(int nlevels, const int *lptr, const int *nt, const int *pos, int *mptr,
+                                int *nmerged) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= nlevels) return;
+    if (i == 0) mptr[0] = 0;
+    const bool kept = i > 0 && !(nt[i] == 1 && nt[i - 1] == 1);
+    if (kept) mptr[pos[i]] = lptr[i];
+    if (i == nlevels - 1) {
+        mptr[pos[i] + 1] = lptr[nlevels];
+        *nmerged = pos[i] + 1;
     }
-    m++;
-    mptr[m] = lptr[nlevels];
-    *nmerged = m;
+}
+
+// costComputation (merge/Coarsen.cpp:854-891, the SpTRSV_LL form every
+// runtime calls): a group's cost is the nonzeros of its rows in the lower
+// CSR, summed over the group's nodes
+__global__ void k_group_cost(int ngroups, const int *gptr, const int *gset, const int *rowptr, int *cost) {
+    const int g = blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= ngroups) return;
+    int c = 0;
+    for (int k = gptr[g]; k < gptr[g + 1]; k++) {
+        const int v = gset[k];
+        c += rowptr[v + 1] - rowptr[v];
+    }
+    cost[g] = c;
 }
 
 // lower part of a full matrix: diagptr, lengths, copy (test/ICRuntime.hpp:416-435)
@@ -702,14 +741,19 @@ int run_analysis(baas_schedule *s, const int *thr_rowptr) {
     int nl = 0;
     BAAS_TRY(level_set(ngf, g2p.p, g2s.p, g2e, lp, pto, ntl, &nl, st));
     // threads per level (on group_set before the in-group sort, as the reference)
-    DBuf<int> nt, tmpi, mp, nm;
+    DBuf<int> nt, pos, mp, nm, cost;
     BAAS_TRY(nt.alloc(nl));
-    BAAS_TRY(tmpi.alloc(nl));
+    BAAS_TRY(pos.alloc(nl));
     BAAS_TRY(mp.alloc(nl + 1));
     BAAS_TRY(nm.alloc(1));
-    hipLaunchKernelGGL(k_nthreads, dim3(nb(nl)), dim3(256), 0, st, s->kind, nl, lp.p, pto.p, np_d.p, ns_d.p,
+    hipLaunchKernelGGL(k_nthreads, dim3(nl), dim3(NT_BLOCK), 0, st, s->kind, nl, lp.p, pto.p, np_d.p, ns_d.p,
                        thr_rowptr, s->opts.ncore, nt.p);
-    hipLaunchKernelGGL(k_level_merge, dim3(1), dim3(1), 0, st, nl, lp.p, nt.p, tmpi.p, mp.p, nm.p);
+    hipLaunchKernelGGL(k_merge_marks, dim3(nb(nl)), dim3(256), 0, st, nl, nt.p, pos.p);
+    BAAS_TRY(inclusive_scan_inplace(tmp, pos.p, nl, st));
+    hipLaunchKernelGGL(k_merge_scatter, dim3(nb(nl)), dim3(256), 0, st, nl, lp.p, nt.p, pos.p, mp.p, nm.p);
+    // group costs on the lower pattern (the in-group order does not change them)
+    BAAS_TRY(cost.alloc(ngf));
+    hipLaunchKernelGGL(k_group_cost, dim3(nb(ngf)), dim3(256), 0, st, ngf, np_d.p, ns_d.p, s->l_rowptr.p, cost.p);
     // sort nodes inside each group (orderingGroupSet, test/SpTRSVRuntime.hpp:633-640)
     DBuf<int> gid, gid2, ns2;
     BAAS_TRY(gid.alloc(n));
@@ -737,6 +781,7 @@ int run_analysis(baas_schedule *s, const int *thr_rowptr) {
     BAAS_TRY(to_host(s->node_to_level, ntl.p, ngf, st));
     BAAS_TRY(to_host(s->nthreads, nt.p, nl, st));
     BAAS_TRY(to_host(s->merged_ptr, mp.p, nmerged + 1, st));
+    BAAS_TRY(to_host(s->group_cost, cost.p, ngf, st));
     double total = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     // execution plans
     th = std::chrono::steady_clock::now();
@@ -859,6 +904,7 @@ extern "C" int baas_schedule_export(const baas_schedule *s, int which, int *out,
         case BAAS_EXPORT_REDUCED_SET: v = &s->reduced_set; break;
         case BAAS_EXPORT_UNIT_PTR: v = &s->fwd.h_unit_ptr; break;
         case BAAS_EXPORT_UNIT_ROWS: v = &s->fwd.h_rows; break;
+        case BAAS_EXPORT_GROUP_COST: v = &s->group_cost; break;
         default: return BAAS_INVALID_ARG;
     }
     *len = (long long)v->size();

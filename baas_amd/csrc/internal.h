@@ -219,7 +219,7 @@ struct baas_schedule {
     baas::DBuf<int> l_row_of;                        // row of each L nonzero
     // schedule exports (host copies)
     std::vector<int> group_ptr, group_set, level_ptr, perm_to_orig, node_to_level, nthreads,
-        merged_ptr, row_level_ptr, row_perm, row_level, reduced_ptr, reduced_set;
+        merged_ptr, row_level_ptr, row_perm, row_level, reduced_ptr, reduced_set, group_cost;
     int ngroups = 0, nlevels = 0, nmerged = 0, reduced_edges = 0, nrowlevels = 0;
     // plans (forward over L, backward over U) and ready flags
     baas::Plan fwd, bwd;

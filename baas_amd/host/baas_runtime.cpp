@@ -265,6 +265,51 @@ CSR::~CSR() {
     delete[] idiag;
 }
 
+CSR *CSR::permute(const int *columnPerm, const int *rowInversePerm, bool sort) const {
+    const int nnz = getNnz();
+    CSR *r = new CSR(m, n, nnz);
+    for (int i = 0; i < m; i++) {
+        const int row = rowInversePerm ? rowInversePerm[i] : i;
+        r->rowptr[i + 1] = r->rowptr[i] + (rowptr[row + 1] - rowptr[row]);
+    }
+    std::vector<std::pair<int, double>> ent;
+    for (int i = 0; i < m; i++) {
+        const int row = rowInversePerm ? rowInversePerm[i] : i;
+        const int b = rowptr[row], e = rowptr[row + 1], o = r->rowptr[i];
+        ent.clear();
+        for (int j = b; j < e; j++) ent.emplace_back(columnPerm[colidx[j]], values[j]);
+        if (sort)
+            std::stable_sort(ent.begin(), ent.end(),
+                             [](const std::pair<int, double> &x, const std::pair<int, double> &y) {
+                                 return x.first < y.first;
+                             });
+        for (int k = 0; k < e - b; k++) {
+            r->colidx[o + k] = ent[k].first;
+            r->values[o + k] = ent[k].second;
+        }
+        if (diagptr) {
+            if (!r->diagptr) r->diagptr = new int[m];
+            // the renamed diagonal column of the original row
+            const int dcol = diagptr[row] < e && colidx[diagptr[row]] == row ? columnPerm[row] : -1;
+            r->diagptr[i] = r->rowptr[i + 1];
+            for (int k = 0; k < e - b; k++)
+                if (r->colidx[o + k] == dcol) {
+                    r->diagptr[i] = o + k;
+                    break;
+                }
+        }
+    }
+    if (diag) {
+        r->diag = new double[m];
+        for (int i = 0; i < m; i++) r->diag[i] = diag[rowInversePerm ? rowInversePerm[i] : i];
+    }
+    if (idiag) {
+        r->idiag = new double[m];
+        for (int i = 0; i < m; i++) r->idiag[i] = idiag[rowInversePerm ? rowInversePerm[i] : i];
+    }
+    return r;
+}
+
 void CSR::constructDiagPtr() {
     if (diagptr) return;
     diagptr = new int[m];
@@ -454,13 +499,15 @@ FusionDemo::FusionDemo(SPM::CSR *A, double *cx, int n_, int nnz_, double alpha_,
 }
 
 FusionDemo::~FusionDemo() {
+    delete csrA_perm;
     FREE(x);
     FREE(y);
     dfree(d_b);
     dfree(d_x);
 }
 
-// y_i = sum_j L_ij * alpha, x := 0 (test/FusionDemo.cpp:47-65)
+// y_i = sum_j L_ij * alpha, x := 0 (test/FusionDemo.cpp:47-65); a Perm
+// variant solves on y renumbered like its matrix (test/FusionDemo.cpp:96-112)
 void FusionDemo::setup() {
     for (int i = 0; i < n; i++) {
         double s = 0.0;
@@ -470,12 +517,32 @@ void FusionDemo::setup() {
     }
     if (!d_b) d_b = dalloc<double>(n);
     if (!d_x) d_x = dalloc<double>(n);
-    hip_ok(hipMemcpy(d_b, y, sizeof(double) * n, hipMemcpyHostToDevice), "upload b");
+    if (hasPerm) {
+        std::vector<double> yp(n);
+        for (int i = 0; i < n; i++) yp[i] = y[permToOrig[i]];
+        hip_ok(hipMemcpy(d_b, yp.data(), sizeof(double) * n, hipMemcpyHostToDevice), "upload b");
+    } else {
+        hip_ok(hipMemcpy(d_b, y, sizeof(double) * n, hipMemcpyHostToDevice), "upload b");
+    }
     hip_ok(hipMemset(d_x, 0, sizeof(double) * n), "clear x");
 }
 
+// With hasPerm the matrix is renumbered by the first analysis's execution
+// order (every work unit's rows consecutive: the GPU form of the
+// reference's thread-contiguous groupPerm, test/SpTRSVRuntime.hpp:751-772)
+// and analysed again; both analyses count as analysis time.
 void FusionDemo::buildset() {
     gs.analyze(*csrA, BAAS_KIND_TRSV, num_thread, plan_mode, rows_per_unit, block, dataflow);
+    if (!hasPerm) return;
+    long long len = 0;
+    baas_rt::check(baas_schedule_export(gs.sched, BAAS_EXPORT_UNIT_ROWS, nullptr, &len), "export");
+    permToOrig.assign((size_t)len, 0);
+    baas_rt::check(baas_schedule_export(gs.sched, BAAS_EXPORT_UNIT_ROWS, permToOrig.data(), &len), "export");
+    origToPerm.assign((size_t)n, 0);
+    for (int i = 0; i < n; i++) origToPerm[permToOrig[i]] = i;
+    delete csrA_perm;
+    csrA_perm = csrA->permute(origToPerm.data(), permToOrig.data(), true);
+    gs.analyze(*csrA_perm, BAAS_KIND_TRSV, num_thread, plan_mode, rows_per_unit, block, dataflow);
 }
 
 TimeMeasure FusionDemo::fused_code() {
@@ -490,6 +557,10 @@ TimeMeasure FusionDemo::fused_code() {
 void FusionDemo::testing() {
     baas_rt::check(baas_check(gs.eng.h), "sptrsv kernel");
     hip_ok(hipMemcpy(x, d_x, sizeof(double) * n, hipMemcpyDeviceToHost), "download x");
+    if (hasPerm) {  // back to the original numbering (test/FusionDemo.cpp:127-137)
+        std::vector<double> xp(x, x + n);
+        for (int i = 0; i < n; i++) x[permToOrig[i]] = xp[i];
+    }
     passed_ = true;
     if (correct_x == nullptr) return;
     for (int i = 0; i < n; i++)
@@ -793,6 +864,7 @@ Scheduler::Scheduler(const SPM::CSR *A, bool sym, bool perm, ALG_SCHEDULE a, int
 
 Scheduler::~Scheduler() {
     for (double **p : {&d_lu, &d_lv, &d_uv, &d_x, &d_b}) dfree(*p);
+    delete csrAPerm;
 }
 
 // the IC0 schedule of A's lower part serves IC0 and both sweeps
@@ -810,35 +882,48 @@ void Scheduler::preprocessing() {
     origToPermByThread = permToOrigByThread;
 }
 
+// test/Schedule.hpp:298-314: the renumbering the Perm entry points use,
+// here the IC0 plan's row order (parents before children, each unit's rows
+// consecutive), and the schedule of A renumbered by it
 void Scheduler::permPreprocessing() {
-    permToOrigByThread.resize(csrA->m);
-    std::iota(permToOrigByThread.begin(), permToOrigByThread.end(), 0);
-    origToPermByThread = permToOrigByThread;
+    if (!gs.sched) throw std::logic_error("Scheduler::permPreprocessing: call preprocessing() first");
+    long long len = 0;
+    baas_rt::check(baas_schedule_export(gs.sched, BAAS_EXPORT_UNIT_ROWS, nullptr, &len), "export");
+    permToOrigByThread.assign((size_t)len, 0);
+    baas_rt::check(baas_schedule_export(gs.sched, BAAS_EXPORT_UNIT_ROWS, permToOrigByThread.data(), &len),
+                   "export");
+    origToPermByThread.assign((size_t)csrA->m, 0);
+    for (int i = 0; i < csrA->m; i++) origToPermByThread[permToOrigByThread[i]] = i;
+    delete csrAPerm;
+    csrAPerm = csrA->permute(origToPermByThread.data(), permToOrigByThread.data(), true);
+    gsPerm.analyze(*csrAPerm, BAAS_KIND_IC0, num_thread, BAAS_PLAN_AUTO, 0, 0, 0);
 }
 
-void Scheduler::runIC0(const SPM::CSR *mat, double *lu) {
-    hip_ok(hipMemcpy(gs.dA.values, mat->values, sizeof(double) * gs.dA.nnz, hipMemcpyHostToDevice), "upload A");
-    baas_rt::check(baas_ic0_f64(gs.eng.h, gs.sched, gs.dA.values, d_lu), "baas_ic0_f64");
-    const int st = baas_check(gs.eng.h);
+void Scheduler::ic0_on(baas_rt::GpuSchedule &g, const SPM::CSR *mat, double *lu) {
+    if (!g.sched) throw std::logic_error("Scheduler: not preprocessed");
+    hip_ok(hipMemcpy(g.dA.values, mat->values, sizeof(double) * g.dA.nnz, hipMemcpyHostToDevice), "upload A");
+    baas_rt::check(baas_ic0_f64(g.eng.h, g.sched, g.dA.values, d_lu), "baas_ic0_f64");
+    const int st = baas_check(g.eng.h);
     if (st != BAAS_OK && st != BAAS_NONPOSITIVE_PIVOT) baas_rt::check(st, "ic0 kernel");
-    hip_ok(hipMemcpy(lu, d_lu, sizeof(double) * gs.dA.nnz, hipMemcpyDeviceToHost), "download lu");
-    baas_rt::check(baas_ic0_split_f64(gs.eng.h, gs.sched, d_lu, d_lv, d_uv), "baas_ic0_split_f64");
+    hip_ok(hipMemcpy(lu, d_lu, sizeof(double) * g.dA.nnz, hipMemcpyDeviceToHost), "download lu");
+    baas_rt::check(baas_ic0_split_f64(g.eng.h, g.sched, d_lu, d_lv, d_uv), "baas_ic0_split_f64");
 }
 
-void Scheduler::runForwardTRSVNoPerm(const SPM::CSR *L, double *x, const double *b) {
-    hip_ok(hipMemcpy(d_lv, L->values, sizeof(double) * nnzL, hipMemcpyHostToDevice), "upload L");
-    hip_ok(hipMemcpy(d_b, b, sizeof(double) * gs.dA.n, hipMemcpyHostToDevice), "upload b");
-    baas_rt::check(baas_sptrsv_f64(gs.eng.h, gs.sched, 0, d_lv, d_b, d_x, 1), "forward");
-    baas_rt::check(baas_check(gs.eng.h), "forward kernel");
-    hip_ok(hipMemcpy(x, d_x, sizeof(double) * gs.dA.n, hipMemcpyDeviceToHost), "download x");
+void Scheduler::trsv_on(baas_rt::GpuSchedule &g, int uplo, const SPM::CSR *T, double *x, const double *b) {
+    if (!g.sched) throw std::logic_error("Scheduler: not preprocessed");
+    double *dv = uplo ? d_uv : d_lv;
+    hip_ok(hipMemcpy(dv, T->values, sizeof(double) * nnzL, hipMemcpyHostToDevice), "upload factor");
+    hip_ok(hipMemcpy(d_b, b, sizeof(double) * g.dA.n, hipMemcpyHostToDevice), "upload b");
+    baas_rt::check(baas_sptrsv_f64(g.eng.h, g.sched, uplo, dv, d_b, d_x, 1), uplo ? "backward" : "forward");
+    baas_rt::check(baas_check(g.eng.h), uplo ? "backward kernel" : "forward kernel");
+    hip_ok(hipMemcpy(x, d_x, sizeof(double) * g.dA.n, hipMemcpyDeviceToHost), "download x");
 }
 
-void Scheduler::runBackwardTRSVNoPerm(const SPM::CSR *U, double *x, const double *b) {
-    hip_ok(hipMemcpy(d_uv, U->values, sizeof(double) * nnzL, hipMemcpyHostToDevice), "upload U");
-    hip_ok(hipMemcpy(d_b, b, sizeof(double) * gs.dA.n, hipMemcpyHostToDevice), "upload b");
-    baas_rt::check(baas_sptrsv_f64(gs.eng.h, gs.sched, 1, d_uv, d_b, d_x, 1), "backward");
-    baas_rt::check(baas_check(gs.eng.h), "backward kernel");
-    hip_ok(hipMemcpy(x, d_x, sizeof(double) * gs.dA.n, hipMemcpyDeviceToHost), "download x");
-}
+void Scheduler::runIC0(const SPM::CSR *mat, double *lu) { ic0_on(gs, mat, lu); }
+void Scheduler::runIC0Perm(const SPM::CSR *matPerm, double *lu) { ic0_on(gsPerm, matPerm, lu); }
+void Scheduler::runForwardTRSVNoPerm(const SPM::CSR *L, double *x, const double *b) { trsv_on(gs, 0, L, x, b); }
+void Scheduler::runForwardTRSVPerm(const SPM::CSR *L, double *x, const double *b) { trsv_on(gsPerm, 0, L, x, b); }
+void Scheduler::runBackwardTRSVNoPerm(const SPM::CSR *U, double *x, const double *b) { trsv_on(gs, 1, U, x, b); }
+void Scheduler::runBackwardTRSVPerm(const SPM::CSR *U, double *x, const double *b) { trsv_on(gsPerm, 1, U, x, b); }
 
 }  // namespace Schedule

@@ -150,6 +150,11 @@ public:
     CSR &operator=(const CSR &) = delete;
 
     int getNnz() const { return rowptr[m] - rowptr[0]; }
+    // spm/Permute.cpp:215-405: row i of the result is row rowInversePerm[i]
+    // (the identity if NULL) with column c renamed columnPerm[c]; sort puts
+    // each row's entries in ascending column order (the GPU analysis needs
+    // it); diagptr / diag / idiag follow the rows when present.
+    CSR *permute(const int *columnPerm, const int *rowInversePerm, bool sort = false) const;
     // first column >= row in each row (spm/CSR.cpp:853-892)
     void constructDiagPtr();
     void computeInverseDiag();
@@ -485,17 +490,23 @@ enum ALG_SCHEDULE { P2P, P2P_MAPPING, P2P_TIME, P2P_RULE };
 
 // test/Schedule.hpp:39-353: one schedule for IC0 and the two triangular
 // solves of ICCG.  Matrices and vectors passed to run* are host arrays.
-// The Perm entry points take matrices renumbered by permToOrigByThread,
-// which here is the identity (the GPU plans need no renumbering), so they
-// are the NoPerm ones.
+// permPreprocessing() (after preprocessing()) sets permToOrigByThread to the
+// execution plan's row order -- the GPU counterpart of the reference's
+// thread-contiguous order: each work unit's rows become consecutive -- and
+// analyses A renumbered by it, A->permute(origToPermByThread.data(),
+// permToOrigByThread.data(), true); the Perm entry points take matrices and
+// vectors renumbered the same way.
 class Scheduler {
-    baas_rt::GpuSchedule gs;
+    baas_rt::GpuSchedule gs, gsPerm;
     const SPM::CSR *csrA;
+    SPM::CSR *csrAPerm = nullptr;
     bool isSymmetric, isTRSVPerm;
     ALG_SCHEDULE alg;
     int num_thread;
     int nnzL = 0;
     double *d_lu = nullptr, *d_lv = nullptr, *d_uv = nullptr, *d_x = nullptr, *d_b = nullptr;
+    void ic0_on(baas_rt::GpuSchedule &g, const SPM::CSR *mat, double *lu);
+    void trsv_on(baas_rt::GpuSchedule &g, int uplo, const SPM::CSR *T, double *x, const double *b);
 
 public:
     std::vector<int> permToOrigByThread, origToPermByThread;
@@ -505,14 +516,13 @@ public:
     void preprocessing();
     void permPreprocessing();
     void runIC0(const SPM::CSR *mat, double *lu);  // lu: nnz(A), host
-    void runIC0Perm(const SPM::CSR *matPerm, double *lu) { runIC0(matPerm, lu); }
+    void runIC0Perm(const SPM::CSR *matPerm, double *lu);
     void runForwardTRSVNoPerm(const SPM::CSR *L, double *x, const double *b);
-    void runForwardTRSVPerm(const SPM::CSR *LPerm, double *x, const double *b) { runForwardTRSVNoPerm(LPerm, x, b); }
+    void runForwardTRSVPerm(const SPM::CSR *LPerm, double *x, const double *b);
     void runBackwardTRSVNoPerm(const SPM::CSR *U, double *x, const double *b);
-    void runBackwardTRSVPerm(const SPM::CSR *UPerm, double *x, const double *b) {
-        runBackwardTRSVNoPerm(UPerm, x, b);
-    }
+    void runBackwardTRSVPerm(const SPM::CSR *UPerm, double *x, const double *b);
     baas_schedule *schedule() const { return gs.sched; }
+    baas_schedule *schedulePerm() const { return gsPerm.sched; }
     Scheduler(const Scheduler &) = delete;
     Scheduler &operator=(const Scheduler &) = delete;
 };

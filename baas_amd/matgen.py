@@ -142,6 +142,14 @@ def random_symmetric_permutation(A: Csr, seed: int = 1):
     return Csr.from_scipy(M), perm
 
 
+def symmetric_permute(A: Csr, perm_to_orig) -> Csr:
+    """P A P^T for a given order (new row i is old row perm_to_orig[i]),
+    rows sorted by column: the renumbering by a schedule's execution order
+    (Schedule.unit_order()) that makes each work unit's rows consecutive."""
+    perm = np.asarray(perm_to_orig, dtype=np.int64)
+    return Csr.from_scipy(A.to_scipy()[perm][:, perm])
+
+
 def rhs_uniform(n: int, seed: int = 0, k: int = 1) -> np.ndarray:
     r = np.random.default_rng(seed).uniform(-1.0, 1.0, size=(n, k) if k > 1 else n)
     return np.ascontiguousarray(r)

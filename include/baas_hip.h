@@ -62,7 +62,10 @@ enum baas_export {
     BAAS_EXPORT_REDUCED_PTR = 10,  /* int[n+1]        two-hop reduced CSC DAG (spm/DAG.cpp:295-344) */
     BAAS_EXPORT_REDUCED_SET = 11,  /* int[edges] */
     BAAS_EXPORT_UNIT_PTR = 12,     /* int[nunits+1]   execution plan: rows per work unit */
-    BAAS_EXPORT_UNIT_ROWS = 13     /* int[n]          execution plan: row order */
+    BAAS_EXPORT_UNIT_ROWS = 13,    /* int[n]          execution plan: row order */
+    BAAS_EXPORT_GROUP_COST = 14    /* int[ngroups]    nonzeros of each group's rows in the lower
+                                    *                 CSR (costComputation, merge/Coarsen.cpp:854-891;
+                                    *                 the reference keeps them as doubles) */
 };
 
 /* Execution-plan choice (the autotuner fills these when mode = AUTO). */

@@ -58,6 +58,7 @@ _sig(lib, "orc_csr_to_csc", None, [_i, _i, _I, _I, C.c_void_p, _I, _I, C.c_void_
 _sig(lib, "orc_reverse_coarsen", _i, [_i, _I, _I, _I, _I])
 _sig(lib, "orc_forward_coarsen", _i, [_i, _I, _I, _I, _I])
 _sig(lib, "orc_baas_schedule", None, [_i, _i, _I, _I, _I, _i, _I, _I, _I, _I, _I, _I, _I, _I])
+_sig(lib, "orc_group_cost", None, [_i, _I, _I, _I, _I])
 _sig(lib, "orc_sptrsv_lower", None, [_i, _I, _I, _D, _D, _D])
 _sig(lib, "orc_sptrsv_upper", None, [_i, _I, _I, _D, _D, _D])
 _sig(lib, "orc_sptrsv_upper_desc", None, [_i, _I, _I, _D, _D, _D])
@@ -94,6 +95,7 @@ def get_ref():
         _sig(r, "ref_dcsrmv", None, [_i, _I, _I, _D, _D, _D, _d, _d])
         _sig(r, "ref_dot", _d, [_i, _D, _D])
         _sig(r, "ref_waxpby", None, [_i, _D, _d, _D, _d, _D])
+        _sig(r, "ref_group_cost", None, [_i, _I, _I, _I, _I, _D])
     _ref_lib = r
     return r
 
@@ -167,6 +169,20 @@ def baas_schedule(kind, n, lrowptr, lcolidx, thr_rowptr=None, ncore=8, use_ref=F
                 group_ptr=gp[: ng + 1], group_set=gs, level_ptr=lp[: nl + 1],
                 perm_to_orig=pto[:ng], node_to_level=ntl[:ng], nthreads=nt[:nl],
                 merged_ptr=mp[: nm + 1])
+
+
+def group_cost(group_ptr, group_set, lrowptr, lcolidx=None, use_ref=False):
+    """costComputation (merge/Coarsen.cpp:854-891): nonzeros of each group's
+    rows in the lower CSR (int32; the reference returns doubles, its
+    restatement ints of the same values)."""
+    ng = len(group_ptr) - 1
+    if use_ref:
+        out = np.zeros(ng)
+        get_ref().ref_group_cost(ng, _i32(group_ptr), _i32(group_set), _i32(lrowptr), _i32(lcolidx), out)
+        return out
+    out = np.zeros(ng, np.int32)
+    lib.orc_group_cost(ng, _i32(group_ptr), _i32(group_set), _i32(lrowptr), out)
+    return out
 
 
 # --------------------------------------------------------------------------

@@ -335,6 +335,18 @@ void orc_nthreads_per_level(int kind, int nlevels, const int *level_ptr, const i
 /* Merge runs of consecutive single-thread levels: the two-pointer loop of
  * merge/LevelMerge.cpp:331-355 (level_merge_group_TRSV).  nthreads is
  * compacted in place; returns the merged level count.  Pinned: M, R. */
+/* costComputation, merge/Coarsen.cpp:854-891 (kernel SpTRSV_LL, grouped):
+ * cost[g] = nonzeros of group g's rows in the lower CSR.
+ * Pinned: the compiled reference (ref_group_cost) on the schedule test
+ * matrices, and the invariant sum(cost) = nnz(L) (every row in one group). */
+void orc_group_cost(int ngroups, const int *group_ptr, const int *group_set, const int *lrowptr, int *cost) {
+    for (int g = 0; g < ngroups; g++) {
+        int c = 0;
+        for (int k = group_ptr[g]; k < group_ptr[g + 1]; k++) c += lrowptr[group_set[k] + 1] - lrowptr[group_set[k]];
+        cost[g] = c;
+    }
+}
+
 int orc_level_merge(int nlevels, const int *level_ptr, int *nthreads, int *merged_ptr) {
     for (int i = 0; i <= nlevels; i++) merged_ptr[i] = 0;
     int m = 0;

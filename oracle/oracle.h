@@ -49,6 +49,7 @@ void orc_nthreads_per_level(int kind, int nlevels, const int *level_ptr, const i
                             const int *group_ptr, const int *group_set, const int *rowptr,
                             int ncore, int *nthreads);
 int  orc_level_merge(int nlevels, const int *level_ptr, int *nthreads, int *merged_ptr);
+void orc_group_cost(int ngroups, const int *group_ptr, const int *group_set, const int *lrowptr, int *cost);
 
 /* The whole BAAS analysis of a lower-triangular pattern (buildset()).
  * Outputs (caller allocates n+1 / n ints each):

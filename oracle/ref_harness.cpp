@@ -160,6 +160,17 @@ void ref_baas_schedule(int kind, int n, const int *lrowptr, const int *lcolidx,
     delete C;
 }
 
+// costComputation (merge/Coarsen.cpp:854-891) as every runtime calls it:
+// SpTRSV_LL on the lower CSR, grouped (test/SpTRSVRuntime.hpp:720,
+// test/ICRuntime.hpp:476-477, test/Schedule.hpp:229-230).
+void ref_group_cost(int ngroups, const int *group_ptr, const int *group_set, const int *lrowptr,
+                    const int *lcolidx, double *cost_out) {
+    std::vector<double> cost;
+    Coarsen::costComputation(ngroups, nullptr, nullptr, lrowptr, lcolidx, Coarsen::Kernel::SpTRSV_LL,
+                             group_ptr, group_set, true, cost);
+    std::copy(cost.begin(), cost.end(), cost_out);
+}
+
 // IC0, merge/IC.cpp:32-89.  lu must start as a copy of val.
 void ref_ic0(int n, const int *rowptr, const int *colidx, const double *val, double *lu) {
     CSR A(n, n, const_cast<int *>(rowptr), const_cast<int *>(colidx), const_cast<double *>(val));

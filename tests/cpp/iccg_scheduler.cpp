@@ -1,7 +1,8 @@
 // ICCG driven through Schedule::Scheduler the way the reference's pcg_opt
-// drives it (test/iccg_level_merge.cpp:380-533): one schedule, IC0 by
-// runIC0, the factor split into L and U = L^T, then per CG iteration a
-// forward and a backward solve (runForwardTRSVNoPerm / runBackwardTRSVNoPerm).
+// and pcg_opt_perm drive it (test/iccg_level_merge.cpp:215-340, :380-533):
+// one schedule, IC0 by runIC0, the factor split into L and U = L^T, then per
+// CG iteration a forward and a backward solve; then the same on A renumbered
+// by the scheduler (permPreprocessing, A->permute, the Perm entry points).
 // SpMV and BLAS-1 run on the host here (the reference uses MKL for them).
 //   iccg_scheduler <matrix.mtx> <nthread> [tol] [maxiter]
 #include <cmath>
@@ -25,26 +26,13 @@ static double dot(int n, const double *a, const double *b) {
     return s;
 }
 
-int main(int argc, char *argv[]) {
-    if (argc < 3) {
-        fprintf(stderr, "usage: %s matrix.mtx nthread [tol] [maxiter]\n", argv[0]);
-        return 2;
-    }
-    const int nthread = atoi(argv[2]);
-    const double tol = argc > 3 ? atof(argv[3]) : 1e-8;
-    const int maxiter = argc > 4 ? atoi(argv[4]) : 5000;
-    omp_set_num_threads(nthread);
-    CSR *A = new CSR(argv[1]);
-    const int n = A->m, nnz = A->getNnz();
-
-    Schedule::Scheduler *scheduler = new Schedule::Scheduler(A, true, false, Schedule::P2P_RULE, nthread);
-    scheduler->preprocessing();
-    std::vector<double> lu(nnz);
-    scheduler->runIC0(A, lu.data());
-    // L (diagonal last) and U = L^T (diagonal first) from the factor
+// L (diagonal last) and U = L^T (diagonal first) from an IC0 factor in A's layout
+static void split(const CSR *A, const std::vector<double> &lu, CSR *&L, CSR *&U) {
+    const int n = A->m;
     int nl = 0;
     for (int i = 0; i < n; i++) nl += A->diagptr[i] - A->rowptr[i] + 1;
-    CSR *L = new CSR(n, n, nl), *U = new CSR(n, n, nl);
+    L = new CSR(n, n, nl);
+    U = new CSR(n, n, nl);
     std::vector<int> fill(n + 1, 0);
     for (int i = 0; i < n; i++) {
         L->rowptr[i + 1] = L->rowptr[i] + A->diagptr[i] - A->rowptr[i] + 1;
@@ -61,12 +49,31 @@ int main(int argc, char *argv[]) {
             U->colidx[q] = i;
             U->values[q] = lu[j];
         }
-    // PCG with x0 = 0 and b = A * ones
+}
+
+// PCG with x0 = 0 and b = A * ones on A (perm: A renumbered by the
+// scheduler, the runPerm entry points, as pcg_opt_perm,
+// test/iccg_level_merge.cpp:215-340); returns 0 on convergence
+static int pcg(Schedule::Scheduler *scheduler, const CSR *A, bool perm, double tol, int maxiter) {
+    const int n = A->m, nnz = A->getNnz();
+    std::vector<double> lu(nnz);
+    if (perm) scheduler->runIC0Perm(A, lu.data());
+    else scheduler->runIC0(A, lu.data());
+    CSR *L = nullptr, *U = nullptr;
+    split(A, lu, L, U);
+    auto precond = [&](std::vector<double> &z, std::vector<double> &y, const std::vector<double> &r) {
+        if (perm) {
+            scheduler->runForwardTRSVPerm(L, y.data(), r.data());
+            scheduler->runBackwardTRSVPerm(U, z.data(), y.data());
+        } else {
+            scheduler->runForwardTRSVNoPerm(L, y.data(), r.data());
+            scheduler->runBackwardTRSVNoPerm(U, z.data(), y.data());
+        }
+    };
     std::vector<double> ones(n, 1.0), b(n), x(n, 0.0), r(n), z(n), y(n), p(n), Ap(n);
     spmv(A, ones.data(), b.data());
     r = b;
-    scheduler->runForwardTRSVNoPerm(L, y.data(), r.data());
-    scheduler->runBackwardTRSVNoPerm(U, z.data(), y.data());
+    precond(z, y, r);
     p = z;
     double rz = dot(n, r.data(), z.data());
     const double nb = std::sqrt(dot(n, b.data(), b.data()));
@@ -78,8 +85,7 @@ int main(int argc, char *argv[]) {
         for (int i = 0; i < n; i++) x[i] = std::fma(alpha, p[i], x[i]), r[i] = std::fma(-alpha, Ap[i], r[i]);
         rel = std::sqrt(dot(n, r.data(), r.data())) / nb;
         if (rel < tol) break;
-        scheduler->runForwardTRSVNoPerm(L, y.data(), r.data());
-        scheduler->runBackwardTRSVNoPerm(U, z.data(), y.data());
+        precond(z, y, r);
         const double rz1 = dot(n, r.data(), z.data());
         const double beta = rz1 / rz;
         rz = rz1;
@@ -90,10 +96,34 @@ int main(int argc, char *argv[]) {
     double res = 0.0;
     for (int i = 0; i < n; i++) res += (b[i] - Ap[i]) * (b[i] - Ap[i]);
     res = std::sqrt(res) / nb;
-    printf("ICCG: %d iterations, recursive relres %.3e, true relres %.3e\n", it, rel, res);
+    printf("ICCG%s: %d iterations, recursive relres %.3e, true relres %.3e\n", perm ? " perm" : "", it, rel, res);
     delete L;
     delete U;
+    return (rel < tol && res < 10 * tol) ? 0 : 1;
+}
+
+int main(int argc, char *argv[]) {
+    if (argc < 3) {
+        fprintf(stderr, "usage: %s matrix.mtx nthread [tol] [maxiter]\n", argv[0]);
+        return 2;
+    }
+    const int nthread = atoi(argv[2]);
+    const double tol = argc > 3 ? atof(argv[3]) : 1e-8;
+    const int maxiter = argc > 4 ? atoi(argv[4]) : 5000;
+    omp_set_num_threads(nthread);
+    CSR *A = new CSR(argv[1]);
+
+    Schedule::Scheduler *scheduler = new Schedule::Scheduler(A, true, true, Schedule::P2P_RULE, nthread);
+    scheduler->preprocessing();
+    int failures = pcg(scheduler, A, false, tol, maxiter);
+    // the renumbered run (test/iccg_level_merge.cpp:229-235)
+    scheduler->permPreprocessing();
+    const int *inversePerm = scheduler->permToOrigByThread.data();
+    const int *forwardPerm = scheduler->origToPermByThread.data();
+    CSR *APerm = A->permute(forwardPerm, inversePerm, true);
+    failures += pcg(scheduler, APerm, true, tol, maxiter);
+    delete APerm;
     delete scheduler;
     delete A;
-    return (rel < tol && res < 10 * tol) ? 0 : 1;
+    return failures ? 1 : 0;
 }

@@ -70,6 +70,9 @@ int main(int argc, char *argv[]) {
     run(new SpTRSV_DUAL_TREE_BFS_WF_Merge_P2P(csrA, cscA, correct_x, n, nnz, alpha, "Dual Group Merge P2P", nthread,
                                               false),
         0);
+    run(new SpTRSV_DUAL_TREE_BFS_WF_Merge_P2P(csrA, cscA, correct_x, n, nnz, alpha, "Dual Group Merge P2P Perm",
+                                              nthread, true),
+        0);
     run(new SpTRSV_DUAL_TREE_BFS_WF_Merge_P2P_MapFourRule(csrA, cscA, correct_x, n, nnz, alpha,
                                                           "Dual Group Merge P2P MappingFourRule", nthread, false),
         0);

@@ -43,3 +43,40 @@ def test_cpp_loader_matches_reference_loader(golden, tmp_path):
 def test_reference_shaped_mains_are_built():
     for name in ("ic0_level_merge", "ilu0_level_merge", "sptrsv_level_merge", "iccg_scheduler"):
         assert os.access(_tool(name), os.X_OK), name
+
+
+def test_cpp_csr_permute(tmp_path):
+    """SPM::CSR::permute (spm/Permute.cpp:215-405) against P A P^T: rows
+    taken in permToOrig order, columns renamed by its inverse, entry order
+    kept (sort = false) or ascending (sort = true), diagptr following."""
+    import scipy.io
+    import scipy.sparse as sp
+    from baas_amd import matgen as g
+    A = g.convdiff3d(6).to_scipy().tocsr()
+    mtx = tmp_path / "cd6.mtx"
+    scipy.io.mmwrite(str(mtx), A.tocoo())
+    p2o = np.random.default_rng(5).permutation(A.shape[0]).astype(np.int32)
+    p2o.tofile(tmp_path / "perm.int32")
+    r = subprocess.run([_tool("mtx_dump"), str(mtx), str(tmp_path / "cd"), str(tmp_path / "perm.int32")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    o2p = np.argsort(p2o)
+    full = sp.csr_matrix(A)
+    full.sort_indices()
+    for tag, sort in (("perm", False), ("perm_sorted", True)):
+        rp = np.fromfile(tmp_path / f"cd.{tag}.rowptr", np.int32)
+        ci = np.fromfile(tmp_path / f"cd.{tag}.colidx", np.int32)
+        va = np.fromfile(tmp_path / f"cd.{tag}.values", np.float64)
+        dg = np.fromfile(tmp_path / f"cd.{tag}.diagptr", np.int32)
+        for i in range(A.shape[0]):
+            row = p2o[i]
+            cols = o2p[full.indices[full.indptr[row]:full.indptr[row + 1]]]
+            vals = full.data[full.indptr[row]:full.indptr[row + 1]]
+            if sort:
+                o = np.argsort(cols, kind="stable")
+                cols, vals = cols[o], vals[o]
+            np.testing.assert_array_equal(ci[rp[i]:rp[i + 1]], cols)
+            np.testing.assert_array_equal(va[rp[i]:rp[i + 1]], vals)
+            assert ci[dg[i]] == i
+        P = sp.csr_matrix((va, ci, rp), shape=A.shape)
+        assert abs(P - A[p2o][:, p2o]).max() == 0

@@ -72,9 +72,9 @@ def test_reference_shaped_ic0_main(mtx, name):
 
 def test_reference_shaped_sptrsv_main(mtx):
     out = _run(os.path.join(BIN, "sptrsv_level_merge"), mtx / "lap3d20.mtx", 8, mtx / "trsv", 5, cwd=mtx)
-    assert out.count("code solutions check pass.") == 7
+    assert out.count("code solutions check pass.") == 8
     rows = open(mtx / "trsv.csv").read().strip().splitlines()
-    assert len(rows) == 1 + 6
+    assert len(rows) == 1 + 7
 
 
 @pytest.mark.parametrize("name", ["convdiff16", "bcsstk18"])
@@ -86,4 +86,4 @@ def test_reference_shaped_ilu0_main(mtx, name):
 @pytest.mark.parametrize("name", ["lap3d20", "bcsstk18"])
 def test_scheduler_iccg(mtx, name):
     out = _run(os.path.join(BIN, "iccg_scheduler"), mtx / f"{name}.mtx", 8, 1e-8, 5000)
-    assert "ICCG:" in out
+    assert "ICCG:" in out and "ICCG perm:" in out

@@ -129,6 +129,24 @@ def test_schedule_random_dags_vs_reference(seed):
                     assert a[k] == b[k]
 
 
+@pytest.mark.parametrize("name", NAMES)
+def test_group_cost(name):
+    """costComputation: the compiled reference's doubles on the schedule's
+    groups (where oracle/_ref is built), and the invariant that the costs of
+    a partition of the rows add up to nnz(L); a 3-row case by hand."""
+    L, _, _ = workload(name)
+    s = O.baas_schedule(0, L.n, L.rowptr, L.colidx)
+    c = O.group_cost(s["group_ptr"], s["group_set"], L.rowptr)
+    assert c.dtype == np.int32 and len(c) == s["ngroups"]
+    assert int(c.astype(np.int64).sum()) == L.nnz
+    if O.ref is not None:
+        r = O.group_cost(s["group_ptr"], s["group_set"], L.rowptr, L.colidx, use_ref=True)
+        np.testing.assert_array_equal(c.astype(np.float64), r)
+    # rows of lengths 1, 2, 3 in groups {0, 2} and {1}
+    np.testing.assert_array_equal(O.group_cost(np.array([0, 2, 3]), np.array([0, 2, 1]), np.array([0, 1, 3, 6])),
+                                  [4, 2])
+
+
 def test_groups_are_a_partition_and_topological():
     L, _, _ = workload("C1_lap2d_128")
     s = O.baas_schedule(0, L.n, L.rowptr, L.colidx)
