@@ -53,6 +53,8 @@ _SIGS = {
     "baas_schedule_export": (_i, [_P, _i, _P, C.POINTER(C.c_longlong)]),
     "baas_schedule_times": (_i, [_P, C.POINTER(_d), C.POINTER(_d)]),
     "baas_schedule_upper_pattern": (_i, [_P, C.POINTER(_P), C.POINTER(_P)]),
+    "baas_schedule_save": (_i, [_P, _P, C.POINTER(C.c_size_t)]),
+    "baas_schedule_load": (_i, [_P, _i, _P, _P, _i, _P, C.c_size_t, C.POINTER(_P)]),
     "baas_replan": (_i, [_P, _i, _i, _i]),
     "baas_replan_tiles": (_i, [_P, _i, _i, _i]),
     "baas_schedule_plan": (_i, [_P, _P]),

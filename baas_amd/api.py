@@ -100,18 +100,35 @@ class Schedule:
     """BAAS analysis of an n x n pattern (lower L for TRSV, full A for IC0/ILU0)."""
 
     def __init__(self, h: Handle, rowptr, colidx, kind=L.KIND_TRSV, ncore=8, plan_mode=L.PLAN_AUTO,
-                 rows_per_unit=0, block_threads=0):
+                 rows_per_unit=0, block_threads=0, saved: bytes = None):
+        """saved: bytes from save() for the same pattern and kind -- the
+        schedule is rebuilt from them (analysis and plan included) instead
+        of analysed."""
         self.h = h
         self.rowptr = _dev_i32(rowptr)
         self.colidx = _dev_i32(colidx)
         self.n = self.rowptr.numel() - 1
         self.kind = kind
-        o = L.Opts(ncore, plan_mode, rows_per_unit, block_threads, 0)
         self._s = C.c_void_p()
-        L.call("baas_analyze", h.ptr, self.n, _ptr(self.rowptr), _ptr(self.colidx), kind, C.byref(o),
-               C.byref(self._s))
+        if saved is not None:
+            blob = (C.c_ubyte * len(saved)).from_buffer_copy(saved)
+            L.call("baas_schedule_load", h.ptr, self.n, _ptr(self.rowptr), _ptr(self.colidx), kind,
+                   C.cast(blob, C.c_void_p), len(saved), C.byref(self._s))
+        else:
+            o = L.Opts(ncore, plan_mode, rows_per_unit, block_threads, 0)
+            L.call("baas_analyze", h.ptr, self.n, _ptr(self.rowptr), _ptr(self.colidx), kind, C.byref(o),
+                   C.byref(self._s))
         self.nnz = self.colidx.numel()
         self.nnz_lower = self.info()["nnz_lower"]
+
+    def save(self) -> bytes:
+        """The analysis and the plan in use (an autotuned one included) as
+        bytes; Schedule(..., saved=those bytes) rebuilds this schedule."""
+        n = C.c_size_t()
+        L.call("baas_schedule_save", self._s, None, C.byref(n))
+        buf = (C.c_ubyte * n.value)()
+        L.call("baas_schedule_save", self._s, C.cast(buf, C.c_void_p), C.byref(n))
+        return bytes(buf)
 
     @property
     def ptr(self):

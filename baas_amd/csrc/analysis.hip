@@ -18,6 +18,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstddef>
 #include <cstring>
 #include <vector>
 
@@ -630,13 +631,16 @@ static int to_dev(DBuf<T> &d, const std::vector<T> &h, hipStream_t st) {
 }
 
 int build_plans(baas_schedule *s);  // plan.cpp
+int build_plans_mode(baas_schedule *s, int mode, int rows_per_unit);
 
 // The whole analysis.  l_rowptr/l_colidx (device) hold the lower pattern.
-int run_analysis(baas_schedule *s, const int *thr_rowptr) {
+// The arrays every schedule derives from its lower pattern alone: the
+// pattern's validation, each entry's row, U = L^T's pattern and the
+// row-level level set (also what a schedule restored from saved bytes
+// recomputes).
+int derive_patterns(baas_schedule *s) {
     hipStream_t st = s->h->stream;
     const int n = s->n, nnzL = s->nnz_lower;
-    double host_s = 0;
-    auto t0 = std::chrono::steady_clock::now();
     // validate the lower pattern
     {
         DBuf<int> bad;
@@ -671,6 +675,15 @@ int run_analysis(baas_schedule *s, const int *thr_rowptr) {
         BAAS_HIP_TRY(hipMemcpyAsync(s->row_level_ptr_d.p, lp.p, sizeof(int) * (nl + 1), hipMemcpyDeviceToDevice, st));
         BAAS_HIP_TRY(hipMemcpyAsync(s->row_perm_d.p, pto.p, sizeof(int) * n, hipMemcpyDeviceToDevice, st));
     }
+    return BAAS_OK;
+}
+
+int run_analysis(baas_schedule *s, const int *thr_rowptr) {
+    hipStream_t st = s->h->stream;
+    const int n = s->n, nnzL = s->nnz_lower;
+    double host_s = 0;
+    auto t0 = std::chrono::steady_clock::now();
+    BAAS_TRY(derive_patterns(s));
     // two-hop reduction
     DBuf<int> dptr, dset;
     int red = 0;
@@ -848,6 +861,162 @@ extern "C" int baas_analyze(baas_handle *h, int n, const int *rowptr, const int 
     if (s->opts.block_threads <= 0) s->opts.block_threads = 256;
     int st = prepare_lower(s, rowptr, colidx, nnz);
     if (st == BAAS_OK) st = run_analysis(s, kind == BAAS_KIND_TRSV ? s->l_rowptr.p : s->a_rowptr.p);
+    if (st == BAAS_OK) st = s->flags.alloc(n);
+    if (st == BAAS_OK) {
+        hipLaunchKernelGGL(k_fill_int, dim3(nb(n)), dim3(256), 0, h->stream, s->flags.p, 0, n);
+        if (hipStreamSynchronize(h->stream) != hipSuccess) st = BAAS_HIP_ERROR;
+    }
+    if (st != BAAS_OK) {
+        delete s;
+        return st;
+    }
+    *out = s;
+    return BAAS_OK;
+}
+
+// ---------------------------------------------------------------- persistence
+// Saved schedule: a header of int32 words, the pattern's digest, then the
+// analysis arrays as (int64 length, int32 values); little-endian, as the
+// reference's CSR::storeBin writes its matrices (spm/CSR.cpp:399-435).
+namespace {
+constexpr char SAVE_MAGIC[8] = {'B', 'A', 'A', 'S', 'S', 'C', 'H', 'D'};
+constexpr int SAVE_VERSION = 1;
+constexpr int SAVE_HDR = 18;  // int32 words after the magic
+
+// FNV-1a over the pattern's bytes: a saved schedule only loads onto the
+// pattern it was made for
+uint64_t pattern_digest(const std::vector<int> &rp, const std::vector<int> &ci) {
+    uint64_t h = 1469598103934665603ULL;
+    auto mix = [&](const std::vector<int> &v) {
+        const unsigned char *b = reinterpret_cast<const unsigned char *>(v.data());
+        for (size_t i = 0; i < v.size() * sizeof(int); i++) h = (h ^ b[i]) * 1099511628211ULL;
+    };
+    mix(rp);
+    mix(ci);
+    return h;
+}
+
+int pattern_digest_dev(int n, const int *rowptr, const int *colidx, hipStream_t st, uint64_t *out, int *nnz) {
+    std::vector<int> rp(n + 1);
+    BAAS_HIP_TRY(hipMemcpyAsync(rp.data(), rowptr, sizeof(int) * (n + 1), hipMemcpyDeviceToHost, st));
+    BAAS_HIP_TRY(hipStreamSynchronize(st));
+    if (rp[n] <= 0) return BAAS_INVALID_ARG;
+    std::vector<int> ci(rp[n]);
+    BAAS_HIP_TRY(hipMemcpyAsync(ci.data(), colidx, sizeof(int) * rp[n], hipMemcpyDeviceToHost, st));
+    BAAS_HIP_TRY(hipStreamSynchronize(st));
+    *out = pattern_digest(rp, ci);
+    *nnz = rp[n];
+    return BAAS_OK;
+}
+
+std::vector<std::vector<int> *> saved_arrays(baas_schedule *s) {
+    return {&s->group_ptr, &s->group_set, &s->level_ptr, &s->perm_to_orig, &s->node_to_level, &s->nthreads,
+            &s->merged_ptr, &s->reduced_ptr, &s->reduced_set, &s->group_cost};
+}
+}  // namespace
+
+extern "C" int baas_schedule_save(const baas_schedule *cs, void *buf, size_t *len) {
+    if (!cs || !len) return BAAS_INVALID_ARG;
+    baas_schedule *s = const_cast<baas_schedule *>(cs);
+    const int *rp = s->kind == BAAS_KIND_TRSV ? s->l_rowptr.p : s->a_rowptr.p;
+    const int *ci = s->kind == BAAS_KIND_TRSV ? s->l_colidx.p : s->a_colidx.p;
+    uint64_t dig = 0;
+    int nnz = 0;
+    BAAS_HIP_TRY(hipSetDevice(s->h->device));
+    BAAS_TRY(pattern_digest_dev(s->n, rp, ci, s->h->stream, &dig, &nnz));
+    int plan[5];
+    BAAS_TRY(baas_schedule_plan(s, plan));
+    const int hdr[SAVE_HDR] = {SAVE_VERSION, s->kind, s->n, s->nnz_full, s->nnz_lower, s->opts.ncore,
+                               s->opts.merge_threshold, s->ngroups, s->nlevels, s->nmerged, s->reduced_edges,
+                               plan[0], plan[1], plan[2], plan[3], s->tile_band, (int)(dig & 0xffffffffu),
+                               (int)(dig >> 32)};
+    size_t need = sizeof(SAVE_MAGIC) + sizeof(hdr);
+    for (auto *v : saved_arrays(s)) need += sizeof(int64_t) + v->size() * sizeof(int);
+    if (!buf) {
+        *len = need;
+        return BAAS_OK;
+    }
+    if (*len < need) {
+        *len = need;
+        return BAAS_INVALID_ARG;
+    }
+    unsigned char *o = static_cast<unsigned char *>(buf);
+    auto put = [&](const void *p, size_t b) {
+        std::memcpy(o, p, b);
+        o += b;
+    };
+    put(SAVE_MAGIC, sizeof(SAVE_MAGIC));
+    put(hdr, sizeof(hdr));
+    for (auto *v : saved_arrays(s)) {
+        const int64_t k = (int64_t)v->size();
+        put(&k, sizeof(k));
+        put(v->data(), v->size() * sizeof(int));
+    }
+    *len = need;
+    return BAAS_OK;
+}
+
+extern "C" int baas_schedule_load(baas_handle *h, int n, const int *rowptr, const int *colidx, int kind,
+                                  const void *buf, size_t len, baas_schedule **out) {
+    if (!h || !out || !buf || n <= 0 || !rowptr || !colidx || kind < 0 || kind > 2) return BAAS_INVALID_ARG;
+    *out = nullptr;
+    const unsigned char *in = static_cast<const unsigned char *>(buf), *end = in + len;
+    int hdr[SAVE_HDR];
+    if (len < sizeof(SAVE_MAGIC) + sizeof(hdr) || std::memcmp(in, SAVE_MAGIC, sizeof(SAVE_MAGIC)) != 0)
+        return BAAS_INVALID_ARG;
+    std::memcpy(hdr, in + sizeof(SAVE_MAGIC), sizeof(hdr));
+    in += sizeof(SAVE_MAGIC) + sizeof(hdr);
+    if (hdr[0] != SAVE_VERSION || hdr[1] != kind || hdr[2] != n) return BAAS_INVALID_ARG;
+    BAAS_HIP_TRY(hipSetDevice(h->device));
+    uint64_t dig = 0;
+    int nnz = 0;
+    BAAS_TRY(pattern_digest_dev(n, rowptr, colidx, h->stream, &dig, &nnz));
+    if (nnz != hdr[3] || (int)(dig & 0xffffffffu) != hdr[16] || (int)(dig >> 32) != hdr[17]) return BAAS_INVALID_ARG;
+    baas_schedule *s = new baas_schedule();
+    s->h = h;
+    s->kind = kind;
+    s->n = n;
+    s->nnz_full = nnz;
+    s->opts.ncore = hdr[5];
+    s->opts.merge_threshold = hdr[6];
+    int st = BAAS_OK;
+    for (auto *v : saved_arrays(s)) {
+        int64_t k = 0;
+        if (end - in < (ptrdiff_t)sizeof(k)) { st = BAAS_INVALID_ARG; break; }
+        std::memcpy(&k, in, sizeof(k));
+        in += sizeof(k);
+        if (k < 0 || (end - in) / (ptrdiff_t)sizeof(int) < k) { st = BAAS_INVALID_ARG; break; }
+        v->resize((size_t)k);
+        std::memcpy(v->data(), in, (size_t)k * sizeof(int));
+        in += (size_t)k * sizeof(int);
+    }
+    if (st == BAAS_OK) {
+        s->ngroups = hdr[7];
+        s->nlevels = hdr[8];
+        s->nmerged = hdr[9];
+        s->reduced_edges = hdr[10];
+        if ((int)s->group_ptr.size() != s->ngroups + 1 || (int)s->group_set.size() != n ||
+            (int)s->level_ptr.size() != s->nlevels + 1 || (int)s->perm_to_orig.size() != s->ngroups ||
+            (int)s->merged_ptr.size() != s->nmerged + 1)
+            st = BAAS_INVALID_ARG;
+    }
+    auto t0 = std::chrono::steady_clock::now();
+    if (st == BAAS_OK) st = prepare_lower(s, rowptr, colidx, nnz);
+    if (st == BAAS_OK && s->nnz_lower != hdr[4]) st = BAAS_INVALID_ARG;
+    if (st == BAAS_OK) st = derive_patterns(s);
+    double gpu_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    // the saved plan: mode, rows per unit, threads, dataflow kernel, band
+    if (st == BAAS_OK) {
+        auto t1 = std::chrono::steady_clock::now();
+        s->opts.plan_mode = hdr[11];
+        s->opts.rows_per_unit = hdr[12];
+        s->opts.block_threads = hdr[13];
+        s->tile_band = hdr[15];
+        st = build_plans_mode(s, hdr[11], hdr[12]);
+        if (st == BAAS_OK) st = baas_plan_dataflow(s, hdr[14]);
+        s->t_host = std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count();
+        s->t_gpu = gpu_s;
+    }
     if (st == BAAS_OK) st = s->flags.alloc(n);
     if (st == BAAS_OK) {
         hipLaunchKernelGGL(k_fill_int, dim3(nb(n)), dim3(256), 0, h->stream, s->flags.p, 0, n);

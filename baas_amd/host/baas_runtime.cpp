@@ -228,10 +228,13 @@ static void fill_from_coo(CSR &A, const Coo &a, int base) {
     }
 }
 
-CSR::CSR(const char *fileName, int base, bool, int) {
+CSR::CSR(const char *fileName, int base, bool forceSymmetric, int) {
     const size_t l = strlen(fileName);
-    if (l > 4 && !strcmp(fileName + l - 4, ".bin"))
-        throw std::invalid_argument("SPM::CSR: PETSc .bin files are not supported");
+    if (l > 4 && !strcmp(fileName + l - 4, ".bin")) {
+        if (forceSymmetric) throw std::invalid_argument("SPM::CSR: forceSymmetric with a .bin file");
+        loadBin(fileName, base);
+        return;
+    }
     Coo a = read_matrix_market(fileName);
     n = a.n;
     alloc(a.m, (int)a.r.size());
@@ -308,6 +311,66 @@ CSR *CSR::permute(const int *columnPerm, const int *rowInversePerm, bool sort) c
         for (int i = 0; i < m; i++) r->idiag[i] = idiag[rowInversePerm ? rowInversePerm[i] : i];
     }
     return r;
+}
+
+static const int MAT_FILE_CLASSID = 1211216;
+
+void CSR::loadBin(const char *fileName, int base) {
+    if (base != 0) throw std::invalid_argument("SPM::CSR::loadBin: only 0-based matrices");
+    FILE *fp = fopen(fileName, "rb");
+    if (!fp) throw std::runtime_error(std::string("SPM::CSR::loadBin: cannot open ") + fileName);
+    int hdr[4];
+    if (fread(hdr, sizeof(int), 4, fp) != 4 || hdr[0] != MAT_FILE_CLASSID || hdr[1] < 0 || hdr[2] < 0 ||
+        hdr[3] < 0) {
+        fclose(fp);
+        throw std::runtime_error(std::string("SPM::CSR::loadBin: not a matrix file: ") + fileName);
+    }
+    if (ownData_) {
+        delete[] rowptr;
+        delete[] colidx;
+        delete[] values;
+    }
+    delete[] diagptr;
+    delete[] diag;
+    delete[] idiag;
+    diagptr = nullptr;
+    diag = idiag = nullptr;
+    n = hdr[2];
+    alloc(hdr[1], hdr[3]);
+    const bool ok = fread(rowptr + 1, sizeof(int), (size_t)m, fp) == (size_t)m &&
+                    fread(colidx, sizeof(int), (size_t)hdr[3], fp) == (size_t)hdr[3] &&
+                    fread(values, sizeof(double), (size_t)hdr[3], fp) == (size_t)hdr[3];
+    fclose(fp);
+    if (!ok) throw std::runtime_error(std::string("SPM::CSR::loadBin: truncated file ") + fileName);
+    rowptr[0] = 0;
+    for (int i = 0; i < m; i++) rowptr[i + 1] += rowptr[i];
+    if (rowptr[m] != hdr[3]) throw std::runtime_error("SPM::CSR::loadBin: row lengths do not add up to nnz");
+    // the exact diagonal of each row (rows without one keep diagptr = -1)
+    diagptr = new int[m];
+    diag = new double[m]();
+    idiag = new double[m]();
+    for (int i = 0; i < m; i++) {
+        diagptr[i] = -1;
+        for (int j = rowptr[i]; j < rowptr[i + 1]; j++)
+            if (colidx[j] == i) {
+                diagptr[i] = j;
+                diag[i] = values[j];
+                idiag[i] = 1 / values[j];
+            }
+    }
+}
+
+void CSR::storeBin(const char *fileName) {
+    FILE *fp = fopen(fileName, "wb");
+    if (!fp) throw std::runtime_error(std::string("SPM::CSR::storeBin: cannot open ") + fileName);
+    const int nnz = getNnz();
+    const int hdr[4] = {MAT_FILE_CLASSID, m, n, nnz};
+    std::vector<int> len(m);
+    for (int i = 0; i < m; i++) len[i] = rowptr[i + 1] - rowptr[i];
+    const bool ok = fwrite(hdr, sizeof(int), 4, fp) == 4 && fwrite(len.data(), sizeof(int), m, fp) == (size_t)m &&
+                    fwrite(colidx + rowptr[0], sizeof(int), nnz, fp) == (size_t)nnz &&
+                    fwrite(values + rowptr[0], sizeof(double), nnz, fp) == (size_t)nnz;
+    if (fclose(fp) != 0 || !ok) throw std::runtime_error(std::string("SPM::CSR::storeBin: write failed ") + fileName);
 }
 
 void CSR::constructDiagPtr() {

@@ -124,7 +124,8 @@ private:
 };
 
 // spm/include/CSR.hpp:16-40: 0-based CSR with rows sorted by column.  The
-// Matrix Market constructors follow spm/CSR.cpp:90-125: the full loader
+// file constructors follow spm/CSR.cpp:90-125 (a name ending in .bin is read
+// with loadBin, anything else as Matrix Market): the full loader
 // expands symmetric files and builds diagptr / diag / idiag (exact diagonal
 // match), the lower loader keeps the strictly lower entries and puts 1.0 on
 // the diagonal (spm/COO.cpp:554-622).
@@ -155,6 +156,11 @@ public:
     // each row's entries in ascending column order (the GPU analysis needs
     // it); diagptr / diag / idiag follow the rows when present.
     CSR *permute(const int *columnPerm, const int *rowInversePerm, bool sort = false) const;
+    // spm/CSR.cpp:332-435: the binary file {1211216, m, n, nnz, row
+    // lengths[m], colidx[nnz], values[nnz]} (native int32 / double);
+    // loadBin rebuilds rowptr and diagptr / diag / idiag.  0-based only.
+    void loadBin(const char *fileName, int base = 0);
+    void storeBin(const char *fileName);
     // first column >= row in each row (spm/CSR.cpp:853-892)
     void constructDiagPtr();
     void computeInverseDiag();

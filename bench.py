@@ -576,7 +576,14 @@ def main():
         plan = forced
         apply_plan(s, plan)
     t_tune = time.perf_counter() - t0
-    print(f"[bench] analysis {t_an:.3f} s, autotune {t_tune:.2f} s, plan {plan}", file=sys.stderr, flush=True)
+    an_gpu, an_host = s.times()
+    # the same schedule rebuilt from its saved bytes (analysis and tuned plan)
+    blob = s.save()
+    t0 = time.perf_counter()
+    Schedule(h, L.rowptr, L.colidx, kind=_lib.KIND_TRSV, saved=blob)
+    t_load = time.perf_counter() - t0
+    print(f"[bench] analysis {t_an:.3f} s, autotune {t_tune:.2f} s, load {t_load:.3f} s, plan {plan}",
+          file=sys.stderr, flush=True)
 
     total_ms, mean_ms, med_ms = time_events(lambda: s.solve(lv, bd, x), args.steps, args.warmup)
     h.check()
@@ -600,7 +607,9 @@ def main():
         "config": {"workload": "FP64 forward SpTRSV x = L^-1 b, L = lower triangle of the 3D 7-point Laplacian "
                                "on a 64^3 grid, natural ordering, 1 right-hand side",
                    "n": L.n, "nnz_L": L.nnz, "bytes_per_solve": by, "plan": plan,
-                   "analysis_s": round(t_an, 4), "autotune_s": round(t_tune, 3)},
+                   "analysis_s": round(t_an, 4), "analysis_gpu_phase_s": round(an_gpu, 4),
+                   "analysis_host_phase_s": round(an_host, 4), "autotune_s": round(t_tune, 3),
+                   "saved_schedule_bytes": len(blob), "saved_schedule_load_s": round(t_load, 4)},
         "roofline": {"bound": "hbm", "achieved": round(kern_gbs, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(kern_gbs / HBM_PEAK_GBS, 6), "traffic": traffic,
                      "traffic_plan": traffic_plan,

@@ -19,6 +19,7 @@
  */
 #ifndef BAAS_HIP_H
 #define BAAS_HIP_H
+#include <stddef.h>
 #include <stdint.h>
 #ifdef __cplusplus
 extern "C" {
@@ -133,6 +134,20 @@ int baas_schedule_info(const baas_schedule *s, int *counts);
 int baas_schedule_export(const baas_schedule *s, int which, int *out, long long *len);
 /* Seconds the analysis took, split into GPU phase and host phase (host). */
 int baas_schedule_times(const baas_schedule *s, double *gpu_s, double *host_s);
+/* Persistence.  The reference keeps its inputs across runs as binary
+ * matrices (CSR::storeBin/loadBin, spm/CSR.cpp:332-435) and redoes its
+ * buildset() analysis every run; a schedule can be kept too.  save writes the
+ * analysis (groups, levels, threads per level, merged levels, reduced DAG,
+ * group costs), the plan in use (mode, rows per unit, threads, dataflow
+ * kernel, TILES band: an autotuned choice included) and a digest of the
+ * pattern into buf (host); buf == NULL only queries *len.  load rebuilds the
+ * schedule for the same pattern and kind (device rowptr/colidx as for
+ * baas_analyze) without re-running the coarsening analysis or the
+ * autotuner: the exports, the plan and every result are the original's.
+ * Bytes from another pattern, kind or format version: BAAS_INVALID_ARG. */
+int baas_schedule_save(const baas_schedule *s, void *buf, size_t *len);
+int baas_schedule_load(baas_handle *h, int n, const int *rowptr, const int *colidx, int kind,
+                       const void *buf, size_t len, baas_schedule **out);
 /* Device pointers of U = L^T's CSR pattern (rows with the diagonal first). */
 int baas_schedule_upper_pattern(const baas_schedule *s, const int **rowptr, const int **colidx);
 

@@ -96,6 +96,7 @@ def get_ref():
         _sig(r, "ref_dot", _d, [_i, _D, _D])
         _sig(r, "ref_waxpby", None, [_i, _D, _d, _D, _d, _D])
         _sig(r, "ref_group_cost", None, [_i, _I, _I, _I, _I, _D])
+        _sig(r, "ref_store_bin", None, [_i, _i, _I, _I, _D, C.c_char_p])
     _ref_lib = r
     return r
 

@@ -201,7 +201,14 @@ void ref_waxpby(int n, double *w, double alpha, const double *x, double beta, co
     waxpby(n, w, alpha, x, beta, y);
 }
 
-// Matrix Market loaders (spm/CSR.cpp:90-125): lower != 0 uses the "lower"
+// CSR::storeBin (spm/CSR.cpp:399-435) of a 0-based CSR.
+void ref_store_bin(int m, int n, const int *rowptr, const int *colidx, const double *values, const char *path) {
+    CSR A(m, n, const_cast<int *>(rowptr), const_cast<int *>(colidx), const_cast<double *>(values));
+    A.storeBin(path);
+}
+
+// Matrix Market loaders (spm/CSR.cpp:90-125; a name ending in .bin goes
+// through CSR::loadBin, spm/CSR.cpp:334-397): lower != 0 uses the "lower"
 // loader (strictly lower part + unit diagonal, spm/COO.cpp:554-622).  Call
 // with rowptr == nullptr to get m and nnz, then again with buffers.
 int ref_load_mtx(const char *path, int lower, int *m_out, int *nnz_out,

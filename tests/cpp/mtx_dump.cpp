@@ -2,7 +2,9 @@
 // CSR(file); lower with a unit diagonal: CSR(file, true, 0)) and writes the
 // arrays as raw little-endian files <prefix>.{full,lower}.{rowptr,colidx,values}
 // so a test can compare them with the reference loader's digests; with a
-// permutation file it also writes CSR::permute's result.  No GPU.
+// permutation file it also writes CSR::permute's result; the full matrix is
+// also stored with CSR::storeBin as <prefix>.full.bin (a .bin input is read
+// with loadBin).  No GPU.
 #include <cstdio>
 #include <fstream>
 #include <string>
@@ -23,6 +25,12 @@ int main(int argc, char *argv[]) {
     }
     SPM::CSR full(argv[1]);
     dump(full, std::string(argv[2]) + ".full");
+    full.storeBin((std::string(argv[2]) + ".full.bin").c_str());
+    const std::string in(argv[1]);
+    if (in.size() > 4 && in.compare(in.size() - 4, 4, ".bin") == 0) {  // a binary input: no lower loader
+        printf("full %d x %d, %d nnz\n", full.m, full.n, full.getNnz());
+        return 0;
+    }
     if (argc > 3) {
         // CSR::permute(origToPerm, permToOrig, sort) with permToOrig read from
         // a raw int32 file; the result's diagptr is dumped as well

@@ -80,3 +80,53 @@ def test_cpp_csr_permute(tmp_path):
             assert ci[dg[i]] == i
         P = sp.csr_matrix((va, ci, rp), shape=A.shape)
         assert abs(P - A[p2o][:, p2o]).max() == 0
+
+
+def _ref_load(path):
+    """The compiled reference's CSR(file) (a .bin name goes through loadBin)."""
+    import ctypes as C
+    import oracle as O
+    f = O.ref.ref_load_mtx
+    f.argtypes = [C.c_char_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.c_void_p, C.c_void_p, C.c_void_p]
+    m, nz = C.c_int(), C.c_int()
+    f(str(path).encode(), 0, C.byref(m), C.byref(nz), None, None, None)
+    rp = np.zeros(m.value + 1, np.int32)
+    ci = np.zeros(nz.value, np.int32)
+    va = np.zeros(nz.value)
+    f(str(path).encode(), 0, C.byref(m), C.byref(nz), rp.ctypes.data, ci.ctypes.data, va.ctypes.data)
+    return rp, ci, va
+
+
+def test_cpp_csr_binary_files(tmp_path):
+    """CSR::storeBin / loadBin (spm/CSR.cpp:332-435): the file layout
+    {1211216, m, n, nnz, row lengths, colidx, values}; where oracle/_ref is
+    built, byte-identical to the reference's storeBin and readable by its
+    loadBin, and the reference's files readable here."""
+    import oracle as O
+    from baas_amd import matgen as g
+    import scipy.io
+    A = g.convdiff3d(7)
+    mtx = tmp_path / "cd7.mtx"
+    scipy.io.mmwrite(str(mtx), A.to_scipy().tocoo())
+    r = subprocess.run([_tool("mtx_dump"), str(mtx), str(tmp_path / "cd")], capture_output=True, text=True,
+                       timeout=120)
+    assert r.returncode == 0, r.stderr
+    ours = (tmp_path / "cd.full.bin").read_bytes()
+    want = (np.array([1211216, A.n, A.n, A.nnz], np.int32).tobytes() + np.diff(A.rowptr).astype(np.int32).tobytes()
+            + A.colidx.astype(np.int32).tobytes() + A.values.astype(np.float64).tobytes())
+    assert ours == want
+    # read back through the runtime's .bin constructor
+    r = subprocess.run([_tool("mtx_dump"), str(tmp_path / "cd.full.bin"), str(tmp_path / "rb")],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    for k, dt in (("rowptr", np.int32), ("colidx", np.int32), ("values", np.float64)):
+        np.testing.assert_array_equal(np.fromfile(tmp_path / f"rb.full.{k}", dt), getattr(A, k).astype(dt))
+    if O.ref is None:
+        return
+    O.ref.ref_store_bin(A.n, A.n, O._i32(A.rowptr), O._i32(A.colidx), O._f64(A.values),
+                        str(tmp_path / "ref.bin").encode())
+    assert (tmp_path / "ref.bin").read_bytes() == ours
+    rp, ci, va = _ref_load(tmp_path / "cd.full.bin")
+    np.testing.assert_array_equal(rp, A.rowptr)
+    np.testing.assert_array_equal(ci, A.colidx)
+    np.testing.assert_array_equal(va, A.values)

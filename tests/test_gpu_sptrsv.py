@@ -483,3 +483,63 @@ def test_renumbered_by_unit_order(handle, name):
     back = np.empty_like(xp)
     back[order] = xp
     np.testing.assert_allclose(back, x, rtol=1e-12, atol=1e-12 * np.abs(x).max())
+
+
+@pytest.mark.parametrize("name,kind,plan", [("C2_lap3d_64", 0, None), ("C3_knn", 1, None),
+                                             ("C1_lap2d_128", 0, (4, 2048, 512, 2, 8)),
+                                             ("C2_convdiff_64", 2, None)])
+def test_schedule_save_load(handle, name, kind, plan):
+    """A saved schedule rebuilds the same analysis, plan and bits without
+    re-analysing; bytes for another pattern or kind, or damaged ones, are
+    refused."""
+    import time
+    from baas_amd import _lib as L
+    from baas_amd.api import Schedule
+    announce("save/load", name, kind, plan)
+    Lw, thr, A = workload(name)
+    M = Lw if kind == 0 else A
+    s = Schedule(handle, M.rowptr, M.colidx, kind=kind)
+    if plan is None:
+        if kind == 0:
+            s.autotune(up(Lw.values))
+        else:
+            s.autotune_factor(up(A.values))
+    else:
+        s.replan_tiles(plan[1], plan[4], plan[2], plan[3])
+    blob = s.save()
+    t0 = time.perf_counter()
+    r = Schedule(handle, M.rowptr, M.colidx, kind=kind, saved=blob)
+    print("[save/load]", name, len(blob), "bytes, load", round(time.perf_counter() - t0, 4), "s", r.plan())
+    assert r.plan() == s.plan() and r.info() == s.info()
+    for k in list(L.EXPORTS):
+        np.testing.assert_array_equal(r.export(k), s.export(k), err_msg=k)
+    b = g.rhs_uniform(Lw.n, 0)
+    if kind == 0:
+        lv = up(Lw.values)
+        np.testing.assert_array_equal(down(r.solve(lv, up(b))), down(s.solve(lv, up(b))))
+    else:
+        av = up(A.values)
+        f = "ic0" if kind == 1 else "ilu0"
+        np.testing.assert_array_equal(down(getattr(r, f)(av)), down(getattr(s, f)(av)))
+    handle.check()
+    with pytest.raises(L.BaasError):  # another kind
+        Schedule(handle, M.rowptr, M.colidx, kind=(kind + 1) % 3, saved=blob)
+    # another valid pattern of the same size (a symmetric renumbering)
+    import scipy.sparse as sps
+    perm = np.random.default_rng(9).permutation(Lw.n)
+    S = Lw.to_scipy()
+    full = A if A is not None else g.Csr.from_scipy(S + S.T - sps.diags(S.diagonal()))
+    other = g.symmetric_permute(full, perm)
+    other = other.lower() if kind == 0 else other
+    assert other.n == M.n and len(other.colidx) == len(M.colidx)
+    with pytest.raises(L.BaasError):
+        Schedule(handle, other.rowptr, other.colidx, kind=kind, saved=blob)
+    with pytest.raises(L.BaasError):  # another size
+        small = g.laplace2d_lower(32)
+        Schedule(handle, small.rowptr, small.colidx, kind=kind, saved=blob)
+    bad = bytearray(blob)
+    bad[0] ^= 1
+    with pytest.raises(L.BaasError):
+        Schedule(handle, M.rowptr, M.colidx, kind=kind, saved=bytes(bad))
+    with pytest.raises(L.BaasError):
+        Schedule(handle, M.rowptr, M.colidx, kind=kind, saved=blob[: len(blob) // 2])
