@@ -5,8 +5,9 @@ on the same inputs, in the container that has the reference tree.
 
 TEST INFRASTRUCTURE ONLY.  The reference's serial IC0 (spic0_csr_uL_serial,
 merge/IC.cpp:32-89), ILU0 (ilu0csr_uplooking_ref, merge/ILU.cpp:350-386) and
-SpMV (spmv_omp, blas/spmv.cpp:8-27) are timed against oracle.c's
-restatements; its triangular solves (merge/SpTRSV.cpp) do not build here
+SpMV (spmv_omp, blas/spmv.cpp:8-27) and the buildset() analysis
+(ref_baas_schedule: the call sequence of test/SpTRSVRuntime.hpp:675-735) are
+timed against oracle.c's restatements; its triangular solves (merge/SpTRSV.cpp) do not build here
 (papi.h is absent), so the solve is timed on the restatement only.  Prints a
 Markdown table.
 """
@@ -57,9 +58,14 @@ def main():
              lambda: (lu.__setitem__(slice(None), va), O.lib.orc_ilu0(A.n, rp, ci, dp, lu))),
             ("SpMV", lambda: ref.ref_spmv(A.n, rp, ci, va, x, y), lambda: O.lib.orc_spmv(A.n, rp, ci, va, x, y)),
         )
+        L, _, _ = workload(name)
+        cases = cases + (
+            ("analysis (buildset, SpTRSV, ncore 8)",
+             lambda: O.baas_schedule(0, L.n, L.rowptr, L.colidx, ncore=8, use_ref=True),
+             lambda: O.baas_schedule(0, L.n, L.rowptr, L.colidx, ncore=8)),)
         for kern, f_ref, f_orc in cases:
             r, o = best_of(f_ref), best_of(f_orc)
-            print(f"| {name} | {kern} | {r:.3f} | {o:.3f} | {o / r:.2f} |")
+            print(f"| {name} | {kern} | {r:.3f} | {o:.3f} | {o / r:.2f} |", flush=True)
 
 
 if __name__ == "__main__":
