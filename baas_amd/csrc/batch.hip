@@ -8,8 +8,9 @@
 // FP64 with one (alpha, beta, ||r||) per system and right-hand side; a
 // column stops updating at its own first ||r||/||r0|| < tol, the whole batch
 // stops when every column has.  `chunk` iterations are captured in a hipGraph.
-// The execution plan's unit sizes are chosen per system-size bucket by the
-// autotuner (timed FP32 sweeps of the actual batch).
+// The execution plan -- the level-set plan, boxes of each system's own
+// stencil grid on the multi-RHS lane kernel, or row ranges aligned to
+// systems -- is chosen by timing FP32 sweeps of the actual batch.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -23,6 +24,7 @@ namespace baas {
 int build_plans_mode(baas_schedule *s, int mode, int rows_per_unit);
 
 int make_plan_custom(baas_schedule *s, const std::vector<int> &unit_ptr, int block);  // plan.cpp
+size_t lds_limit(baas_handle *h);
 
 struct Chunk {
     int sys, r0, r1;
@@ -39,7 +41,8 @@ __global__ void k_spmv_multi(int N, int k, const int *__restrict__ rp, const int
     if (stopped(alldone)) return;
     long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long long)N * k) return;
-    int i = (int)(t / k), r = (int)(t - (long long)i * k);
+    const int sh = __builtin_ctz(k);  // k is a power of two
+    int i = (int)(t >> sh), r = (int)(t & (k - 1));
     const int b = rp[i], e = rp[i + 1];
     double s = 0.0;
     for (int j0 = b; j0 < e; j0 += CH) {
@@ -91,13 +94,131 @@ __global__ void k_chunk_dot(int k, const Chunk *__restrict__ ch, const double *_
     }
 }
 
+// The iteration's fused kernels: one workgroup per chunk (256 rows of one
+// system, every right-hand side), each element computed where it is first
+// needed and its dot-product term accumulated in the same pass, in the
+// order k_chunk_dot uses (thread t walks elements t, t + 256, ... of the
+// chunk: always column t % k), so the partial sums are the same bits.
+__device__ __forceinline__ void chunk_reduce(int k, double acc, double *red, double *part) {
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    if (threadIdx.x < (unsigned)k) {
+        double s = 0.0;
+        for (int q = threadIdx.x; q < (int)blockDim.x; q += k) s += red[q];
+        part[(size_t)blockIdx.x * k + threadIdx.x] = s;
+    }
+}
+
+// Ap = A p and the chunk's p . Ap partials
+template <int CH>
+__global__ void k_spmv_pap(int k, const Chunk *__restrict__ ch, const int *__restrict__ rp, const int *__restrict__ ci,
+                           const double *__restrict__ va, const double *__restrict__ p, double *__restrict__ Ap,
+                           double *part, const int *alldone, const int *__restrict__ coldone) {
+    extern __shared__ double red[];
+    if (stopped(alldone)) return;
+    const Chunk c = ch[blockIdx.x];
+    const int sh = __builtin_ctz(k), col = (int)(threadIdx.x & (unsigned)(k - 1));
+    double acc = 0.0;
+    if (!coldone[c.sys * k + col]) {
+        for (size_t e = (size_t)c.r0 * k + threadIdx.x; e < (size_t)c.r1 * k; e += blockDim.x) {
+            const int i = (int)(e >> sh);
+            const int b = rp[i], en = rp[i + 1];
+            double s = 0.0;
+            for (int j0 = b; j0 < en; j0 += CH) {
+                double xv[CH], av[CH];
+#pragma unroll
+                for (int q = 0; q < CH; q++) {
+                    const int j = j0 + q < en ? j0 + q : en - 1;
+                    av[q] = va[j];
+                    xv[q] = p[(size_t)ci[j] * k + col];
+                }
+#pragma unroll
+                for (int q = 0; q < CH; q++)
+                    if (j0 + q < en) s = fma(av[q], xv[q], s);
+            }
+            Ap[e] = s;
+            acc = fma(p[e], s, acc);
+        }
+    }
+    chunk_reduce(k, acc, red, part);
+}
+
+// x += alpha p, r -= alpha Ap, r32 = (float) r and the chunk's r . r partials
+__global__ void k_update_xr_rr(int k, const Chunk *__restrict__ ch, double *x, double *r,
+                               const double *__restrict__ p, const double *__restrict__ Ap,
+                               const double *__restrict__ rz, const double *__restrict__ pap,
+                               const int *__restrict__ coldone, float *r32, double *part, const int *alldone) {
+    extern __shared__ double red[];
+    if (stopped(alldone)) return;
+    const Chunk c = ch[blockIdx.x];
+    const int col = (int)(threadIdx.x & (unsigned)(k - 1)), cc = c.sys * k + col;
+    double acc = 0.0;
+    if (!coldone[cc]) {
+        const double alpha = rz[cc] / pap[cc];
+        for (size_t e = (size_t)c.r0 * k + threadIdx.x; e < (size_t)c.r1 * k; e += blockDim.x) {
+            x[e] = fma(alpha, p[e], x[e]);
+            const double ri = fma(-alpha, Ap[e], r[e]);
+            r[e] = ri;
+            r32[e] = (float)ri;
+            acc = fma(ri, ri, acc);
+        }
+    }
+    chunk_reduce(k, acc, red, part);
+}
+
+// the chunk's r . z partials with z = (double) z32 (exact)
+__global__ void k_rz32(int k, const Chunk *__restrict__ ch, const double *__restrict__ r,
+                       const float *__restrict__ z32, double *part, const int *alldone,
+                       const int *__restrict__ coldone) {
+    extern __shared__ double red[];
+    if (stopped(alldone)) return;
+    const Chunk c = ch[blockIdx.x];
+    const int col = (int)(threadIdx.x & (unsigned)(k - 1));
+    double acc = 0.0;
+    if (!(coldone && coldone[c.sys * k + col]))
+        for (size_t e = (size_t)c.r0 * k + threadIdx.x; e < (size_t)c.r1 * k; e += blockDim.x)
+            acc = fma(r[e], (double)z32[e], acc);
+    chunk_reduce(k, acc, red, part);
+}
+
+// p = z + (rz_new / rz_old) p for running columns, z = (double) z32; with
+// rz_old == nullptr p = z (the first direction)
+__global__ void k_update_p32(int k, const Chunk *__restrict__ ch, double *p, const float *__restrict__ z32,
+                             const double *__restrict__ rz_new, const double *__restrict__ rz_old,
+                             const int *__restrict__ coldone, const int *alldone) {
+    if (stopped(alldone)) return;
+    const Chunk c = ch[blockIdx.x];
+    const int col = (int)(threadIdx.x & (unsigned)(k - 1)), cc = c.sys * k + col;
+    if (coldone && coldone[cc]) return;
+    if (!rz_old) {
+        for (size_t e = (size_t)c.r0 * k + threadIdx.x; e < (size_t)c.r1 * k; e += blockDim.x) p[e] = (double)z32[e];
+        return;
+    }
+    const double beta = rz_new[cc] / rz_old[cc];
+    for (size_t e = (size_t)c.r0 * k + threadIdx.x; e < (size_t)c.r1 * k; e += blockDim.x)
+        p[e] = fma(beta, p[e], (double)z32[e]);
+}
+
+// per work unit of a plan: skip it when every column of its system is done
+__global__ void k_unit_skip(int nunits, int k, const int *__restrict__ usys, const int *__restrict__ coldone,
+                            int *usk, const int *alldone) {
+    if (stopped(alldone)) return;
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nunits) return;
+    const int *c = coldone + (size_t)usys[u] * k;
+    int all = 1;
+    for (int r = 0; r < k; r++) all &= c[r] != 0;
+    usk[u] = all;
+}
+
 // per system and column: sum the chunk partials of the system in order
 __global__ void k_sys_sum(int nsys, int k, const int *__restrict__ sys_chunk, const double *__restrict__ part,
                           double *out, int do_sqrt, const int *alldone) {
     if (stopped(alldone)) return;
     int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nsys * k) return;
-    int s = t / k, r = t - s * k;
+    const int sh = __builtin_ctz(k);
+    int s = t >> sh, r = t & (k - 1);
     double acc = 0.0;
     for (int c = sys_chunk[s]; c < sys_chunk[s + 1]; c++) acc += part[(size_t)c * k + r];
     out[t] = do_sqrt ? sqrt(acc) : acc;
@@ -111,25 +232,6 @@ __global__ void k_bres(long long M, const double *__restrict__ b, const double *
     double v = b[t] - Ap[t];
     r[t] = v;
     r32[t] = (float)v;
-}
-
-// x += alpha p, r -= alpha Ap for columns still running; r32 = (float) r
-__global__ void k_bupdate_xr(int N, int k, const int *__restrict__ sys_of, double *x, double *r,
-                             const double *__restrict__ p, const double *__restrict__ Ap, const double *__restrict__ rz,
-                             const double *__restrict__ pap, const int *__restrict__ coldone, float *r32,
-                             const int *alldone) {
-    if (stopped(alldone)) return;
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long long)N * k) return;
-    int i = (int)(t / k), c = sys_of[i] * k + (int)(t - (long long)i * k);
-    double ri = r[t];
-    if (!coldone[c]) {
-        const double alpha = rz[c] / pap[c];
-        x[t] = fma(alpha, p[t], x[t]);
-        ri = fma(-alpha, Ap[t], ri);
-        r[t] = ri;
-    }
-    r32[t] = (float)ri;
 }
 
 // convergence per column (rel = ||r|| / ||r0|| < tol), count of running columns
@@ -155,24 +257,6 @@ __global__ void k_bconv_finish(int *ctl) {
     ctl[1] = it;
     if (ctl[2] == 0 || it >= ctl[3]) st_agent(&ctl[0], 1);
     ctl[2] = 0;
-}
-
-__global__ void k_cast_back(long long M, const float *__restrict__ z32, double *__restrict__ z, const int *alldone) {
-    if (stopped(alldone)) return;
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < M) z[t] = (double)z32[t];
-}
-
-// p = z + (rz_new / rz_old) p for running columns
-__global__ void k_bupdate_p(int N, int k, const int *__restrict__ sys_of, double *p, const double *__restrict__ z,
-                            const double *__restrict__ rz_new, const double *__restrict__ rz_old,
-                            const int *__restrict__ coldone, const int *alldone) {
-    if (stopped(alldone)) return;
-    long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= (long long)N * k) return;
-    int i = (int)(t / k), c = sys_of[i] * k + (int)(t - (long long)i * k);
-    if (coldone[c]) return;
-    p[t] = fma(rz_new[c] / rz_old[c], p[t], z[t]);
 }
 
 __global__ void k_bcopy(long long M, const double *__restrict__ a, double *__restrict__ b) {
@@ -207,11 +291,15 @@ struct baas_batch {
     const double *a_vals = nullptr;
     const int *rowptr = nullptr, *colidx = nullptr;
     std::vector<int> sys_ptr;
-    DBuf<int> sys_of, sys_chunk;
+    DBuf<int> sys_chunk;
     DBuf<Chunk> chunks;
-    DBuf<double> lu, x, r, z, p, Ap, part, rz0, rz1, pap, rr, norm0, tol;
+    DBuf<double> lu, x, r, p, Ap, part, rz0, rz1, pap, rr, norm0, tol;
     DBuf<float> lv32, uv32, r32, y32, z32;
     DBuf<int> coldone, coliter, ctl;
+    // per unit of the forward / backward plan: its system, and the skip
+    // flags the iteration sets for units of finished systems (box plans)
+    DBuf<int> usys_f, usys_b, usk_f, usk_b;
+    int nu_f = 0, nu_b = 0;
     int *h_ctl = nullptr;  // pinned
     hipStream_t st = nullptr;
     hipEvent_t ev = nullptr;
@@ -240,9 +328,42 @@ static std::vector<int> batch_units(const baas_batch *B, const std::vector<int> 
     return up;
 }
 
-static int batch_precond(baas_batch *B, const int *skip) {
-    BAAS_TRY(sweep<float>(B->h, B->s, 0, B->lv32.p, B->r32.p, B->y32.p, B->k, B->st, skip));
-    BAAS_TRY(sweep<float>(B->h, B->s, 1, B->uv32.p, B->y32.p, B->z32.p, B->k, B->st, skip));
+static int batch_precond(baas_batch *B, const int *skip, bool skip_done = false) {
+    const int *uf = skip_done && B->nu_f ? B->usk_f.p : nullptr, *ub = skip_done && B->nu_b ? B->usk_b.p : nullptr;
+    BAAS_TRY(sweep<float>(B->h, B->s, 0, B->lv32.p, B->r32.p, B->y32.p, B->k, B->st, skip, nullptr, 0, uf));
+    BAAS_TRY(sweep<float>(B->h, B->s, 1, B->uv32.p, B->y32.p, B->z32.p, B->k, B->st, skip, nullptr, 0, ub));
+    return BAAS_OK;
+}
+
+// The system of every unit of the plans in use (each unit of a box plan
+// lies in one system), for skipping finished systems' units; other plans
+// get none.
+static int build_unit_sys(baas_batch *B) {
+    B->nu_f = B->nu_b = 0;
+    if (B->s->fwd.mode != BAAS_PLAN_BOXES || B->s->fwd.dataflow < 3) return BAAS_OK;
+    std::vector<int> sys_of(B->N);
+    for (int s = 0; s < B->nsys; s++)
+        for (int q = B->sys_ptr[s]; q < B->sys_ptr[s + 1]; q++) sys_of[q] = s;
+    int nus[2] = {0, 0};
+    for (int dir = 0; dir < 2; dir++) {
+        const Plan &p = dir ? B->s->bwd : B->s->fwd;
+        const int nu = p.nunits;
+        if ((int)p.h_unit_ptr.size() != nu + 1 || (int)p.h_rows.size() != B->N) return BAAS_OK;
+        std::vector<int> us(nu);
+        for (int u = 0; u < nu; u++) {
+            us[u] = sys_of[p.h_rows[p.h_unit_ptr[u]]];
+            for (int q = p.h_unit_ptr[u]; q < p.h_unit_ptr[u + 1]; q++)
+                if (sys_of[p.h_rows[q]] != us[u]) return BAAS_OK;  // a unit across systems: no skipping
+        }
+        DBuf<int> &d = dir ? B->usys_b : B->usys_f;
+        DBuf<int> &f = dir ? B->usk_b : B->usk_f;
+        BAAS_TRY(d.alloc(nu));
+        BAAS_TRY(f.alloc(nu));
+        BAAS_HIP_TRY(hipMemcpy(d.p, us.data(), sizeof(int) * nu, hipMemcpyHostToDevice));
+        nus[dir] = nu;
+    }
+    B->nu_f = nus[0];
+    B->nu_b = nus[1];
     return BAAS_OK;
 }
 
@@ -264,27 +385,22 @@ static float time_sweeps(baas_batch *B, int reps) {
     return best;
 }
 
-// Coordinate search over rows-per-unit, one size bucket at a time, timing
-// the FP32 forward + backward sweeps of the whole batch with all RHS; then
-// the level-set plan (all systems' rows of one level at a time), which is
-// kept if faster.
+// Plan search for the batch, timing the FP32 forward + backward sweeps of
+// the whole batch with all RHS: the level-set plan, per-system boxes on the
+// multi-RHS lane kernel, and (where boxes do not apply) row ranges aligned
+// to systems; the fastest is kept.
 static int autotune(baas_batch *B) {
     auto t0 = std::chrono::steady_clock::now();
-    const int cand[] = {512, 1024, 2048, 4096, 8192};
-    std::vector<int> rpu = {2048, 2048, 2048};
     const int block = B->s->opts.block_threads;
-    BAAS_TRY(make_plan_custom(B->s, batch_units(B, rpu), block));
-    float best = time_sweeps(B, 3);
-    for (int bk = 0; bk < 3; bk++) {
-        for (int c : cand) {
-            if (c == rpu[bk]) continue;
-            std::vector<int> trial = rpu;
-            trial[bk] = c;
-            BAAS_TRY(make_plan_custom(B->s, batch_units(B, trial), block));
-            float t = time_sweeps(B, 3);
-            if (t < best) best = t, rpu = trial;
-        }
-    }
+    auto done = [&](std::vector<int> rpu) {
+        B->s->opts.block_threads = block;
+        B->tuned_rpu = rpu;
+        B->tuned = true;
+        B->t_tune = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        return BAAS_OK;
+    };
+    // the level-set plan (all systems' rows of one level at a time)
+    float best = 1e30f;
     int lev_block = 0;
     for (int lb : {256, 512, 1024}) {
         B->s->opts.block_threads = lb;
@@ -292,21 +408,60 @@ static int autotune(baas_batch *B) {
         const float t_lev = time_sweeps(B, 3);
         if (t_lev < best) best = t_lev, lev_block = lb;
     }
-    B->s->opts.block_threads = block;
-    if (lev_block) {
-        B->s->opts.block_threads = lev_block;
-        BAAS_TRY(build_plans_mode(B->s, BAAS_PLAN_LEVELS, 0));
-        B->s->opts.block_threads = block;
-        B->tuned_rpu = {0, 0, lev_block};
-        B->tuned = true;
-        B->t_tune = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        return BAAS_OK;
+    // every system a stencil grid: boxes of each system's own grid on the
+    // multi-RHS lane kernel (b staged in LDS or read from memory), no
+    // device-wide barrier; a system that is not a stencil makes the plan
+    // unsupported
+    int box_rows = 0, box_block = 0, box_df = 0;
+    bool boxes = true;
+    for (int R : {64, 144, 256, 400, 576}) {
+        for (int blk : {512, 1024}) {
+            B->s->opts.block_threads = blk;
+            if (build_plans_mode(B->s, BAAS_PLAN_BOXES, R) != BAAS_OK) {
+                boxes = false;
+                break;
+            }
+            for (int df : {3, 4}) {
+                B->s->fwd.dataflow = B->s->bwd.dataflow = df;
+                if (B->s->fwd.mr_lds((int)sizeof(float), B->k, false) > lds_limit(B->h) ||
+                    B->s->bwd.mr_lds((int)sizeof(float), B->k, false) > lds_limit(B->h))
+                    continue;
+                const float t = time_sweeps(B, 3);
+                if (t < best) best = t, box_rows = R, box_block = blk, box_df = df;
+            }
+        }
+        if (!boxes) break;
     }
-    BAAS_TRY(make_plan_custom(B->s, batch_units(B, rpu), block));
-    B->tuned_rpu = rpu;
-    B->tuned = true;
-    B->t_tune = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-    return BAAS_OK;
+    if (box_rows) {
+        B->s->opts.block_threads = box_block;
+        BAAS_TRY(build_plans_mode(B->s, BAAS_PLAN_BOXES, box_rows));
+        B->s->fwd.dataflow = B->s->bwd.dataflow = box_df;
+        return done({box_rows, box_rows, box_rows});
+    }
+    // otherwise row ranges aligned to systems: a coordinate search over rows
+    // per unit, one size bucket at a time (only where boxes do not apply)
+    std::vector<int> rpu = {2048, 2048, 2048};
+    bool range_won = false;
+    if (!boxes) {
+        const int cand[] = {512, 1024, 2048, 4096, 8192};
+        for (int bk = 0; bk < 3; bk++) {
+            for (int c : cand) {
+                std::vector<int> trial = rpu;
+                trial[bk] = c;
+                if (c == rpu[bk] && bk > 0) continue;
+                BAAS_TRY(make_plan_custom(B->s, batch_units(B, trial), block));
+                const float t = time_sweeps(B, 3);
+                if (t < best) best = t, rpu = trial, range_won = true;
+            }
+        }
+    }
+    if (range_won) {
+        BAAS_TRY(make_plan_custom(B->s, batch_units(B, rpu), block));
+        return done(rpu);
+    }
+    B->s->opts.block_threads = lev_block;
+    BAAS_TRY(build_plans_mode(B->s, BAAS_PLAN_LEVELS, 0));
+    return done({0, 0, lev_block});
 }
 
 static int chunk_dot_sys(baas_batch *B, const double *u, const double *v, double *out, int do_sqrt,
@@ -319,24 +474,48 @@ static int chunk_dot_sys(baas_batch *B, const double *u, const double *v, double
     return BAAS_OK;
 }
 
+static int sys_sum(baas_batch *B, double *out, int do_sqrt, const int *skip) {
+    hipLaunchKernelGGL(k_sys_sum, dim3(nblk((long long)B->nsys * B->k)), dim3(256), 0, B->st, B->nsys, B->k,
+                       B->sys_chunk.p, B->part.p, out, do_sqrt, skip);
+    BAAS_HIP_TRY(hipGetLastError());
+    return BAAS_OK;
+}
+
 static int batch_iteration(baas_batch *B, int cur) {
-    const int N = B->N, k = B->k, ncol = B->nsys * B->k;
-    const long long M = (long long)N * k;
+    const int k = B->k, ncol = B->nsys * B->k;
     int *ctl = B->ctl.p;
+    const int *cd = B->coldone.p;
     double *rz_old = cur ? B->rz1.p : B->rz0.p, *rz_new = cur ? B->rz0.p : B->rz1.p;
-    launch_spmv_multi(B->s->fwd.max_deg, B->st, N, k, B->rowptr, B->colidx, B->a_vals, B->p.p, B->Ap.p, ctl);
-    BAAS_TRY(chunk_dot_sys(B, B->p.p, B->Ap.p, B->pap.p, 0, ctl));
-    hipLaunchKernelGGL(k_bupdate_xr, dim3(nblk(M)), dim3(256), 0, B->st, N, k, B->sys_of.p, B->x.p, B->r.p, B->p.p,
-                       B->Ap.p, rz_old, B->pap.p, B->coldone.p, B->r32.p, ctl);
-    BAAS_TRY(chunk_dot_sys(B, B->r.p, B->r.p, B->rr.p, 0, ctl));
+    const dim3 grid(B->nchunks), block(256);
+    const size_t red = 256 * sizeof(double);
+    const int w = 2 * B->s->fwd.max_deg + 1;  // the symmetric pattern's longest row
+    if (w <= 5)
+        hipLaunchKernelGGL(k_spmv_pap<5>, grid, block, red, B->st, k, B->chunks.p, B->rowptr, B->colidx, B->a_vals,
+                           B->p.p, B->Ap.p, B->part.p, ctl, cd);
+    else if (w <= 7)
+        hipLaunchKernelGGL(k_spmv_pap<7>, grid, block, red, B->st, k, B->chunks.p, B->rowptr, B->colidx, B->a_vals,
+                           B->p.p, B->Ap.p, B->part.p, ctl, cd);
+    else
+        hipLaunchKernelGGL(k_spmv_pap<8>, grid, block, red, B->st, k, B->chunks.p, B->rowptr, B->colidx, B->a_vals,
+                           B->p.p, B->Ap.p, B->part.p, ctl, cd);
+    BAAS_TRY(sys_sum(B, B->pap.p, 0, ctl));
+    hipLaunchKernelGGL(k_update_xr_rr, grid, block, red, B->st, k, B->chunks.p, B->x.p, B->r.p, B->p.p, B->Ap.p,
+                       rz_old, B->pap.p, cd, B->r32.p, B->part.p, ctl);
+    BAAS_TRY(sys_sum(B, B->rr.p, 0, ctl));
     hipLaunchKernelGGL(k_bconv, dim3(nblk(ncol)), dim3(256), 0, B->st, ncol, B->rr.p, B->norm0.p, B->tol.p,
                        B->coldone.p, B->coliter.p, ctl);
     hipLaunchKernelGGL(k_bconv_finish, dim3(1), dim3(1), 0, B->st, ctl);
-    BAAS_TRY(batch_precond(B, ctl));
-    hipLaunchKernelGGL(k_cast_back, dim3(nblk(M)), dim3(256), 0, B->st, M, B->z32.p, B->z.p, ctl);
-    BAAS_TRY(chunk_dot_sys(B, B->r.p, B->z.p, rz_new, 0, ctl));
-    hipLaunchKernelGGL(k_bupdate_p, dim3(nblk(M)), dim3(256), 0, B->st, N, k, B->sys_of.p, B->p.p, B->z.p, rz_new,
-                       rz_old, B->coldone.p, ctl);
+    if (B->nu_f && B->nu_b) {
+        hipLaunchKernelGGL(k_unit_skip, dim3(nblk(B->nu_f)), dim3(256), 0, B->st, B->nu_f, k, B->usys_f.p, cd,
+                           B->usk_f.p, ctl);
+        hipLaunchKernelGGL(k_unit_skip, dim3(nblk(B->nu_b)), dim3(256), 0, B->st, B->nu_b, k, B->usys_b.p, cd,
+                           B->usk_b.p, ctl);
+    }
+    BAAS_TRY(batch_precond(B, ctl, B->nu_f && B->nu_b));
+    hipLaunchKernelGGL(k_rz32, grid, block, red, B->st, k, B->chunks.p, B->r.p, B->z32.p, B->part.p, ctl, cd);
+    BAAS_TRY(sys_sum(B, rz_new, 0, ctl));
+    hipLaunchKernelGGL(k_update_p32, grid, block, 0, B->st, k, B->chunks.p, B->p.p, B->z32.p, rz_new, rz_old, cd,
+                       ctl);
     BAAS_HIP_TRY(hipGetLastError());
     return BAAS_OK;
 }
@@ -388,14 +567,14 @@ extern "C" int baas_batch_create(baas_handle *h, int nsys, const int *sys_ptr_ho
     if (opts) o = *opts;
     o.plan_mode = BAAS_PLAN_RANGE;
     int rc = baas_analyze(h, N, rowptr, colidx, BAAS_KIND_IC0, &o, &B->s);
+    if (rc == BAAS_OK) B->s->block_ptr = B->sys_ptr;  // BOXES plans cut each system's grid
     // independent systems: the level-set plan runs every system's level at
     // once (the first factorisation uses it; the autotuner may keep it)
     if (rc == BAAS_OK) rc = build_plans_mode(B->s, BAAS_PLAN_LEVELS, 0);
-    std::vector<int> sys_of(N), sys_chunk(nsys + 1);
+    std::vector<int> sys_chunk(nsys + 1);
     std::vector<Chunk> ch;
     for (int s = 0; s < nsys && rc == BAAS_OK; s++) {
         sys_chunk[s] = (int)ch.size();
-        for (int q = sys_ptr_host[s]; q < sys_ptr_host[s + 1]; q++) sys_of[q] = s;
         for (int q = sys_ptr_host[s]; q < sys_ptr_host[s + 1]; q += 256)
             ch.push_back(Chunk{s, q, std::min(q + 256, sys_ptr_host[s + 1])});
     }
@@ -407,13 +586,12 @@ extern "C" int baas_batch_create(baas_handle *h, int nsys, const int *sys_ptr_ho
         if (rc == BAAS_OK) rc = r;
     };
     if (rc == BAAS_OK) {
-        A(B->sys_of.alloc(N));
         A(B->sys_chunk.alloc(nsys + 1));
         A(B->chunks.alloc(ch.size()));
         A(B->lu.alloc(B->s->nnz_full));
         A(B->lv32.alloc(nl));
         A(B->uv32.alloc(nl));
-        for (auto *d : {&B->x, &B->r, &B->z, &B->p, &B->Ap}) A(d->alloc(M));
+        for (auto *d : {&B->x, &B->r, &B->p, &B->Ap}) A(d->alloc(M));
         for (auto *f : {&B->r32, &B->y32, &B->z32}) A(f->alloc(M));
         A(B->part.alloc((size_t)B->nchunks * nrhs));
         for (auto *d : {&B->rz0, &B->rz1, &B->pap, &B->rr, &B->norm0}) A(d->alloc(ncol));
@@ -427,8 +605,7 @@ extern "C" int baas_batch_create(baas_handle *h, int nsys, const int *sys_ptr_ho
     if (rc == BAAS_OK && hipHostMalloc((void **)&B->h_ctl, 4 * sizeof(int), hipHostMallocDefault) != hipSuccess)
         rc = BAAS_HIP_ERROR;
     if (rc == BAAS_OK) {
-        if (hipMemcpy(B->sys_of.p, sys_of.data(), N * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(B->sys_chunk.p, sys_chunk.data(), (nsys + 1) * sizeof(int), hipMemcpyHostToDevice) !=
+        if (hipMemcpy(B->sys_chunk.p, sys_chunk.data(), (nsys + 1) * sizeof(int), hipMemcpyHostToDevice) !=
                 hipSuccess ||
             hipMemcpy(B->chunks.p, ch.data(), ch.size() * sizeof(Chunk), hipMemcpyHostToDevice) != hipSuccess)
             rc = BAAS_HIP_ERROR;
@@ -480,13 +657,18 @@ extern "C" int baas_batch_pcg(baas_batch *B, const double *Bm, double *X, double
                        B->ctl.p, B->norm0.p);
     BAAS_HIP_TRY(hipStreamSynchronize(B->st));
     auto t1 = std::chrono::steady_clock::now();
-    if (!B->tuned) BAAS_TRY(autotune(B));  // times sweeps on r32 only; z32/y32 are recomputed below
+    if (!B->tuned) {
+        BAAS_TRY(autotune(B));  // times sweeps on r32 only; z32/y32 are recomputed below
+        BAAS_TRY(build_unit_sys(B));
+    }
     auto t2 = std::chrono::steady_clock::now();
     // z = M^-1 r, p = z, rz
     BAAS_TRY(batch_precond(B, nullptr));
-    hipLaunchKernelGGL(k_cast_back, dim3(nblk(M)), dim3(256), 0, B->st, M, B->z32.p, B->z.p, nullptr);
-    BAAS_TRY(chunk_dot_sys(B, B->r.p, B->z.p, B->rz0.p, 0, nullptr));
-    hipLaunchKernelGGL(k_bcopy, dim3(nblk(M)), dim3(256), 0, B->st, M, B->z.p, B->p.p);
+    hipLaunchKernelGGL(k_rz32, dim3(B->nchunks), dim3(256), 256 * sizeof(double), B->st, k, B->chunks.p, B->r.p,
+                       B->z32.p, B->part.p, (const int *)nullptr, (const int *)nullptr);
+    BAAS_TRY(sys_sum(B, B->rz0.p, 0, nullptr));
+    hipLaunchKernelGGL(k_update_p32, dim3(B->nchunks), dim3(256), 0, B->st, k, B->chunks.p, B->p.p, B->z32.p,
+                       (const double *)nullptr, (const double *)nullptr, (const int *)nullptr, (const int *)nullptr);
     BAAS_HIP_TRY(hipGetLastError());
     BAAS_TRY(batch_capture(B));
     for (;;) {

@@ -37,9 +37,11 @@ int dot_dev(baas_handle *h, int n, const double *x, const double *y, double *res
 // fixed_xp: device address of a caller-chosen publication area of
 // fixed_bytes (re-armed by this call) instead of the plan-owned one, which
 // the plan allocates on first use; nullptr = the plan-owned area.
+// unit_skip: optional per-unit flags of the plan in use (nonzero = skip;
+// honoured by the multi-RHS lane kernel only).
 template <class T>
 int sweep(baas_handle *h, baas_schedule *s, int uplo, const T *vals, const T *b, T *x, int nrhs, hipStream_t st,
-          const int *skip, void *fixed_xp = nullptr, size_t fixed_bytes = 0);
+          const int *skip, void *fixed_xp = nullptr, size_t fixed_bytes = 0, const int *unit_skip = nullptr);
 // Look up this device's copy of the allocation-free entry point's scratch
 // array (once per device; the device must be current).
 int noalloc_scratch_init(int device);

@@ -248,4 +248,7 @@ struct baas_schedule {
     bool icl_allfast = false;  // every row with lower entries is marked
     double t_gpu = 0, t_host = 0;
     baas_opts opts{};
+    // block-diagonal matrices (a batch of independent systems): the first
+    // row of each block and n; BOXES plans then cut each block's own grid
+    std::vector<int> block_ptr;
 };

@@ -410,6 +410,100 @@ static int box_units(const baas_schedule *s, const HostCsr &L, int R, std::vecto
     return BAAS_OK;
 }
 
+// Grid geometry of one stencil block from its lower pattern: at most three
+// distinct offsets 1 < sy < sz, sy dividing sz (rows r0 .. r1 of L); the box
+// edge for about R rows per box.
+struct BoxGeom {
+    long long nx = 0, ny = 1, nz = 1, bx = 1, by = 1, bz = 1, gx = 1, gy = 1, gz = 1;
+};
+static int box_geometry(const HostCsr &L, int r0, int r1, int R, BoxGeom &g) {
+    std::vector<long long> offs;
+    for (int r = r0; r < r1; r++)
+        for (int j = L.ptr[r]; j < L.ptr[r + 1] - 1; j++) {
+            const long long d = (long long)r - L.idx[j];
+            if (L.idx[j] < r0) return BAAS_UNSUPPORTED;  // a parent outside the block
+            if (std::find(offs.begin(), offs.end(), d) == offs.end()) {
+                offs.push_back(d);
+                if (offs.size() > 3) return BAAS_UNSUPPORTED;
+            }
+        }
+    std::sort(offs.begin(), offs.end());
+    const long long n = r1 - r0;
+    if (!offs.empty() && offs[0] != 1) return BAAS_UNSUPPORTED;
+    const long long sy = offs.size() >= 2 ? offs[1] : 0, sz = offs.size() == 3 ? offs[2] : 0;
+    if (sz && sz % sy != 0) return BAAS_UNSUPPORTED;
+    g.nx = sy ? sy : n;
+    g.ny = sz ? sz / sy : (sy ? (n + sy - 1) / sy : 1);
+    g.nz = sz ? (n + sz - 1) / sz : 1;
+    const int dims = (g.nx > 1) + (g.ny > 1) + (g.nz > 1);
+    const double edge = std::pow((double)R, 1.0 / std::max(dims, 1));
+    auto clampl = [](long long v, long long hi) { return std::max(1LL, std::min(v, hi)); };
+    g.bx = clampl(std::llround(edge), g.nx);
+    g.by = g.ny > 1 ? clampl(std::llround(edge), g.ny) : 1;
+    g.bz = g.nz > 1 ? clampl(R / (g.bx * g.by), g.nz) : 1;
+    g.gx = (g.nx + g.bx - 1) / g.bx;
+    g.gy = (g.ny + g.by - 1) / g.by;
+    g.gz = (g.nz + g.bz - 1) / g.bz;
+    return BAAS_OK;
+}
+
+// BOXES units of a block-diagonal matrix: each block is a stencil grid of
+// its own size, cut into boxes of about R rows; the boxes of all blocks are
+// claimed in one wavefront order (box-coordinate sum, then block, then box),
+// so every block's first boxes start at once and each box's parents (same
+// block, smaller sum) come earlier.
+static int block_box_units(const baas_schedule *s, const HostCsr &L, int R, std::vector<int> &order,
+                           std::vector<int> &fu) {
+    const std::vector<int> &bp = s->block_ptr;
+    const int nb = (int)bp.size() - 1;
+    if (R < 1 || nb < 1 || bp[0] != 0 || bp[nb] != s->n) return BAAS_INVALID_ARG;
+    std::vector<BoxGeom> geo(nb);
+    std::vector<long long> box0(nb + 1, 0);
+    for (int b = 0; b < nb; b++) {
+        BAAS_TRY(box_geometry(L, bp[b], bp[b + 1], R, geo[b]));
+        box0[b + 1] = box0[b] + geo[b].gx * geo[b].gy * geo[b].gz;
+    }
+    const long long nbox = box0[nb];
+    if (nbox > (1LL << 26)) return BAAS_INVALID_ARG;
+    struct Key {
+        long long d, b, t;
+    };
+    std::vector<Key> keys((size_t)nbox);
+    for (int b = 0; b < nb; b++) {
+        const BoxGeom &g = geo[b];
+        for (long long t = 0; t < box0[b + 1] - box0[b]; t++) {
+            const long long tx = t % g.gx, ty = (t / g.gx) % g.gy, tz = t / (g.gx * g.gy);
+            keys[(size_t)(box0[b] + t)] = Key{tx + ty + tz, b, t};
+        }
+    }
+    std::vector<long long> ids((size_t)nbox), rank((size_t)nbox);
+    for (long long q = 0; q < nbox; q++) ids[q] = q;
+    std::sort(ids.begin(), ids.end(), [&](long long a, long long c) {
+        const Key &x = keys[a], &y = keys[c];
+        return x.d != y.d ? x.d < y.d : (x.b != y.b ? x.b < y.b : x.t < y.t);
+    });
+    for (long long q = 0; q < nbox; q++) rank[ids[q]] = q;
+    const int n = s->n;
+    std::vector<long long> box_of(n);
+    for (int b = 0; b < nb; b++) {
+        const BoxGeom &g = geo[b];
+        for (int r = bp[b]; r < bp[b + 1]; r++) {
+            const long long q = r - bp[b];
+            const long long x = q % g.nx, y = (q / g.nx) % g.ny, z = q / (g.nx * g.ny);
+            box_of[r] = rank[box0[b] + ((z / g.bz) * g.gy + y / g.by) * g.gx + x / g.bx];
+        }
+    }
+    std::vector<int> start((size_t)nbox + 1, 0);
+    for (int r = 0; r < n; r++) start[box_of[r] + 1]++;
+    for (long long t = 0; t < nbox; t++) start[t + 1] += start[t];
+    std::vector<int> next(start.begin(), start.end() - 1);
+    for (int r = 0; r < n; r++) order[next[box_of[r]]++] = r;
+    for (long long t = 0; t < nbox; t++)
+        if (start[t + 1] > start[t]) fu.push_back(start[t]);
+    fu.push_back(n);
+    return BAAS_OK;
+}
+
 int build_plans_mode(baas_schedule *s, int mode, int rows_per_unit) {
     s->plan_gen++;
     hipStream_t st = s->h->stream;
@@ -449,7 +543,10 @@ int build_plans_mode(baas_schedule *s, int mode, int rows_per_unit) {
     if (mode == BAAS_PLAN_TILES) {
         BAAS_TRY(tile_units(s, rows_per_unit > 0 ? rows_per_unit : 16384, s->tile_band, fwd_order, fu));
     } else if (mode == BAAS_PLAN_BOXES) {
-        BAAS_TRY(box_units(s, L, rows_per_unit > 0 ? rows_per_unit : 512, fwd_order, fu));
+        if (!s->block_ptr.empty())
+            BAAS_TRY(block_box_units(s, L, rows_per_unit > 0 ? rows_per_unit : 512, fwd_order, fu));
+        else
+            BAAS_TRY(box_units(s, L, rows_per_unit > 0 ? rows_per_unit : 512, fwd_order, fu));
     } else {
         BAAS_TRY(schedule_units(s, rows_per_unit > 0 ? rows_per_unit : block, fwd_order, fu));
     }

@@ -137,6 +137,10 @@ struct DfArgs {
     int *pcount;  // the plan's sweep counter (parity selects the publication half)
     Ctx ctx;
     const int *skip;
+    // per work unit, nonzero = skip the unit (multi-RHS lane kernel only):
+    // its rows' values are not needed and no unit that reads them runs (a
+    // batch's converged systems); the unit only re-arms its published rows
+    const int *unit_skip;
     DevStatus *status;
     long long spin_limit;
 };
@@ -760,6 +764,16 @@ __global__ __launch_bounds__(1024) void k_sweep_mr(DfArgs a, const int *__restri
         const int u = claim_unit(a.ctx.ticket, &s_unit);
         if (u >= a.nunits) break;
         const int q0 = a.unit_ptr[u], q1 = a.unit_ptr[u + 1], nk = q1 - q0;
+        if (a.unit_skip && a.unit_skip[u]) {
+            // the other half of every published row must be the sentinel
+            // when the next sweep starts, as after a solved unit
+            for (int i = threadIdx.x; i < nk * nr; i += blockDim.x) {
+                const int q = i / nr;
+                const int mx = a.meta[q0 + q].x;
+                if (mx < 0) nxt[(size_t)(mx & 0x7fffffff) * nr + (i - q * nr)] = Sentinel<T>::value();
+            }
+            continue;
+        }
         const int e0 = a.ext_ptr[u];
         const MrLayout lay = mr_layout(nk, eptr[u + 1] - eptr[u], sizeof(T), nr, SB);
         T *sl = reinterpret_cast<T *>(smem_raw + lay.sl);
@@ -1148,7 +1162,7 @@ static int launch_levels(baas_handle *h, baas_schedule *s, const Plan &p, int up
 
 template <class T>
 int sweep(baas_handle *h, baas_schedule *s, int uplo, const T *vals, const T *b, T *x, int nrhs,
-          hipStream_t stream, const int *skip, void *fixed_xp, size_t fixed_bytes) {
+          hipStream_t stream, const int *skip, void *fixed_xp, size_t fixed_bytes, const int *unit_skip) {
     if (!h || !s || !vals || !b || !x || nrhs < 1 || (uplo != 0 && uplo != 1)) return BAAS_INVALID_ARG;
     if ((const void *)b == (const void *)x) return BAAS_INVALID_ARG;
     // a fixed publication area must hold both halves at any plan's layout
@@ -1202,6 +1216,7 @@ int sweep(baas_handle *h, baas_schedule *s, int uplo, const T *vals, const T *b,
         d.ext_cols = p.ext_cols.p;
         d.ctx = Ctx{h->counters, h->counters + 1, h->counters + 2};
         d.skip = skip;
+        d.unit_skip = mr ? unit_skip : nullptr;
         d.status = h->status;
         d.spin_limit = h->spin_limit;
         if (mr) {
@@ -1244,9 +1259,9 @@ int sweep(baas_handle *h, baas_schedule *s, int uplo, const T *vals, const T *b,
 }
 
 template int sweep<double>(baas_handle *, baas_schedule *, int, const double *, const double *, double *, int,
-                           hipStream_t, const int *, void *, size_t);
+                           hipStream_t, const int *, void *, size_t, const int *);
 template int sweep<float>(baas_handle *, baas_schedule *, int, const float *, const float *, float *, int,
-                          hipStream_t, const int *, void *, size_t);
+                          hipStream_t, const int *, void *, size_t, const int *);
 
 // ---------------------------------------------------------------------------
 // Scratch of the allocation-free entry point (baas_sptrsv_f64_noalloc): the

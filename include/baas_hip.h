@@ -254,8 +254,11 @@ int baas_batch_destroy(baas_batch *b);
 int baas_batch_pcg(baas_batch *b, const double *B, double *X, double tol, int maxiter, int *iters);
 /* times[0..2] (host) = IC0 + setup, CG loop, autotuning seconds of the last
  * call; rpu[0..2] (host) = tuned rows per unit for systems of <= 4096,
- * <= 16384 and more rows, or {0, 0, workgroup size} when the level-set plan
- * (all systems' rows of one level at a time) won. */
+ * <= 16384 and more rows, {0, 0, workgroup size} when the level-set plan
+ * (all systems' rows of one level at a time) won, or the box size three
+ * times when boxes of each system's stencil grid on the multi-RHS lane
+ * kernel won (every system's pattern a 5- or 7-point grid in natural
+ * order; the boxes of all systems are claimed in one wavefront order). */
 int baas_batch_info(const baas_batch *b, double *times, int *rpu);
 
 /* Re-pack a schedule's execution plan (baas_plan_mode RANGE, SCHEDULE,
