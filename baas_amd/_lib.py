@@ -15,7 +15,7 @@ LIB_PATH = os.path.join(HERE, "libbaas_hip.so")
 OK, INVALID_ARG, ZERO_PIVOT, NONPOSITIVE_PIVOT, HIP_ERROR, TIMEOUT, NOT_CONVERGED, ALLOC, UNSUPPORTED, TOO_LARGE = range(10)
 NOALLOC_MAX_VALUES = 4194304
 KIND_TRSV, KIND_IC0, KIND_ILU0 = 0, 1, 2
-PLAN_AUTO, PLAN_RANGE, PLAN_SCHEDULE, PLAN_LEVELS, PLAN_TILES, PLAN_BOXES = 0, 1, 2, 3, 4, 5
+PLAN_AUTO, PLAN_RANGE, PLAN_SCHEDULE, PLAN_LEVELS, PLAN_TILES, PLAN_BOXES, PLAN_PIPE = 0, 1, 2, 3, 4, 5, 6
 EXPORTS = {
     "group_ptr": 0, "group_set": 1, "level_ptr": 2, "perm_to_orig": 3, "node_to_level": 4,
     "nthreads": 5, "merged_ptr": 6, "row_level_ptr": 7, "row_perm": 8, "row_level": 9,

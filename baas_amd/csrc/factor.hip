@@ -715,7 +715,7 @@ static int factor(baas_handle *h, baas_schedule *s, const double *a_vals, double
     a.ctx = Ctx{h->counters, h->counters + 1, h->counters + 2};
     a.status = h->status;
     a.spin_limit = h->spin_limit;
-    if (p.dataflow == 2 && p.block >= 128) {
+    if ((p.dataflow == 2 || p.dataflow == 5) && p.block >= 128) {
         const int rc = build_factor_lane(s);
         if (rc != BAAS_OK && rc != BAAS_UNSUPPORTED) return rc;
         if (rc == BAAS_OK && s->icl_lds <= lds_limit(h)) {

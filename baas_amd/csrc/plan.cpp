@@ -195,6 +195,7 @@ int make_plan(Plan &p, int n, const std::vector<int> &order, const std::vector<i
     p.xp_nrhs = 0;
     p.lane_lds_key = -1;
     p.mr_lds_key = -1;
+    p.pipe = 0;  // a PIPE plan's boxes are set by its builder after this
     BAAS_TRY(upload(p.scol, scol, st));
     BAAS_TRY(upload(p.ext_ptr, ext_ptr, st));
     BAAS_TRY(upload(p.ext_cols, ext_cols, st));
@@ -410,6 +411,99 @@ static int box_units(const baas_schedule *s, const HostCsr &L, int R, std::vecto
     return BAAS_OK;
 }
 
+// PIPE units: boxes of a stencil grid shaped for one wavefront each, swept
+// by the pipelined grid kernel (pipe.hip).  The grid is recognised as for
+// BOXES (offsets 1 < sy < sz, sy dividing sz, natural order), and must also
+// fill whole planes (n a multiple of sz; of sy on a 2D grid) with every
+// parent in the direction its offset names: an entry at offset 1 only in
+// rows with x > 0, at sy only with y > 0, at sz only with z > 0.  A 2D grid
+// (offsets 1 < sy) is taken as nx = sy, ny = 1, nz = n / sy, its rows
+// becoming the pipeline's planes.  Boxes are TX x TY rows across (8 x 8 on
+// a 3D grid, 64 x 1 on a 2D grid, narrower where the grid is) by
+// K = R / (TX TY) planes, numbered in wavefront order like BOXES units.
+// Per unit, box[4u..4u+3] = origin x, y, z and tx | ty << 8 | kz << 16 in
+// the forward direction's coordinates; the backward plan flips every
+// coordinate (x -> nx - 1 - x, ...) and takes the boxes in reverse order.
+struct PipeGeom {
+    int nx = 0, ny = 1, nz = 1, sy = -1, sz = 0, TX = 1, TY = 1, K = 1;
+};
+static int pipe_units(const baas_schedule *s, const HostCsr &L, int R, std::vector<int> &order, std::vector<int> &fu,
+                      PipeGeom &g, std::vector<int> &box) {
+    const int n = s->n;
+    if (R < 64 || !s->block_ptr.empty()) return R < 64 ? BAAS_INVALID_ARG : BAAS_UNSUPPORTED;
+    std::vector<long long> offs;
+    for (int r = 0; r < n; r++) {
+        // the diagonal last, at most three parents in ascending columns
+        const int cnt = L.ptr[r + 1] - L.ptr[r];
+        if (cnt < 1 || cnt > 4 || L.idx[L.ptr[r + 1] - 1] != r) return BAAS_UNSUPPORTED;
+        for (int j = L.ptr[r] + 1; j < L.ptr[r + 1]; j++)
+            if (L.idx[j] <= L.idx[j - 1]) return BAAS_UNSUPPORTED;
+        for (int j = L.ptr[r]; j < L.ptr[r + 1] - 1; j++) {
+            const long long d = (long long)r - L.idx[j];
+            if (std::find(offs.begin(), offs.end(), d) == offs.end()) {
+                offs.push_back(d);
+                if (offs.size() > 3) return BAAS_UNSUPPORTED;
+            }
+        }
+    }
+    std::sort(offs.begin(), offs.end());
+    if (offs.size() < 2 || offs[0] != 1) return BAAS_UNSUPPORTED;
+    const bool three = offs.size() == 3;
+    const long long s1 = offs[1], s2 = three ? offs[2] : offs[1];
+    if (s2 % s1 != 0 || n % s2 != 0 || s2 > (1LL << 30)) return BAAS_UNSUPPORTED;
+    g.nx = (int)s1;
+    g.ny = three ? (int)(s2 / s1) : 1;
+    g.nz = (int)(n / s2);
+    g.sy = three ? (int)s1 : -1;
+    g.sz = (int)s2;
+    // every parent in the direction its offset names
+    for (int r = 0; r < n; r++) {
+        const int x = r % g.nx, y = (r / g.nx) % g.ny, z = r / g.sz;
+        for (int j = L.ptr[r]; j < L.ptr[r + 1] - 1; j++) {
+            const int d = r - L.idx[j];
+            if ((d == 1 && x == 0) || (three && d == g.sy && y == 0) || (d == g.sz && z == 0))
+                return BAAS_UNSUPPORTED;
+        }
+    }
+    g.TX = three ? std::min(8, g.nx) : std::min(64, g.nx);
+    g.TY = three ? std::min(64 / g.TX, g.ny) : 1;
+    g.K = std::max(1, std::min({R / (g.TX * g.TY), g.nz, 255}));
+    // a box's parents in other boxes are staged a fixed number at a time
+    while (g.K > 1 && pipe_ext_max(g.TX, g.TY, g.K, three) > 1024) g.K--;
+    const long long gx = (g.nx + g.TX - 1) / g.TX, gy = (g.ny + g.TY - 1) / g.TY, gz = (g.nz + g.K - 1) / g.K;
+    const long long nbox = gx * gy * gz;
+    if (nbox > (1LL << 26)) return BAAS_INVALID_ARG;
+    std::vector<long long> ids((size_t)nbox);
+    for (long long t = 0; t < nbox; t++) ids[t] = t;
+    auto dsum = [&](long long t) { return t % gx + (t / gx) % gy + t / (gx * gy); };
+    std::stable_sort(ids.begin(), ids.end(), [&](long long a, long long b) { return dsum(a) < dsum(b); });
+    std::vector<long long> rank((size_t)nbox);
+    for (long long q = 0; q < nbox; q++) rank[ids[q]] = q;
+    std::vector<int> start((size_t)nbox + 1, 0);
+    auto box_of = [&](int r) {
+        const int x = r % g.nx, y = (r / g.nx) % g.ny, z = r / g.sz;
+        return rank[((z / g.K) * gy + y / g.TY) * gx + x / g.TX];
+    };
+    for (int r = 0; r < n; r++) start[box_of(r) + 1]++;
+    for (long long t = 0; t < nbox; t++) start[t + 1] += start[t];
+    std::vector<int> next(start.begin(), start.end() - 1);
+    for (int r = 0; r < n; r++) order[next[box_of(r)]++] = r;
+    box.assign(4 * (size_t)nbox, 0);
+    for (long long q = 0; q < nbox; q++) {
+        const long long t = ids[q];
+        const int bx = (int)(t % gx), by = (int)((t / gx) % gy), bz = (int)(t / (gx * gy));
+        const int x0 = bx * g.TX, y0 = by * g.TY, z0 = bz * g.K;
+        const int tx = std::min(g.TX, g.nx - x0), ty = std::min(g.TY, g.ny - y0), kz = std::min(g.K, g.nz - z0);
+        box[4 * q + 0] = x0;
+        box[4 * q + 1] = y0;
+        box[4 * q + 2] = z0;
+        box[4 * q + 3] = tx | (ty << 8) | (kz << 16);
+        fu.push_back(start[q]);
+    }
+    fu.push_back(n);
+    return BAAS_OK;
+}
+
 // Grid geometry of one stencil block from its lower pattern: at most three
 // distinct offsets 1 < sy < sz, sy dividing sz (rows r0 .. r1 of L); the box
 // edge for about R rows per box.
@@ -529,6 +623,8 @@ int build_plans_mode(baas_schedule *s, int mode, int rows_per_unit) {
     }
     int block = s->opts.block_threads > 0 ? s->opts.block_threads : 256;
     std::vector<int> fwd_order(n), bwd_order(n), fu, bu;
+    PipeGeom pg;
+    std::vector<int> pbox;
     if (mode == BAAS_PLAN_RANGE) {
         int R = rows_per_unit > 0 ? rows_per_unit : 4096;
         for (int r = 0; r < n; r++) fwd_order[r] = r, bwd_order[r] = n - 1 - r;
@@ -542,6 +638,8 @@ int build_plans_mode(baas_schedule *s, int mode, int rows_per_unit) {
     }
     if (mode == BAAS_PLAN_TILES) {
         BAAS_TRY(tile_units(s, rows_per_unit > 0 ? rows_per_unit : 16384, s->tile_band, fwd_order, fu));
+    } else if (mode == BAAS_PLAN_PIPE) {
+        BAAS_TRY(pipe_units(s, L, rows_per_unit > 0 ? rows_per_unit : 1024, fwd_order, fu, pg, pbox));
     } else if (mode == BAAS_PLAN_BOXES) {
         if (!s->block_ptr.empty())
             BAAS_TRY(block_box_units(s, L, rows_per_unit > 0 ? rows_per_unit : 512, fwd_order, fu));
@@ -556,6 +654,27 @@ int build_plans_mode(baas_schedule *s, int mode, int rows_per_unit) {
     for (int u = 0; u <= nu; u++) bu[u] = n - fu[nu - u];
     BAAS_TRY(make_plan(s->fwd, n, fwd_order, fu, lp, li, up, ui, 2, mode, block, st, L, false));
     BAAS_TRY(make_plan(s->bwd, n, bwd_order, bu, up, ui, lp, li, 2, mode, block, st, U, true));
+    if (mode == BAAS_PLAN_PIPE) {
+        // the backward sweep runs the same boxes in reverse order, each in
+        // flipped coordinates (parents of U at +1, +sy, +sz become -1, ...)
+        std::vector<int> bbox(pbox.size());
+        for (int u = 0; u < nu; u++) {
+            const int *f = &pbox[4 * (size_t)(nu - 1 - u)];
+            const int tx = f[3] & 255, ty = (f[3] >> 8) & 255, kz = (f[3] >> 16) & 255;
+            bbox[4 * (size_t)u + 0] = pg.nx - (f[0] + tx);
+            bbox[4 * (size_t)u + 1] = pg.ny - (f[1] + ty);
+            bbox[4 * (size_t)u + 2] = pg.nz - (f[2] + kz);
+            bbox[4 * (size_t)u + 3] = f[3];
+        }
+        const int g8[8] = {pg.nx, pg.ny, pg.nz, pg.sy, pg.sz, pg.TX, pg.TY, pg.K};
+        BAAS_TRY(upload(s->fwd.pipe_box, pbox, st));
+        BAAS_TRY(upload(s->bwd.pipe_box, bbox, st));
+        if (hipStreamSynchronize(st) != hipSuccess) return BAAS_HIP_ERROR;
+        for (Plan *p : {&s->fwd, &s->bwd}) {
+            std::copy(g8, g8 + 8, p->pg);
+            p->pipe = 1;
+        }
+    }
     return BAAS_OK;
 }
 
@@ -854,7 +973,7 @@ int build_factor_lane(baas_schedule *s) {
 }  // namespace baas
 
 extern "C" int baas_replan(baas_schedule *s, int mode, int rows_per_unit, int block_threads) {
-    if (!s || mode < 1 || mode > 5 || rows_per_unit < 0 || block_threads < 0 || block_threads > 1024 ||
+    if (!s || mode < 1 || mode > 6 || rows_per_unit < 0 || block_threads < 0 || block_threads > 1024 ||
         (block_threads % 64) != 0)
         return BAAS_INVALID_ARG;
     const baas_opts old = s->opts;

@@ -908,6 +908,10 @@ static int launch_sweep_k(baas_handle *h, const Plan &p, const SweepArgs &a, con
 }
 
 size_t lds_limit(baas_handle *h);
+size_t pipe_lds(const Plan &p, int ts);
+template <class T>
+int launch_pipe(baas_handle *h, baas_schedule *s, const Plan &p, int uplo, const T *vals, const T *b, T *x, T *xp,
+                int nrhs, hipStream_t stream, const int *skip);
 
 template <class T, bool UP>
 static int launch_df(baas_handle *h, const Plan &p, const DfArgs &a, const T *vals, const T *b, T *x, T *xp,
@@ -1175,8 +1179,11 @@ int sweep(baas_handle *h, baas_schedule *s, int uplo, const T *vals, const T *b,
                     p.mr_lds((int)sizeof(T), nrhs, false) <= lds_limit(h);
     const bool mr_sb = mr && p.dataflow == 3 && p.mr_lds((int)sizeof(T), nrhs, true) <= lds_limit(h);
     const bool lane = mr || (p.dataflow == 2 && p.block >= 128 && p.lane_lds((int)sizeof(T), nrhs) <= lds_limit(h));
-    if (lane || (nrhs == 1 && p.dataflow == 1 && p.block >= 192 && p.block <= 512 &&
-                 p.df_lds[sizeof(T) == 8] <= lds_limit(h))) {
+    // the pipelined grid sweep (PIPE plans): one value per row published
+    const bool pipe = !lane && p.pipe && p.dataflow == 5 && pipe_lds(p, (int)sizeof(T)) <= lds_limit(h);
+    if (lane || pipe ||
+        (nrhs == 1 && p.dataflow == 1 && p.block >= 192 && p.block <= 512 &&
+         p.df_lds[sizeof(T) == 8] <= lds_limit(h))) {
         // publication halves (one value per row and right-hand side), armed
         // with this value type's sentinel; any change of layout re-arms them
         const int xr = lane ? nrhs : 1;
@@ -1219,6 +1226,7 @@ int sweep(baas_handle *h, baas_schedule *s, int uplo, const T *vals, const T *b,
         d.unit_skip = mr ? unit_skip : nullptr;
         d.status = h->status;
         d.spin_limit = h->spin_limit;
+        if (pipe) return launch_pipe<T>(h, s, p, uplo, vals, b, x, xp, nrhs, stream, skip);
         if (mr) {
             const size_t lds = p.mr_lds((int)sizeof(T), nrhs, mr_sb);
             return uplo ? launch_mr<T, true>(h, p, d, vals, b, x, xp, lds, mr_sb, stream)
@@ -1371,7 +1379,7 @@ extern "C" int baas_csr_outer_grad_f64(baas_handle *h, int m, const int *rp, con
 
 extern "C" int baas_plan_dataflow(baas_schedule *s, int enable) {
     if (!s) return BAAS_INVALID_ARG;
-    if (enable < 0 || enable > 4) return BAAS_INVALID_ARG;
+    if (enable < 0 || enable > 5) return BAAS_INVALID_ARG;
     s->fwd.dataflow = s->bwd.dataflow = enable;
     s->plan_gen++;
     return BAAS_OK;

@@ -19,6 +19,7 @@ namespace baas {
 int build_plans_mode(baas_schedule *s, int mode, int rows_per_unit);
 int build_factor_lane(baas_schedule *s);
 size_t lds_limit(baas_handle *h);
+size_t pipe_lds(const Plan &p, int ts);
 }  // namespace baas
 
 using namespace baas;
@@ -94,6 +95,10 @@ int autotune(baas_handle *h, baas_schedule *s, const T *vals, int uplo, int nrhs
         }
         cands.push_back({BAAS_PLAN_BOXES, R, 256, 0, 0});
     }
+    // the wave-pipelined grid sweep: boxes of 8 x 8 (64 x 1 on a 2D grid)
+    // rows across by R / 64 planes
+    if (nrhs == 1)
+        for (int R : {512, 1024, 2048}) cands.push_back({BAAS_PLAN_PIPE, R, 128, 5, 0});
     if (nrhs > 1) {
         for (int R : {64, 125, 216}) cands.push_back({BAAS_PLAN_BOXES, R, 1024, 2, 0});
         // the multi-RHS lane kernel (parents of other units read directly,
@@ -136,7 +141,8 @@ int autotune(baas_handle *h, baas_schedule *s, const T *vals, int uplo, int nrhs
         Plan &p = uplo ? s->bwd : s->fwd;
         if (c.df == 1 && p.df_lds[sizeof(T) == 8] > lds_limit(h)) continue;
         if (c.df == 2 && p.lane_lds((int)sizeof(T), nrhs) > lds_limit(h)) continue;
-        if (c.df >= 3 && p.mr_lds((int)sizeof(T), nrhs, false) > lds_limit(h)) continue;
+        if ((c.df == 3 || c.df == 4) && p.mr_lds((int)sizeof(T), nrhs, false) > lds_limit(h)) continue;
+        if (c.df == 5 && (!p.pipe || pipe_lds(p, (int)sizeof(T)) > lds_limit(h))) continue;
         s->fwd.dataflow = s->bwd.dataflow = c.df;
         float t_best = 1e30f;
         bool ok = true;

@@ -112,6 +112,13 @@ def laplace2d_lower(side: int = 128) -> Csr:
     return laplace2d(side).lower()
 
 
+def laplace_grid(shape) -> Csr:
+    """Laplacian on a grid of any shape (x fastest, natural ordering): 2 x
+    dimensions on the diagonal, -1 to each grid neighbour."""
+    offs = [(a, s, -1.0) for a in range(len(shape)) for s in (-1, 1)]
+    return _grid_operator(tuple(shape), 2.0 * len(shape), offs)
+
+
 def laplace3d(side: int = 64) -> Csr:
     """3D 7-point Laplacian, diag 6, off -1, natural ordering (x fastest)."""
     offs = [(a, s, -1.0) for a in (0, 1, 2) for s in (-1, 1)]

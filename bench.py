@@ -518,7 +518,7 @@ def main():
     ap.add_argument("--plan", default="auto",
                     help="auto (autotune) or MODE:ROWS_PER_UNIT:BLOCK:DATAFLOW[:BAND], MODE 1 = RANGE, "
                          "2 = SCHEDULE, 3 = LEVELS, 4 = TILES (ROWS_PER_UNIT = row range, BAND = levels), "
-                         "5 = BOXES; with --only it applies to that workload")
+                         "5 = BOXES, 6 = PIPE (DATAFLOW 5); with --only it applies to that workload")
     ap.add_argument("--only", choices=sorted(EXTRAS), default=None,
                     help="run one secondary measurement alone (no headline solve) and print its JSON")
     ap.add_argument("--calib-copy-mib", type=int, default=0,
@@ -613,8 +613,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(kern_gbs, 3), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(kern_gbs / HBM_PEAK_GBS, 6), "traffic": traffic,
                      "traffic_plan": traffic_plan,
-                     "kernel": {0: "k_sweep", 1: "k_sweep_df (dataflow)", 2: "k_sweep_lane (dataflow, lanes)"}[
-                         int(plan["dataflow"])] if plan["mode"] != 3 else "k_sweep_levels",
+                     "kernel": {0: "k_sweep", 1: "k_sweep_df (dataflow)", 2: "k_sweep_lane (dataflow, lanes)",
+                                5: "k_sweep_pipe (wave-pipelined grid boxes)"}.get(
+                         int(plan["dataflow"]), "k_sweep_mr") if plan["mode"] != 3 else "k_sweep_levels",
                      "algorithmic_bytes_per_launch": by, "mean_launch_ms": round(mean_ms, 5),
                      "median_launch_ms": round(med_ms, 5)},
     }

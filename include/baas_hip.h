@@ -85,6 +85,14 @@ enum baas_export {
  * 7-point stencil grid in natural order, recognised from the pattern alone
  * -- units are boxes of the grid of about rows_per_unit rows (default 512);
  * other matrices get BAAS_UNSUPPORTED.
+ * PIPE: boxes of such a grid shaped for one wavefront each -- 8 x 8 rows
+ * across (x, y) by rows_per_unit / 64 planes along z on a 3D grid, 64 rows
+ * along x by rows_per_unit / 64 rows along y on a 2D grid -- which the
+ * wavefront sweeps as a diagonal pipeline (every lane solves one row per
+ * step and hands its value to its neighbours' lanes); also requires every
+ * parent to lie in the grid direction its offset names (no wrap-around).
+ * Single right-hand-side sweeps with dataflow 5 run that kernel; other
+ * sweeps and factorisations run these boxes on the other kernels.
  * For LEVELS plans rows_per_unit caps the workgroups (0: as many as the
  * widest level needs); with one workgroup the levels are separated by the
  * workgroup's own barrier instead of a device-wide one. */
@@ -94,7 +102,8 @@ enum baas_plan_mode {
     BAAS_PLAN_SCHEDULE = 2,
     BAAS_PLAN_LEVELS = 3,
     BAAS_PLAN_TILES = 4,
-    BAAS_PLAN_BOXES = 5
+    BAAS_PLAN_BOXES = 5,
+    BAAS_PLAN_PIPE = 6
 };
 
 typedef struct baas_opts {
@@ -285,7 +294,10 @@ int baas_schedule_plan(const baas_schedule *s, int *plan);
  * publication array by the lane that needs them (no exchange wavefront),
  * x written as each item is solved, so LDS holds only the unit's own
  * values; 3 also stages the unit's b in LDS where it fits, 4 reads b from
- * memory.  Sweeps only (factorisations keep the unit kernels). */
+ * memory.  Sweeps only (factorisations keep the unit kernels).  Enable 5
+ * (PIPE plans) = the wave-pipelined grid sweep for one right-hand side (a
+ * launch per column for several); factorisations then take the lane
+ * kernel's path. */
 int baas_plan_dataflow(baas_schedule *s, int enable);
 /* Autotune the execution plan for sweeps with `uplo` and `nrhs` on these
  * values: times RANGE, SCHEDULE, LEVELS and TILES packings over their unit

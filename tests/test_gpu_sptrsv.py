@@ -207,6 +207,97 @@ def test_box_plans_bit_exact(handle, name):
         np.testing.assert_array_equal(down(y), y_ref, err_msg=f"bwd {name} {rpu} {blk} {df}")
 
 
+PIPE_GRIDS = [("C1_lap2d_128", None), ("C2_lap3d_64", None), ("C5_diff3d_48", None), ("grid", (13, 11, 9)),
+              ("grid", (70, 5)), ("grid", (5, 17, 6)), ("grid", (20, 3, 7)), ("grid", (9, 9, 9))]
+
+
+def _pipe_matrix(name, shape):
+    if shape is not None:
+        return g.laplace_grid(shape).lower(), f"grid{'x'.join(map(str, shape))}"
+    return workload(name)[0], name
+
+
+@pytest.mark.parametrize("name,shape", PIPE_GRIDS)
+def test_pipe_plans_bit_exact(handle, name, shape):
+    """PIPE plans (boxes swept by one wavefront as a diagonal pipeline) on
+    stencil grids, partial boxes and narrow grids included: forward and
+    backward bit for bit, every box depth."""
+    from baas_amd.api import Schedule
+    L, tag = _pipe_matrix(name, shape)
+    s = Schedule(handle, L.rowptr, L.colidx, kind=0)
+    b = g.rhs_uniform(L.n, 7)
+    x_ref = O.sptrsv_lower(L, b)
+    U = g.Csr.from_scipy(L.to_scipy().T)
+    y_ref = O.sptrsv_upper(U, b)
+    lv, bd = up(L.values), up(b)
+    uv = s.transpose_values(lv)
+    for rpu in (64, 256, 512, 1024, 2048):
+        announce("pipe", tag, rpu)
+        s.replan(6, rpu, 128, dataflow=5)
+        for _ in range(2):  # the second sweep runs on the other publication half
+            x = s.solve(lv, bd)
+            y = s.solve(uv, bd, uplo=1)
+            handle.check()
+            np.testing.assert_array_equal(down(x), x_ref, err_msg=f"fwd {tag} {rpu}")
+            np.testing.assert_array_equal(down(y), y_ref, err_msg=f"bwd {tag} {rpu}")
+
+
+@pytest.mark.parametrize("name,shape", [("C2_lap3d_64", None), ("grid", (70, 5))])
+def test_pipe_plans_f32_and_multi_rhs(handle, name, shape):
+    """FP32 sweeps on PIPE plans against the oracle's FP32 solves, and three
+    right-hand sides (one launch per column) against the FP64 oracle."""
+    import torch
+    from baas_amd.api import Schedule
+    L, tag = _pipe_matrix(name, shape)
+    s = Schedule(handle, L.rowptr, L.colidx, kind=0)
+    s.replan(6, 1024, 128, dataflow=5)
+    U = g.Csr.from_scipy(L.to_scipy().T)
+    b = g.rhs_uniform(L.n, 8)
+    announce("pipe f32", tag)
+    lv32 = up(L.values, torch.float32)
+    uv32 = up(U.values, torch.float32)
+    x = s.solve(lv32, up(b, torch.float32))
+    y = s.solve(uv32, up(b, torch.float32), uplo=1)
+    handle.check()
+    np.testing.assert_array_equal(down(x), O.sptrsv_lower_f32(L, b))
+    np.testing.assert_array_equal(down(y), O.sptrsv_upper_f32(U, b))
+    announce("pipe 3 rhs", tag)
+    B = g.rhs_uniform(L.n, 9, 3)
+    lv = up(L.values)
+    X = down(s.solve(lv, up(B)))
+    Y = down(s.solve(s.transpose_values(lv), up(B), uplo=1))
+    handle.check()
+    for c in range(3):
+        np.testing.assert_array_equal(X[:, c], O.sptrsv_lower(L, np.ascontiguousarray(B[:, c])))
+        np.testing.assert_array_equal(Y[:, c], O.sptrsv_upper(U, np.ascontiguousarray(B[:, c])))
+
+
+@pytest.mark.parametrize("case", ["C2_lap3d_64_perm1", "C3_knn", "wrap"])
+def test_pipe_plans_reject(handle, case):
+    """PIPE plans need a stencil grid in natural order whose parents lie in
+    the directions their offsets name: an unstructured matrix, and a grid
+    with one entry wrapping round from x = 0 to the previous line, get
+    UNSUPPORTED and keep their plan."""
+    from baas_amd import _lib
+    from baas_amd.api import Schedule
+    announce("pipe reject", case)
+    if case == "wrap":
+        import scipy.sparse as sp
+        M = g.laplace_grid((16, 16)).lower().to_scipy().tolil()
+        M[32, 31] = -0.5  # row 32 is (x = 0, y = 2); column 31 is (15, 1)
+        L = g.Csr.from_scipy(M.tocsr())
+    else:
+        L = workload(case)[0]
+    s = Schedule(handle, L.rowptr, L.colidx, kind=0)
+    s.replan(1, 1024, 256)
+    with pytest.raises(_lib.BaasError) as e:
+        s.replan(6, 1024, 128)
+    assert e.value.code == _lib.UNSUPPORTED
+    assert s.plan()["mode"] == 1
+    b = g.rhs_uniform(L.n, 0)
+    np.testing.assert_array_equal(down(s.solve(up(L.values), up(b))), O.sptrsv_lower(L, b))
+
+
 @pytest.mark.parametrize("name", ["C2_lap3d_64_perm1", "bcsstk18", "C3_knn"])
 def test_box_plans_reject_unstructured(handle, name):
     """Matrices that are not stencil grids in natural order get UNSUPPORTED,
