@@ -156,36 +156,11 @@ __host__ __device__ inline MrLayout mr_layout(size_t nk, size_t E, size_t ts, si
     l.total = l.es + lane_align8(E * 4);
     return l;
 }
-// Pipelined grid sweep (PIPE plans): LDS of one box of at most TX x TY x K
-// rows, 16-byte aligned sections: per row the four coefficients (z, y and x
-// parents, diagonal), b, the solution and the mask of parents present, each
-// in an array of its own indexed by pipe_slot(q) (one padding slot per 32
-// rows, so the rows a wavefront reads in one step fall in different LDS
-// banks), with a spare slot for idle lanes; per external parent (the plane
-// below, then each plane's x and y faces) its value slot and its row.
-struct PipeLayout {
-    size_t c0, c1, c2, c3, bv, xv, msk, ev, er, total;
-};
-__host__ __device__ inline size_t pipe_align16(size_t v) { return (v + 15) & ~size_t(15); }
-__host__ __device__ inline int pipe_slot(int q) { return q + (q >> 5); }
+// Pipelined grid sweep (PIPE plans): parents in other boxes of a box of
+// TX x TY x K rows (the plane below, per plane an x face of TY rows and,
+// on a 3D grid, a y face of TX rows).
 __host__ __device__ inline size_t pipe_ext_max(int TX, int TY, int K, bool has_y) {
     return (size_t)TX * TY + (size_t)K * (TY + (has_y ? TX : 0));
-}
-__host__ __device__ inline PipeLayout pipe_layout(int TX, int TY, int K, size_t ts, bool has_y) {
-    const size_t rm = (size_t)TX * TY * K, ne = pipe_ext_max(TX, TY, K, has_y);
-    const size_t ns = rm + rm / 32 + 2;  // slots pipe_slot(0 .. rm), the last one spare
-    PipeLayout l;
-    l.c0 = 0;
-    l.c1 = l.c0 + pipe_align16(ns * ts);
-    l.c2 = l.c1 + pipe_align16(ns * ts);
-    l.c3 = l.c2 + pipe_align16(ns * ts);
-    l.bv = l.c3 + pipe_align16(ns * ts);
-    l.xv = l.bv + pipe_align16(ns * ts);
-    l.msk = l.xv + pipe_align16(ns * ts);
-    l.ev = l.msk + pipe_align16(ns * 4);
-    l.er = l.ev + pipe_align16(ne * ts);
-    l.total = l.er + pipe_align16(ne * 4);
-    return l;
 }
 __host__ __device__ inline LaneLayout lane_layout(size_t nk, size_t ne, size_t E, size_t ts, size_t nrhs = 1) {
     LaneLayout l;

@@ -414,9 +414,10 @@ static int box_units(const baas_schedule *s, const HostCsr &L, int R, std::vecto
 // PIPE units: boxes of a stencil grid shaped for one wavefront each, swept
 // by the pipelined grid kernel (pipe.hip).  The grid is recognised as for
 // BOXES (offsets 1 < sy < sz, sy dividing sz, natural order), and must also
-// fill whole planes (n a multiple of sz; of sy on a 2D grid) with every
-// parent in the direction its offset names: an entry at offset 1 only in
-// rows with x > 0, at sy only with y > 0, at sz only with z > 0.  A 2D grid
+// fill whole planes (n a multiple of sz; of sy on a 2D grid) with a full
+// stencil: a row has an entry at offset 1 exactly when x > 0, at sy
+// exactly when y > 0, at sz exactly when z > 0.  Boxes hold at most 2048
+// rows.  A 2D grid
 // (offsets 1 < sy) is taken as nx = sy, ny = 1, nz = n / sy, its rows
 // becoming the pipeline's planes.  Boxes are TX x TY rows across (8 x 8 on
 // a 3D grid, 64 x 1 on a 2D grid, narrower where the grid is) by
@@ -456,9 +457,12 @@ static int pipe_units(const baas_schedule *s, const HostCsr &L, int R, std::vect
     g.nz = (int)(n / s2);
     g.sy = three ? (int)s1 : -1;
     g.sz = (int)s2;
-    // every parent in the direction its offset names
+    // a full stencil: every row has exactly the parents its position
+    // implies, each in the direction its offset names
     for (int r = 0; r < n; r++) {
         const int x = r % g.nx, y = (r / g.nx) % g.ny, z = r / g.sz;
+        const int expect = (x > 0) + (three && y > 0) + (z > 0);
+        if (L.ptr[r + 1] - L.ptr[r] - 1 != expect) return BAAS_UNSUPPORTED;
         for (int j = L.ptr[r]; j < L.ptr[r + 1] - 1; j++) {
             const int d = r - L.idx[j];
             if ((d == 1 && x == 0) || (three && d == g.sy && y == 0) || (d == g.sz && z == 0))
@@ -467,7 +471,7 @@ static int pipe_units(const baas_schedule *s, const HostCsr &L, int R, std::vect
     }
     g.TX = three ? std::min(8, g.nx) : std::min(64, g.nx);
     g.TY = three ? std::min(64 / g.TX, g.ny) : 1;
-    g.K = std::max(1, std::min({R / (g.TX * g.TY), g.nz, 255}));
+    g.K = std::max(1, std::min({R / (g.TX * g.TY), g.nz, 255, 2048 / (g.TX * g.TY)}));
     // a box's parents in other boxes are staged a fixed number at a time
     while (g.K > 1 && pipe_ext_max(g.TX, g.TY, g.K, three) > 1024) g.K--;
     const long long gx = (g.nx + g.TX - 1) / g.TX, gy = (g.ny + g.TY - 1) / g.TY, gz = (g.nz + g.K - 1) / g.K;

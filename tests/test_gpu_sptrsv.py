@@ -272,12 +272,12 @@ def test_pipe_plans_f32_and_multi_rhs(handle, name, shape):
         np.testing.assert_array_equal(Y[:, c], O.sptrsv_upper(U, np.ascontiguousarray(B[:, c])))
 
 
-@pytest.mark.parametrize("case", ["C2_lap3d_64_perm1", "C3_knn", "wrap"])
+@pytest.mark.parametrize("case", ["C2_lap3d_64_perm1", "C3_knn", "wrap", "hole"])
 def test_pipe_plans_reject(handle, case):
-    """PIPE plans need a stencil grid in natural order whose parents lie in
-    the directions their offsets name: an unstructured matrix, and a grid
-    with one entry wrapping round from x = 0 to the previous line, get
-    UNSUPPORTED and keep their plan."""
+    """PIPE plans need a full stencil grid in natural order: an unstructured
+    matrix, a grid with one entry wrapping round from x = 0 to the previous
+    line, and a grid missing one interior entry get UNSUPPORTED and keep
+    their plan."""
     from baas_amd import _lib
     from baas_amd.api import Schedule
     announce("pipe reject", case)
@@ -286,6 +286,12 @@ def test_pipe_plans_reject(handle, case):
         M = g.laplace_grid((16, 16)).lower().to_scipy().tolil()
         M[32, 31] = -0.5  # row 32 is (x = 0, y = 2); column 31 is (15, 1)
         L = g.Csr.from_scipy(M.tocsr())
+    elif case == "hole":
+        M = g.laplace_grid((16, 16)).lower().to_scipy().tolil()
+        M[40, 39] = 0.0  # row 40 is (8, 2): its x parent removed
+        M = M.tocsr()
+        M.eliminate_zeros()
+        L = g.Csr.from_scipy(M)
     else:
         L = workload(case)[0]
     s = Schedule(handle, L.rowptr, L.colidx, kind=0)
