@@ -9,6 +9,8 @@
 // barriers).  The fastest plan stays on the schedule.  Values need not be
 // the final ones: any non-zero diagonal gives the same timing.
 #include <cstdio>
+#include <cstdlib>
+#include <cstring>
 #include <vector>
 
 #include "blas.h"
@@ -32,6 +34,21 @@ struct Cand {
 
 // a candidate replaces the fastest so far only below this fraction of its time
 constexpr float TUNE_MARGIN = 0.97f;
+
+// BAAS_TUNE_LOG set (to anything but 0) prints every timed candidate to stderr
+bool tune_log() {
+    static const bool on = [] {
+        const char *e = std::getenv("BAAS_TUNE_LOG");
+        return e && *e && std::strcmp(e, "0") != 0;
+    }();
+    return on;
+}
+
+void log_cand(const char *what, const Cand &c, bool ok, float ms) {
+    if (tune_log())
+        std::fprintf(stderr, "[baas tune %s] mode %d rows %d block %d dataflow %d band %d: %s%.1f us\n", what, c.mode,
+                     c.rpu, c.block, c.df, c.band, ok ? "" : "failed ", ok ? ms * 1000.0f : 0.0f);
+}
 
 // TILES candidates: row ranges by level bands
 constexpr int TILE_RANGES[] = {8192, 16384};
@@ -155,6 +172,7 @@ int autotune(baas_handle *h, baas_schedule *s, const T *vals, int uplo, int nrhs
             (void)hipEventElapsedTime(&ms, e0, e1);
             if (rep > 0 && ms < t_best) t_best = ms;
         }
+        log_cand("sweep", c, ok, t_best);
         // a later candidate must be clearly faster to replace the incumbent,
         // so near-ties resolve the same way from run to run
         if (ok && t_best < best * TUNE_MARGIN) best = t_best, win = c;
@@ -240,6 +258,7 @@ static int autotune_factor(baas_handle *h, baas_schedule *s, const double *a_val
             (void)hipEventElapsedTime(&ms, e0, e1);
             if (rep > 0 && ms < t_best) t_best = ms;
         }
+        log_cand("factor", c, ok, t_best);
         // a later candidate must be clearly faster to replace the incumbent,
         // so near-ties resolve the same way from run to run
         if (ok && t_best < best * TUNE_MARGIN) best = t_best, win = c;

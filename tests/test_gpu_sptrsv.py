@@ -540,7 +540,7 @@ def test_autotune_keeps_bits(handle, name):
         vals = lv if uplo == 0 else s.transpose_values(lv)
         chosen = s.autotune(vals, uplo=uplo)
         print("[autotune]", name, uplo, chosen)
-        assert chosen["mode"] in (1, 2, 3, 4, 5) and chosen["us"] > 0
+        assert chosen["mode"] in (1, 2, 3, 4, 5, 6) and chosen["us"] > 0
         b = g.rhs_uniform(L.n, 0)
         got = down(s.solve(vals, up(b), uplo=uplo))
         handle.check()
