@@ -18,7 +18,7 @@ from gpu_util import announce, down, up
 pytestmark = pytest.mark.gpu
 
 # (plan mode, rows per unit, threads per workgroup, dataflow kernel)
-PLANS = [(5, 900, 1024, 2), (2, 64, 512, 2), (1, 4096, 256, 1), (1, 1024, 256, 0), (3, 0, 256, 0)]
+PLANS = [(5, 900, 1024, 2), (6, 1024, 128, 5), (2, 64, 512, 2), (1, 4096, 256, 1), (1, 1024, 256, 0), (3, 0, 256, 0)]
 
 
 @pytest.fixture(scope="module")
@@ -39,7 +39,7 @@ def _case(h, L, plan, nrhs, seed):
     s = Schedule(h, L.rowptr, L.colidx, kind=0)
     mode, rpu, blk, df = plan
     if mode == 5 and L.n != 64 ** 3:
-        mode, rpu = 1, 512  # boxes need the 3D grid; ranges elsewhere
+        mode, rpu = 1, 512  # boxes need the 3D grid; ranges elsewhere (PIPE plans take 2D grids too)
     s.replan(mode, rpu, blk, dataflow=df)
     lv = up(L.values)
     uv = s.transpose_values(lv)
