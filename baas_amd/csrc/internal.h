@@ -110,15 +110,18 @@ struct Plan {
     size_t mr_lds_val = 0;
     DBuf<unsigned char> xp_raw;  // 2 x n x nrhs published values (dataflow kernels)
     int xp_type = 0;             // sizeof the value type the halves are armed for
-    int xp_nrhs = 0;             // ... and their right-hand sides per row
+    int xp_nrhs = 0;             // ... and their right-hand sides per row (-1: a PIPE plan's faces)
     DBuf<int> pcount;            // [1] sweeps run on this plan
     // PIPE plans: the grid (nx, ny, nz, sy (-1 on a 2D grid), sz) and box
     // shape (TX, TY, K) in pg; per unit the box's origin x, y, z and
     // extents tx | ty << 8 | kz << 16 in this sweep's coordinates (the
-    // backward plan's flipped); pipe = 1 once they are built
+    // backward plan's flipped), then the offsets of its own faces and of
+    // the three faces it reads in a publication half (-1: none); each
+    // half holds pipe_half values; pipe = 1 once they are built
     int pipe = 0;
     int pg[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    DBuf<int> pipe_box;  // [4 nunits]
+    DBuf<int> pipe_box;  // [8 nunits]
+    long long pipe_half = 0;
     std::vector<int> h_unit_ptr;  // host copy: unit boundaries in rows
     std::vector<int> h_rows;      // host copy of rows (without flag bit)
 };
@@ -156,11 +159,19 @@ __host__ __device__ inline MrLayout mr_layout(size_t nk, size_t E, size_t ts, si
     l.total = l.es + lane_align8(E * 4);
     return l;
 }
-// Pipelined grid sweep (PIPE plans): parents in other boxes of a box of
-// TX x TY x K rows (the plane below, per plane an x face of TY rows and,
-// on a 3D grid, a y face of TX rows).
-__host__ __device__ inline size_t pipe_ext_max(int TX, int TY, int K, bool has_y) {
-    return (size_t)TX * TY + (size_t)K * (TY + (has_y ? TX : 0));
+// PIPE plans: what a box of tx x ty rows by kz planes publishes for the
+// boxes after it, in the order they consume it -- x: the x face by
+// diagonals d = k + j (ty values each, (kz + ty - 1) ty in all), y: the y
+// face by d = k + i ((kz + tx - 1) tx; none on a 2D grid), z: its top
+// plane (tx ty).  A box stages at most PIPE_FACE_MAX values of each face.
+constexpr int PIPE_FACE_MAX = 1024;
+constexpr int PIPE_DUMP = 8;  // values after a box's faces that its idle publishing lanes write
+__host__ __device__ inline int3 pipe_face_sizes(int tx, int ty, int kz, bool has_y) {
+    return make_int3((kz + ty - 1) * ty, has_y ? (kz + tx - 1) * tx : 0, tx * ty);
+}
+__host__ __device__ inline bool pipe_faces_fit(int TX, int TY, int K, bool has_y) {
+    const int3 f = pipe_face_sizes(TX, TY, K, has_y);
+    return f.x <= PIPE_FACE_MAX && f.y <= PIPE_FACE_MAX && f.z <= 64;
 }
 __host__ __device__ inline LaneLayout lane_layout(size_t nk, size_t ne, size_t E, size_t ts, size_t nrhs = 1) {
     LaneLayout l;

@@ -18,10 +18,12 @@
 // move, three fma and the division -- instead of an LDS hand-off and a
 // poll.  Parents in other boxes (the plane below, the x and y faces) are
 // staged into LDS by the workgroup's other wavefront as their producers
-// publish them (value as flag, into the double-buffered publication array
-// of the dataflow kernels); a box publishes its own x face, y face and top
-// plane for the boxes after it.  Boxes are claimed from a ticket in
-// wavefront order, so every box's producers are claimed before it.
+// publish them.  A box publishes its x face, y face and top plane for the
+// boxes after it (value as flag, into a double-buffered publication array)
+// in the order its consumer needs them: a face by diagonals, so that one
+// step of the producer writes, and one step of the consumer needs, one run
+// of consecutive values.  Boxes are claimed from a ticket in wavefront
+// order, so every box's producers are claimed before it.
 //
 // Plans exist only for full stencils (every row has exactly the parents its
 // position implies), so the step needs no per-row masks: a parent that
@@ -44,7 +46,11 @@ struct PipeArgs {
     int n, nunits, nrhs, rhs;
     int nx, ny, nz, sy, sz;  // grid; sy < 0: a 2D grid (no y direction)
     int TX, TY, K;           // box shape
-    const int4 *box;         // per unit: x0, y0, z0, tx | ty << 8 | kz << 16
+    long long half;          // values per publication half
+    // per unit two records: x0, y0, z0, tx | ty << 8 | kz << 16; then the
+    // offsets of the box's own faces, of the x face before it, of the y
+    // face before it and of the top plane below it (-1: none)
+    const int4 *box;
     const int *rowptr, *colidx;
     int *pcount;
     Ctx ctx;
@@ -53,8 +59,6 @@ struct PipeArgs {
     long long spin_limit;
 };
 
-constexpr int PIPE_MAX_EXT = 1024;  // parents in other boxes per box (the plan keeps boxes within)
-constexpr int PIPE_POLL_AHEAD = 16;  // steps ahead of the progress word the exchange polls
 #ifndef PIPE_POLL_SLEEP
 #define PIPE_POLL_SLEEP 1  // s_sleep units (64 clocks) between the exchange's polling passes
 #endif
@@ -106,65 +110,54 @@ __device__ __forceinline__ int wave_min(int v) {
     return __builtin_amdgcn_readlane(v, 63);
 }
 
-// The exchange wavefront: copy the box's parents in other boxes (item e
-// of the list: row er[e], slot ev[e]) from the publication half into LDS as
-// they are published, and keep *ready at the first pipeline step that
-// needs an item not yet staged (nsteps once all are): the plane below's item
-// (i, j) is needed at step i + j, an x-face item (j, plane k) at k + j, a
-// y-face item (i, plane k) at k + i.  Each lane owns items lane, lane + 64,
-// ..., at most R of them, and polls, in every pass, its missing ones needed
-// within PIPE_POLL_AHEAD steps of the progress word, with a short sleep
-// between passes (more frequent polling -- two passes in flight, or none
-// of the sleep -- measured slower: the polls load the memory system the
-// publications travel through).  A pass has no branches: items not polled
-// load a fixed harmless address, and values not found are written to a
-// dump slot.  A wait beyond the spin limit reports BAAS_TIMEOUT and stages
-// zeros.
-template <class T, int R>
-__device__ __forceinline__ void pipe_stage(const T *pub, T *ev, const int *er, int ne, int nzf, int nxk, int nyk,
-                                           int tx, int nsteps, int *ready, int lane, long long limit,
-                                           DevStatus *status, int dump) {
+// The exchange wavefront: copy the faces the box reads from the boxes
+// before it (ex: the x face, ey: the y face, ez: the plane below; px, py,
+// pz in the publication half, null where there is none) into LDS as they
+// are published, and keep *ready at the first pipeline step that needs a
+// value not yet staged (nsteps once all are).  The faces come in the order
+// they are needed: x-face value c at step c / ty, y-face value c at c / tx;
+// of the plane below, value i + tx j at step i + j.  For each face a cursor
+// marks the first value not yet seen; every pass loads the 64 values from
+// the cursor (rounded down to a multiple of 8) in one coalesced load, and
+// the whole plane below (one value a lane) until it is complete; values
+// found are written to their slots again, which changes nothing.  A pass
+// has no branches: lanes with nothing to load read `harmless`, and values
+// not found are written to the dump slot.  A short sleep separates passes
+// (more frequent polling loads the memory system the publications travel
+// through).  A wait beyond the spin limit reports BAAS_TIMEOUT and stages
+// zeros.  Once every value is staged (or the wait gave up), the faces are
+// re-armed with the sentinel in the other publication half (`other`
+// values away), so the producers' steps carry no second store.
+template <class T>
+__device__ __forceinline__ void pipe_stage(const T *px, int nxf, int ty, const T *py, int nyf, int tx, const T *pz,
+                                           int nzf, T *ex, T *ey, T *ez, T *dump, const T *harmless, int nsteps,
+                                           int *ready, int lane, long long limit, DevStatus *status, long long other) {
     constexpr int BIG = 0x7fffffff;
-    const int fs = nxk + nyk;
-    int need[R], rows[R];
-    unsigned todo = 0;
-#pragma unroll
-    for (int r = 0; r < R; r++) {
-        const int e = lane + 64 * r;
-        const bool have = e < ne;
-        todo |= (have ? 1u : 0u) << r;
-        rows[r] = have ? er[e] : 0;
-        int nd = BIG;
-        if (have) {
-            if (e < nzf) {
-                nd = e % tx + e / tx;
-            } else {
-                const int f = e - nzf, k = f / fs, w = f - k * fs;
-                nd = k + (w < nxk ? w : w - nxk);
-            }
-        }
-        need[r] = nd;
-    }
+    if (!px) nxf = 0;
+    if (!py) nyf = 0;
+    bool zt = pz && lane < nzf;  // this lane's value of the plane below is still missing
+    const int zneed = lane % tx + lane / tx;
+    int cx = 0, cy = 0;
     int last = -1;  // the progress word last written
     long long spins = 0;
     for (;;) {
-        T v[R];
-        unsigned pm = 0;
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-            const bool p = ((todo >> r) & 1u) & (need[r] < last + PIPE_POLL_AHEAD);
-            pm |= (p ? 1u : 0u) << r;
-            v[r] = ld_agent(&pub[p ? rows[r] : 0]);
-        }
-        int first = BIG;
-#pragma unroll
-        for (int r = 0; r < R; r++) {
-            const bool found = ((pm >> r) & 1u) & !Sentinel<T>::is(v[r]);
-            st_lds(&ev[found ? lane + 64 * r : dump], v[r]);
-            todo &= ~((found ? 1u : 0u) << r);
-            first = min(first, ((todo >> r) & 1u) ? need[r] : BIG);
-        }
-        first = min(wave_min(first), nsteps);
+        const int wx = cx & ~7, wy = cy & ~7;
+        const bool qx = (cx < nxf) & (wx + lane < nxf), qy = (cy < nyf) & (wy + lane < nyf);
+        const T vx = ld_agent(qx ? px + wx + lane : harmless);
+        const T vy = ld_agent(qy ? py + wy + lane : harmless);
+        const T vz = ld_agent(zt ? pz + lane : harmless);
+        const bool fx = qx & !Sentinel<T>::is(vx), fy = qy & !Sentinel<T>::is(vy), fz = zt & !Sentinel<T>::is(vz);
+        st_lds(fx ? ex + wx + lane : dump, vx);
+        st_lds(fy ? ey + wy + lane : dump, vy);
+        st_lds(fz ? ez + lane : dump, vz);
+        zt = zt & !fz;
+        const unsigned long long mx = __ballot(qx & !fx), my = __ballot(qy & !fy);
+        cx = mx ? wx + (int)__builtin_ctzll(mx) : min(wx + 64, nxf);
+        cy = my ? wy + (int)__builtin_ctzll(my) : min(wy + 64, nyf);
+        int first = nsteps;
+        if (cx < nxf) first = min(first, cx / ty);
+        if (cy < nyf) first = min(first, cy / tx);
+        if (__ballot(zt)) first = min(first, wave_min(zt ? zneed : BIG));
         // release: the staged values are in LDS before ready moves past them
         if (first != last && lane == 0)
             __hip_atomic_store(ready, first, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -172,14 +165,19 @@ __device__ __forceinline__ void pipe_stage(const T *pub, T *ev, const int *er, i
         if (first >= nsteps) break;
         if (++spins > limit) {
             raise_status(status, BAAS_TIMEOUT, last);
-#pragma unroll
-            for (int r = 0; r < R; r++)
-                if ((todo >> r) & 1u) st_lds(&ev[lane + 64 * r], T(0));
+            for (int c = cx + lane; c < nxf; c += 64) st_lds(ex + c, T(0));
+            for (int c = cy + lane; c < nyf; c += 64) st_lds(ey + c, T(0));
+            if (zt) st_lds(ez + lane, T(0));
             if (lane == 0) __hip_atomic_store(ready, nsteps, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
             break;
         }
         __builtin_amdgcn_s_sleep(PIPE_POLL_SLEEP);
     }
+    // every value of these faces is read by this box alone: re-arm them in
+    // the other half, which the next sweep publishes into
+    for (int c = lane; c < nxf; c += 64) const_cast<T *>(px)[other + c] = Sentinel<T>::value();
+    for (int c = lane; c < nyf; c += 64) const_cast<T *>(py)[other + c] = Sentinel<T>::value();
+    if (pz && lane < nzf) const_cast<T *>(pz)[other + lane] = Sentinel<T>::value();
 }
 
 // The compute wavefront reads the progress word relaxed, then the staged
@@ -196,19 +194,18 @@ __device__ __forceinline__ int ld_ready(const int *ready) {
 // LDS of one box of at most RMX rows (compile-time, so every record read is
 // one address register plus an immediate offset): per row the three parent
 // coefficients (z, y, x), the diagonal, b and the solution, each in an
-// array of RMX + 1 values (index RMX is the
-// spare row idle lanes read and write); then the external parents' slots
-// (PIPE_MAX_EXT values, a zero slot and a dump slot), their rows and the
-// progress word.
+// array of RMX + 1 values (index RMX is the spare row idle lanes read and
+// write); then the staged faces (EX: the x face, EY: the y face, each in
+// the order of its publication, EZ: the plane below), a zero slot and a
+// dump slot; then the progress word.
 template <class T, int RMX>
 struct PipeLds {
     static constexpr int NS = RMX + 1;
+    static constexpr int EX = 0, EY = PIPE_FACE_MAX, EZ = 2 * PIPE_FACE_MAX, ZERO = EZ + 64, DUMP = ZERO + 1,
+                         NE = DUMP + 1;
     static constexpr size_t C0 = 0, C1 = C0 + NS * sizeof(T), C2 = C1 + NS * sizeof(T), DG = C2 + NS * sizeof(T),
-                            BV = DG + NS * sizeof(T), XV = BV + NS * sizeof(T),
-                            EV = XV + NS * sizeof(T), ER = EV + (PIPE_MAX_EXT + 2) * sizeof(T),
-                            RDY = ER + PIPE_MAX_EXT * sizeof(int), TOTAL = RDY + 16;
-    static constexpr int ZERO = PIPE_MAX_EXT;      // the zero slot of ev
-    static constexpr int DUMP = PIPE_MAX_EXT + 1;  // where the exchange writes values it did not find
+                            BV = DG + NS * sizeof(T), XV = BV + NS * sizeof(T), EV = XV + NS * sizeof(T),
+                            RDY = EV + NE * sizeof(T), TOTAL = RDY + 16;
 };
 
 // One step's inputs of a lane, read a step ahead so that their LDS reads
@@ -227,11 +224,16 @@ struct PipeRec {
 // 0 then takes the x-face value as the shift's `old`; the LDS crossbar
 // otherwise).  A lane without a row this step (before its first plane,
 // after its last, outside a partial box) reads and writes the spare row,
-// whose coefficients are -0, diagonal 1 and b +0, and so computes +0.
+// whose coefficients are -0, diagonal 1 and b +0.  hx, hy, hz: the box has
+// staged x, y and z faces; opub: its own faces in the publication half.
+// The x-face lanes (i = tx - 1) write diagonal d = t - (tx - 1) of the x
+// face at step t, all of it, with +0 for lanes that have no row this step;
+// the y face likewise; the top plane row by row.  A face value's offset is
+// linear in t, and whether a step writes a diagonal depends on t alone.
 template <class T, bool UPPER, int XS, int RMX>
 __device__ __forceinline__ void pipe_wave(const PipeArgs &a, int lane, int x0, int y0, int z0, int tx, int ty,
-                                          int kz, int nzf, int nxk, int nyk, unsigned char *smem, const int *ready,
-                                          T *pub, T *nxt) {
+                                          int kz, bool hx, bool hy, bool hz, int3 fsz, unsigned char *smem,
+                                          const int *ready, T *opub) {
     using Lay = PipeLds<T, RMX>;
     const T *c0 = reinterpret_cast<const T *>(smem + Lay::C0);
     const T *c1 = reinterpret_cast<const T *>(smem + Lay::C1);
@@ -245,19 +247,50 @@ __device__ __forceinline__ void pipe_wave(const PipeArgs &a, int lane, int x0, i
     const bool lok = i < tx && j < ty;
     const int nsteps = kz + tx + ty - 2;
     const int txy = tx * ty;
-    const bool pubx = x0 + tx < a.nx, puby = y0 + ty < a.ny, pubz = z0 + kz < a.nz;
-    const bool face_xy = lok && ((i == tx - 1 && pubx) || (j == ty - 1 && puby));
-    const int kpub = pubz ? kz - 1 : -1;
-    const int xaddr = ((lane + 64 - TY) & 63) << 2;
-    const int fs = nxk + nyk;
-    // exchange slots: the plane below, and per plane the x face (lanes with
+    const bool bx = x0 + tx < a.nx, by = XS != 1 && a.sy > 0 && y0 + ty < a.ny;  // consumers exist
+    const bool oz = lok && z0 + kz < a.nz;
+    // The helper lane's x move reads the corner lane (tx - 1, ty - 1)
+    // instead of its x parent: see the y face below.  The helper is lane 0,
+    // whose own x parent is staged, or, in a box one lane wide in x or y
+    // (where lane 0 is itself on a face), lane 63, which then lies outside
+    // the box.
+    const int corner = (tx - 1) * TY + (ty - 1);
+    const int helper = tx == 1 || ty == 1 ? 63 : 0;
+    const int xaddr = (XS != 1 && lane == helper ? corner : (lane + 64 - TY) & 63) << 2;
+    // staged slots: the plane below, and per plane the x face (lanes with
     // i = 0) or the y face (lanes with j = 0); the zero slot where a box
     // has no such face (a parent that does not exist has value +0)
     const bool fx = lok && i == 0, fy = lok && j == 0;
-    const int zslot = lok && nzf > 0 ? i + tx * j : Lay::ZERO;
-    const int xs0 = fx && nxk > 0 ? nzf + j : Lay::ZERO, xst = fx && nxk > 0 ? fs : 0;
-    const int ys0 = fy && nyk > 0 ? nzf + nxk + i : Lay::ZERO, yst = fy && nyk > 0 ? fs : 0;
+    const int zslot = lok && hz ? Lay::EZ + i + tx * j : Lay::ZERO;
+    const int xs0 = fx && hx ? Lay::EX + j * (ty + 1) : Lay::ZERO, xst = fx && hx ? ty : 0;
+    const int ys0 = fy && hy ? Lay::EY + i * (tx + 1) : Lay::ZERO, yst = fy && hy ? tx : 0;
     const int qb = j * tx + i;
+    // Publication, one store of every lane per step (lanes with nothing to
+    // publish write the box's dump slot): at step t a lane writes slot
+    // t sA + cA while t - aA is in [0, nA).  x-face lanes (i = tx - 1) write
+    // diagonal t - (tx - 1) of the x face; y-face lanes (j = ty - 1) that of
+    // the y face, except the corner, which is on the x face too: its y-face
+    // value reaches the helper lane through the helper's x move one step
+    // later, and the helper writes it (a last such store follows the final
+    // step).  The top plane
+    // is a second store, in the steps from kz - 1 on: the row of plane
+    // kz - 1 at step tz, else the dump slot.
+    const int dmp = fsz.x + fsz.y + fsz.z;
+    int sA = 0, cA = 0, aA = 0, nA = 0;
+    const bool cy_help = XS != 1 && lane == helper && by && bx;
+    if (lok && i == tx - 1 && bx) {
+        sA = ty, cA = j - (tx - 1) * ty, aA = tx - 1, nA = kz + ty - 1;
+    } else if (lok && j == ty - 1 && by) {
+        sA = tx, cA = fsz.x + i - (ty - 1) * tx, aA = ty - 1, nA = kz + tx - 1;
+    } else if (cy_help) {
+        sA = tx, cA = fsz.x + (tx - 1) - ty * tx, aA = ty, nA = kz + tx - 1;
+    }
+    const int oz0 = oz ? fsz.x + fsz.y + i + tx * j : dmp;
+    const int tz = kz - 1 + i + j;
+    auto publish_a = [&](int t, T v) {
+        const int o = (unsigned)(t - aA) < (unsigned)nA ? t * sA + cA : dmp;
+        st_agent(opub + (unsigned)o, v);
+    };
     auto slot = [&](int k) { return lok & ((unsigned)k < (unsigned)kz) ? qb + k * txy : RMX; };
     auto load_ext = [&](int k, PipeRec<T> &R) {
         const int kc = min(max(k, 0), kz - 1);
@@ -318,11 +351,11 @@ __device__ __forceinline__ void pipe_wave(const PipeArgs &a, int lane, int x0, i
         xl = (C.bb - s) / C.dd;
         const int q = slot(k);
         xv[q] = xl;
-        if (q != RMX && (face_xy || k == kpub)) {
-            const int row = pipe_row<UPPER>(a, x0 + i, y0 + j, z0 + k);
-            st_agent(&pub[row], xl);
-            nxt[row] = Sentinel<T>::value();
-        }
+        // (a lane without a row this step publishes a value no consumer
+        // reads: idle lanes take parents only from idle lanes, and their
+        // arithmetic never yields the signalling-NaN sentinel)
+        publish_a(t, cy_help ? vx : xl);
+        if (t >= kz - 1) st_agent(opub + (unsigned)(t == tz ? oz0 : dmp), xl);
     };
     // two register sets used in turn, so no step copies one into the other
     PipeRec<T> A, B;
@@ -331,9 +364,15 @@ __device__ __forceinline__ void pipe_wave(const PipeArgs &a, int lane, int x0, i
         step(t, A, B);
         if (t + 1 < nsteps) step(t + 1, B, A);
     }
+    if (XS != 1) {
+        // the corner's last y-face value (its final step's row)
+        T vc = lane_read(xl, xaddr);
+        __asm__ volatile("" : "+v"(vc));
+        if (cy_help) publish_a(nsteps, vc);
+    }
 }
 
-template <class T, bool UPPER, int XS, int RMX, int R>
+template <class T, bool UPPER, int XS, int RMX>
 __global__ __launch_bounds__(128) void k_sweep_pipe(PipeArgs a, const T *__restrict__ vals, const T *__restrict__ b,
                                                     T *x, T *xp) {
     using Lay = PipeLds<T, RMX>;
@@ -341,9 +380,12 @@ __global__ __launch_bounds__(128) void k_sweep_pipe(PipeArgs a, const T *__restr
     __shared__ int s_unit;
     if (a.skip && ld_agent(a.skip)) return;
     const int epoch = read_epoch(a.ctx);
-    const int par = ld_agent(a.pcount) & 1;
-    T *pub = xp + (size_t)par * a.n;
-    T *nxt = xp + (size_t)(par ^ 1) * a.n;
+    // wave-uniform values are made so explicitly (readfirstlane), so that
+    // the box's fields and the publication pointers live in scalar
+    // registers and stores address them as scalar base + 32-bit offset
+    const int par = __builtin_amdgcn_readfirstlane(ld_agent(a.pcount)) & 1;
+    T *pub = xp + (size_t)par * a.half;
+    T *nxt = xp + (size_t)(par ^ 1) * a.half;
     T *c0 = reinterpret_cast<T *>(smem_raw + Lay::C0);
     T *c1 = reinterpret_cast<T *>(smem_raw + Lay::C1);
     T *c2 = reinterpret_cast<T *>(smem_raw + Lay::C2);
@@ -351,7 +393,6 @@ __global__ __launch_bounds__(128) void k_sweep_pipe(PipeArgs a, const T *__restr
     T *bv = reinterpret_cast<T *>(smem_raw + Lay::BV);
     const T *xv = reinterpret_cast<const T *>(smem_raw + Lay::XV);
     T *ev = reinterpret_cast<T *>(smem_raw + Lay::EV);
-    int *er = reinterpret_cast<int *>(smem_raw + Lay::ER);
     int *ready = reinterpret_cast<int *>(smem_raw + Lay::RDY);
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nt = blockDim.x;
     // the spare row (idle lanes compute +0 on it) and the zero slot
@@ -362,37 +403,18 @@ __global__ __launch_bounds__(128) void k_sweep_pipe(PipeArgs a, const T *__restr
         ev[Lay::ZERO] = T(0);
     }
     for (;;) {
-        const int u = claim_unit(a.ctx.ticket, &s_unit);
+        const int u = __builtin_amdgcn_readfirstlane(claim_unit(a.ctx.ticket, &s_unit));
         if (u >= a.nunits) break;
-        const int4 bx = a.box[u];
+        const int4 bx = a.box[2 * u], bf = a.box[2 * u + 1];
         const int x0 = bx.x, y0 = bx.y, z0 = bx.z;
         const int tx = bx.w & 255, ty = (bx.w >> 8) & 255, kz = (bx.w >> 16) & 255;
-        const int nrow = tx * ty * kz, txy = tx * ty, nsteps = kz + tx + ty - 2;
-        // parents in other boxes: the plane below, then per plane the x face
-        // (ty rows) and the y face (tx rows)
-        const int nzf = z0 > 0 ? txy : 0, nxk = x0 > 0 ? ty : 0, nyk = (a.sy > 0 && y0 > 0) ? tx : 0;
-        const int ne = nzf + kz * (nxk + nyk);
-        for (int e = tid; e < ne; e += nt) {
-            int xx, yy, zz;
-            if (e < nzf) {
-                xx = x0 + e % tx;
-                yy = y0 + e / tx;
-                zz = z0 - 1;
-            } else {
-                const int f = e - nzf, k = f / (nxk + nyk), w = f - k * (nxk + nyk);
-                zz = z0 + k;
-                if (w < nxk) {
-                    xx = x0 - 1;
-                    yy = y0 + w;
-                } else {
-                    xx = x0 + (w - nxk);
-                    yy = y0 - 1;
-                }
-            }
-            er[e] = pipe_row<UPPER>(a, xx, yy, zz);
-            st_lds(&ev[e], Sentinel<T>::value());
-        }
-        if (tid == 0) *ready = ne > 0 ? 0 : nsteps;
+        const int nrow = tx * ty * kz, nsteps = kz + tx + ty - 2;
+        // the faces this box reads: the x face and y face of the boxes
+        // before it and the top plane of the box below (none at the grid's
+        // low faces), each as long as this box's own
+        const int3 fsz = pipe_face_sizes(tx, ty, kz, a.sy > 0);
+        const bool hx = bf.y >= 0, hy = bf.z >= 0, hz = bf.w >= 0;
+        if (tid == 0) *ready = hx | hy | hz ? 0 : nsteps;
         // the box's rows: eight per thread per pass, every load of a pass in
         // flight together (row pointers and b, then the entries); a parent
         // the row lacks gets coefficient -0
@@ -447,10 +469,12 @@ __global__ __launch_bounds__(128) void k_sweep_pipe(PipeArgs a, const T *__restr
         }
         __syncthreads();
         if (wv == 0)
-            pipe_stage<T, R>(pub, ev, er, ne, nzf, nxk, nyk, tx, nsteps, ready, lane, a.spin_limit, a.status,
-                             Lay::DUMP);
+            pipe_stage<T>(hx ? pub + bf.y : nullptr, fsz.x, ty, hy ? pub + bf.z : nullptr, fsz.y, tx,
+                          hz ? pub + bf.w : nullptr, fsz.z, ev + Lay::EX, ev + Lay::EY, ev + Lay::EZ, ev + Lay::DUMP,
+                          pub, nsteps, ready, lane, a.spin_limit, a.status, (long long)(nxt - pub));
         else if (wv == 1)
-            pipe_wave<T, UPPER, XS, RMX>(a, lane, x0, y0, z0, tx, ty, kz, nzf, nxk, nyk, smem_raw, ready, pub, nxt);
+            pipe_wave<T, UPPER, XS, RMX>(a, lane, x0, y0, z0, tx, ty, kz, hx, hy, hz, fsz, smem_raw, ready,
+                                         pub + bf.x);
         __syncthreads();
         for (int q = tid; q < nrow; q += nt) {
             const int ii = q % tx, r2 = q / tx, jj = r2 % ty, kk = r2 / ty;
@@ -482,14 +506,14 @@ size_t pipe_lds(const Plan &p, int ts) {
                    : (r1 ? PipeLds<float, 1024>::TOTAL : PipeLds<float, 2048>::TOTAL);
 }
 
-template <class T, int RMX, int R>
+template <class T, int RMX>
 static int launch_pipe_r(baas_handle *h, const PipeArgs &a0, int uplo, const T *vals, const T *b, T *x, T *xp,
                          int nunits, int nrhs, hipStream_t stream) {
     // the x parent sits TY lanes down: a DPP shift for TY = 1 (2D grids),
     // the LDS crossbar otherwise
     using K = void (*)(PipeArgs, const T *, const T *, T *, T *);
-    K kern = a0.TY == 1 ? (uplo ? k_sweep_pipe<T, true, 1, RMX, R> : k_sweep_pipe<T, false, 1, RMX, R>)
-                        : (uplo ? k_sweep_pipe<T, true, 0, RMX, R> : k_sweep_pipe<T, false, 0, RMX, R>);
+    K kern = a0.TY == 1 ? (uplo ? k_sweep_pipe<T, true, 1, RMX> : k_sweep_pipe<T, false, 1, RMX>)
+                        : (uplo ? k_sweep_pipe<T, true, 0, RMX> : k_sweep_pipe<T, false, 0, RMX>);
     const size_t lds = PipeLds<T, RMX>::TOTAL;
     int per_cu = 0;
     BAAS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 128, lds));
@@ -522,6 +546,7 @@ int launch_pipe(baas_handle *h, baas_schedule *s, const Plan &p, int uplo, const
     a.TX = p.pg[5];
     a.TY = p.pg[6];
     a.K = p.pg[7];
+    a.half = p.pipe_half;
     a.box = reinterpret_cast<const int4 *>(p.pipe_box.p);
     a.rowptr = uplo ? s->u_rowptr.p : s->l_rowptr.p;
     a.colidx = uplo ? s->u_colidx.p : s->l_colidx.p;
@@ -530,22 +555,11 @@ int launch_pipe(baas_handle *h, baas_schedule *s, const Plan &p, int uplo, const
     a.skip = skip;
     a.status = h->status;
     a.spin_limit = h->spin_limit;
-    if (a.TX * a.TY * a.K > 2048) return BAAS_UNSUPPORTED;
-    // items per exchange lane: the largest box's parents in other boxes / 64
-    const int nem = (int)pipe_ext_max(a.TX, a.TY, a.K, a.sy > 0);
-    if (nem > PIPE_MAX_EXT) return BAAS_UNSUPPORTED;
-    const bool r1 = pipe_rmx(p) == 1024;
-    if (nem <= 128)
-        return r1 ? launch_pipe_r<T, 1024, 2>(h, a, uplo, vals, b, x, xp, p.nunits, nrhs, stream)
-                  : launch_pipe_r<T, 2048, 2>(h, a, uplo, vals, b, x, xp, p.nunits, nrhs, stream);
-    if (nem <= 256)
-        return r1 ? launch_pipe_r<T, 1024, 4>(h, a, uplo, vals, b, x, xp, p.nunits, nrhs, stream)
-                  : launch_pipe_r<T, 2048, 4>(h, a, uplo, vals, b, x, xp, p.nunits, nrhs, stream);
-    if (nem <= 512)
-        return r1 ? launch_pipe_r<T, 1024, 8>(h, a, uplo, vals, b, x, xp, p.nunits, nrhs, stream)
-                  : launch_pipe_r<T, 2048, 8>(h, a, uplo, vals, b, x, xp, p.nunits, nrhs, stream);
-    return r1 ? launch_pipe_r<T, 1024, 16>(h, a, uplo, vals, b, x, xp, p.nunits, nrhs, stream)
-              : launch_pipe_r<T, 2048, 16>(h, a, uplo, vals, b, x, xp, p.nunits, nrhs, stream);
+    // the kernel's LDS holds boxes of up to 2048 rows and faces of up to
+    // PIPE_FACE_MAX values (the plan builder keeps within both)
+    if (a.TX * a.TY * a.K > 2048 || !pipe_faces_fit(a.TX, a.TY, a.K, a.sy > 0) || a.half < 1) return BAAS_UNSUPPORTED;
+    return pipe_rmx(p) == 1024 ? launch_pipe_r<T, 1024>(h, a, uplo, vals, b, x, xp, p.nunits, nrhs, stream)
+                               : launch_pipe_r<T, 2048>(h, a, uplo, vals, b, x, xp, p.nunits, nrhs, stream);
 }
 
 template int launch_pipe<double>(baas_handle *, baas_schedule *, const Plan &, int, const double *, const double *,

@@ -196,6 +196,7 @@ int make_plan(Plan &p, int n, const std::vector<int> &order, const std::vector<i
     p.lane_lds_key = -1;
     p.mr_lds_key = -1;
     p.pipe = 0;  // a PIPE plan's boxes are set by its builder after this
+    p.pipe_half = 0;
     BAAS_TRY(upload(p.scol, scol, st));
     BAAS_TRY(upload(p.ext_ptr, ext_ptr, st));
     BAAS_TRY(upload(p.ext_cols, ext_cols, st));
@@ -422,14 +423,19 @@ static int box_units(const baas_schedule *s, const HostCsr &L, int R, std::vecto
 // becoming the pipeline's planes.  Boxes are TX x TY rows across (8 x 8 on
 // a 3D grid, 64 x 1 on a 2D grid, narrower where the grid is) by
 // K = R / (TX TY) planes, numbered in wavefront order like BOXES units.
-// Per unit, box[4u..4u+3] = origin x, y, z and tx | ty << 8 | kz << 16 in
-// the forward direction's coordinates; the backward plan flips every
-// coordinate (x -> nx - 1 - x, ...) and takes the boxes in reverse order.
+// The backward sweep takes the same boxes in reverse order, each in flipped
+// coordinates (x -> nx - 1 - x, ...).  Per unit and sweep, box[8u..8u+7] =
+// origin x, y, z and tx | ty << 8 | kz << 16 in that sweep's coordinates,
+// then where faces live in a publication half: the box's own faces (see
+// pipe_face_sizes; then PIPE_DUMP values its lanes with nothing to publish
+// in a step write instead), and the x face of the box before it in x, the y face
+// of the box before it in y and the top plane of the box below it (-1
+// where there is none).  half = the values of all boxes' faces.
 struct PipeGeom {
     int nx = 0, ny = 1, nz = 1, sy = -1, sz = 0, TX = 1, TY = 1, K = 1;
 };
 static int pipe_units(const baas_schedule *s, const HostCsr &L, int R, std::vector<int> &order, std::vector<int> &fu,
-                      PipeGeom &g, std::vector<int> &box) {
+                      PipeGeom &g, std::vector<int> &fbox, std::vector<int> &bbox, long long &half) {
     const int n = s->n;
     if (R < 64 || !s->block_ptr.empty()) return R < 64 ? BAAS_INVALID_ARG : BAAS_UNSUPPORTED;
     std::vector<long long> offs;
@@ -472,8 +478,9 @@ static int pipe_units(const baas_schedule *s, const HostCsr &L, int R, std::vect
     g.TX = three ? std::min(8, g.nx) : std::min(64, g.nx);
     g.TY = three ? std::min(64 / g.TX, g.ny) : 1;
     g.K = std::max(1, std::min({R / (g.TX * g.TY), g.nz, 255, 2048 / (g.TX * g.TY)}));
-    // a box's parents in other boxes are staged a fixed number at a time
-    while (g.K > 1 && pipe_ext_max(g.TX, g.TY, g.K, three) > 1024) g.K--;
+    // a box's staged faces have room for PIPE_FACE_MAX values each
+    while (g.K > 1 && !pipe_faces_fit(g.TX, g.TY, g.K, three)) g.K--;
+    if (!pipe_faces_fit(g.TX, g.TY, g.K, three)) return BAAS_UNSUPPORTED;
     const long long gx = (g.nx + g.TX - 1) / g.TX, gy = (g.ny + g.TY - 1) / g.TY, gz = (g.nz + g.K - 1) / g.K;
     const long long nbox = gx * gy * gz;
     if (nbox > (1LL << 26)) return BAAS_INVALID_ARG;
@@ -492,16 +499,56 @@ static int pipe_units(const baas_schedule *s, const HostCsr &L, int R, std::vect
     for (long long t = 0; t < nbox; t++) start[t + 1] += start[t];
     std::vector<int> next(start.begin(), start.end() - 1);
     for (int r = 0; r < n; r++) order[next[box_of(r)]++] = r;
-    box.assign(4 * (size_t)nbox, 0);
+    // forward unit of the box at grid position (bx, by, bz); -1 outside
+    auto unit_at = [&](long long bx, long long by, long long bz) -> long long {
+        if (bx < 0 || by < 0 || bz < 0 || bx >= gx || by >= gy || bz >= gz) return -1;
+        return rank[(bz * gy + by) * gx + bx];
+    };
+    std::vector<int> dims((size_t)nbox);
+    std::vector<long long> fsz((size_t)nbox);  // face values of forward unit q
+    for (long long q = 0; q < nbox; q++) {
+        const long long t = ids[q];
+        const int x0 = (int)(t % gx) * g.TX, y0 = (int)((t / gx) % gy) * g.TY, z0 = (int)(t / (gx * gy)) * g.K;
+        const int tx = std::min(g.TX, g.nx - x0), ty = std::min(g.TY, g.ny - y0), kz = std::min(g.K, g.nz - z0);
+        dims[q] = tx | (ty << 8) | (kz << 16);
+        const int3 f = pipe_face_sizes(tx, ty, kz, three);
+        fsz[q] = (long long)f.x + f.y + f.z + PIPE_DUMP;
+    }
+    // the faces of forward unit q start at fb[q] in the forward sweep and at
+    // bb[q] in the backward one (which takes the units in reverse order)
+    std::vector<long long> fb((size_t)nbox), bb((size_t)nbox);
+    half = 0;
+    for (long long q = 0; q < nbox; q++) fb[q] = half, half += fsz[q];
+    long long acc = 0;
+    for (long long q = nbox - 1; q >= 0; q--) bb[q] = acc, acc += fsz[q];
+    if (half >= (1LL << 31)) return BAAS_UNSUPPORTED;
+    // offset of part 0 (x face), 1 (y face) or 2 (top plane) of unit q's faces
+    auto part = [&](const std::vector<long long> &base, long long q, int which) -> int {
+        if (q < 0) return -1;
+        const int3 f = pipe_face_sizes(dims[q] & 255, (dims[q] >> 8) & 255, (dims[q] >> 16) & 255, three);
+        return (int)(base[q] + (which > 0 ? f.x : 0) + (which > 1 ? f.y : 0));
+    };
+    fbox.assign(8 * (size_t)nbox, -1);
+    bbox.assign(8 * (size_t)nbox, -1);
     for (long long q = 0; q < nbox; q++) {
         const long long t = ids[q];
         const int bx = (int)(t % gx), by = (int)((t / gx) % gy), bz = (int)(t / (gx * gy));
         const int x0 = bx * g.TX, y0 = by * g.TY, z0 = bz * g.K;
-        const int tx = std::min(g.TX, g.nx - x0), ty = std::min(g.TY, g.ny - y0), kz = std::min(g.K, g.nz - z0);
-        box[4 * q + 0] = x0;
-        box[4 * q + 1] = y0;
-        box[4 * q + 2] = z0;
-        box[4 * q + 3] = tx | (ty << 8) | (kz << 16);
+        const int tx = dims[q] & 255, ty = (dims[q] >> 8) & 255, kz = (dims[q] >> 16) & 255;
+        int *f = &fbox[8 * (size_t)q];
+        f[0] = x0, f[1] = y0, f[2] = z0, f[3] = dims[q];
+        f[4] = (int)fb[q];
+        f[5] = part(fb, unit_at(bx - 1, by, bz), 0);
+        f[6] = three ? part(fb, unit_at(bx, by - 1, bz), 1) : -1;
+        f[7] = part(fb, unit_at(bx, by, bz - 1), 2);
+        // backward: unit nbox-1-q, flipped; its producers are the forward
+        // neighbours after it
+        int *r = &bbox[8 * (size_t)(nbox - 1 - q)];
+        r[0] = g.nx - (x0 + tx), r[1] = g.ny - (y0 + ty), r[2] = g.nz - (z0 + kz), r[3] = dims[q];
+        r[4] = (int)bb[q];
+        r[5] = part(bb, unit_at(bx + 1, by, bz), 0);
+        r[6] = three ? part(bb, unit_at(bx, by + 1, bz), 1) : -1;
+        r[7] = part(bb, unit_at(bx, by, bz + 1), 2);
         fu.push_back(start[q]);
     }
     fu.push_back(n);
@@ -628,7 +675,8 @@ int build_plans_mode(baas_schedule *s, int mode, int rows_per_unit) {
     int block = s->opts.block_threads > 0 ? s->opts.block_threads : 256;
     std::vector<int> fwd_order(n), bwd_order(n), fu, bu;
     PipeGeom pg;
-    std::vector<int> pbox;
+    std::vector<int> pbox, bbox;
+    long long phalf = 0;
     if (mode == BAAS_PLAN_RANGE) {
         int R = rows_per_unit > 0 ? rows_per_unit : 4096;
         for (int r = 0; r < n; r++) fwd_order[r] = r, bwd_order[r] = n - 1 - r;
@@ -643,7 +691,7 @@ int build_plans_mode(baas_schedule *s, int mode, int rows_per_unit) {
     if (mode == BAAS_PLAN_TILES) {
         BAAS_TRY(tile_units(s, rows_per_unit > 0 ? rows_per_unit : 16384, s->tile_band, fwd_order, fu));
     } else if (mode == BAAS_PLAN_PIPE) {
-        BAAS_TRY(pipe_units(s, L, rows_per_unit > 0 ? rows_per_unit : 1024, fwd_order, fu, pg, pbox));
+        BAAS_TRY(pipe_units(s, L, rows_per_unit > 0 ? rows_per_unit : 1024, fwd_order, fu, pg, pbox, bbox, phalf));
     } else if (mode == BAAS_PLAN_BOXES) {
         if (!s->block_ptr.empty())
             BAAS_TRY(block_box_units(s, L, rows_per_unit > 0 ? rows_per_unit : 512, fwd_order, fu));
@@ -659,23 +707,13 @@ int build_plans_mode(baas_schedule *s, int mode, int rows_per_unit) {
     BAAS_TRY(make_plan(s->fwd, n, fwd_order, fu, lp, li, up, ui, 2, mode, block, st, L, false));
     BAAS_TRY(make_plan(s->bwd, n, bwd_order, bu, up, ui, lp, li, 2, mode, block, st, U, true));
     if (mode == BAAS_PLAN_PIPE) {
-        // the backward sweep runs the same boxes in reverse order, each in
-        // flipped coordinates (parents of U at +1, +sy, +sz become -1, ...)
-        std::vector<int> bbox(pbox.size());
-        for (int u = 0; u < nu; u++) {
-            const int *f = &pbox[4 * (size_t)(nu - 1 - u)];
-            const int tx = f[3] & 255, ty = (f[3] >> 8) & 255, kz = (f[3] >> 16) & 255;
-            bbox[4 * (size_t)u + 0] = pg.nx - (f[0] + tx);
-            bbox[4 * (size_t)u + 1] = pg.ny - (f[1] + ty);
-            bbox[4 * (size_t)u + 2] = pg.nz - (f[2] + kz);
-            bbox[4 * (size_t)u + 3] = f[3];
-        }
         const int g8[8] = {pg.nx, pg.ny, pg.nz, pg.sy, pg.sz, pg.TX, pg.TY, pg.K};
         BAAS_TRY(upload(s->fwd.pipe_box, pbox, st));
         BAAS_TRY(upload(s->bwd.pipe_box, bbox, st));
         if (hipStreamSynchronize(st) != hipSuccess) return BAAS_HIP_ERROR;
         for (Plan *p : {&s->fwd, &s->bwd}) {
             std::copy(g8, g8 + 8, p->pg);
+            p->pipe_half = phalf;
             p->pipe = 1;
         }
     }

@@ -1184,26 +1184,30 @@ int sweep(baas_handle *h, baas_schedule *s, int uplo, const T *vals, const T *b,
     if (lane || pipe ||
         (nrhs == 1 && p.dataflow == 1 && p.block >= 192 && p.block <= 512 &&
          p.df_lds[sizeof(T) == 8] <= lds_limit(h))) {
-        // publication halves (one value per row and right-hand side), armed
-        // with this value type's sentinel; any change of layout re-arms them
+        // publication halves (one value per row and right-hand side, or a
+        // PIPE plan's box faces), armed with this value type's sentinel; any
+        // change of layout re-arms them
         const int xr = lane ? nrhs : 1;
+        const size_t half = pipe ? (size_t)p.pipe_half : (size_t)s->n * xr;
+        const int xkey = pipe ? -1 : xr;
         T *xp = reinterpret_cast<T *>(p.xp_raw.p);
         if (fixed_xp) {
             // shared by every call on this GPU and left holding the last
             // call's values: arm both halves (a fill kernel, no memset, so
             // the call can be captured in a graph)
-            const size_t cnt = 2 * (size_t)s->n * xr;
+            const size_t cnt = 2 * half;
+            if (cnt * sizeof(T) > fixed_bytes) return BAAS_TOO_LARGE;
             xp = reinterpret_cast<T *>(fixed_xp);
             hipLaunchKernelGGL(k_fill_sentinel<T>, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, stream, cnt, xp);
             BAAS_HIP_TRY(hipGetLastError());
-        } else if (p.xp_type != (int)sizeof(T) || p.xp_nrhs != xr) {
-            const size_t cnt = 2 * (size_t)s->n * xr;
+        } else if (p.xp_type != (int)sizeof(T) || p.xp_nrhs != xkey) {
+            const size_t cnt = 2 * half;
             BAAS_TRY(p.xp_raw.alloc(cnt * sizeof(T)));
             hipLaunchKernelGGL(k_fill_sentinel<T>, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, stream, cnt,
                                reinterpret_cast<T *>(p.xp_raw.p));
             BAAS_HIP_TRY(hipGetLastError());
             p.xp_type = (int)sizeof(T);
-            p.xp_nrhs = xr;
+            p.xp_nrhs = xkey;
             xp = reinterpret_cast<T *>(p.xp_raw.p);
         }
         DfArgs d;
