@@ -78,6 +78,14 @@ __device__ __forceinline__ double shr1(double old, double v) {
 __device__ __forceinline__ float shr1(float old, float v) {
     return __int_as_float(shr1_i(__float_as_int(old), __float_as_int(v)));
 }
+// The same shift where lane 0's result does not matter (it reads 0).
+__device__ __forceinline__ int shr1z_i(int v) { return __builtin_amdgcn_mov_dpp(v, 0x138, 0xf, 0xf, true); }
+__device__ __forceinline__ double shr1z(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = shr1z_i((int)(b & 0xffffffff)), hi = shr1z_i((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+__device__ __forceinline__ float shr1z(float v) { return __int_as_float(shr1z_i(__float_as_int(v))); }
 __device__ __forceinline__ double lane_read(double v, int addr) {
     const long long b = __double_as_longlong(v);
     const int lo = __builtin_amdgcn_ds_bpermute(addr, (int)(b & 0xffffffff));
@@ -216,7 +224,7 @@ struct PipeLds {
 template <class T>
 struct PipeRec {
     T cz, cy, cx, dd, bb, ez, ey, ex;
-    int rdy;
+    int rdy, q, ax, ay;  // progress word; record slot; staged x- and y-face slots read
 };
 
 // The compute wavefront.  Lane (i, j) = (lane / TY, lane % TY); its y parent
@@ -272,9 +280,10 @@ __device__ __forceinline__ void pipe_wave(const PipeArgs &a, int lane, int x0, i
     // the y face, except the corner, which is on the x face too: its y-face
     // value reaches the helper lane through the helper's x move one step
     // later, and the helper writes it (a last such store follows the final
-    // step).  The top plane
-    // is a second store, in the steps from kz - 1 on: the row of plane
-    // kz - 1 at step tz, else the dump slot.
+    // step).  The top plane is a second store, in the steps from kz - 1 on:
+    // the row of plane kz - 1 at step tz, else the dump slot.  (Sending a
+    // top-plane row with the first store where that is free measured
+    // slower.)
     const int dmp = fsz.x + fsz.y + fsz.z;
     int sA = 0, cA = 0, aA = 0, nA = 0;
     const bool cy_help = XS != 1 && lane == helper && by && bx;
@@ -292,21 +301,30 @@ __device__ __forceinline__ void pipe_wave(const PipeArgs &a, int lane, int x0, i
         st_agent(opub + (unsigned)o, v);
     };
     auto slot = [&](int k) { return lok & ((unsigned)k < (unsigned)kz) ? qb + k * txy : RMX; };
-    auto load_ext = [&](int k, PipeRec<T> &R) {
-        const int kc = min(max(k, 0), kz - 1);
+    // Staged face slots advance by xst (yst) a step.  A lane reads them at
+    // every step but uses them only at the steps it has a row (i = 0 or
+    // j = 0, 0 <= k < kz); at the others the slot is still inside the staged
+    // faces' LDS, and the value read goes to an idle computation.
+    int ax = xs0 + (-i - j) * xst, ay = ys0 + (-i - j) * yst;
+    auto load_ext = [&](PipeRec<T> &R) {
         R.ez = ev[zslot];
-        if (XS != 1) R.ey = ev[ys0 + kc * yst];
-        R.ex = ev[xs0 + kc * xst];
+        if (XS != 1) R.ey = ev[R.ay];
+        R.ex = ev[R.ax];
     };
     auto load = [&](int k, PipeRec<T> &R) {
         R.rdy = ld_ready(ready);
         const int q = slot(k);
+        R.q = q;
         R.cz = c0[q];
         if (XS != 1) R.cy = c1[q];
         R.cx = c2[q];
         R.dd = dg[q];
         R.bb = bv[q];
-        load_ext(k, R);
+        R.ax = ax;
+        R.ay = ay;
+        ax += xst;
+        ay += yst;
+        load_ext(R);
     };
     T xl = T(0);  // the value this lane solved at the previous step
     auto step = [&](int t, PipeRec<T> &C, PipeRec<T> &N) {
@@ -324,7 +342,7 @@ __device__ __forceinline__ void pipe_wave(const PipeArgs &a, int lane, int x0, i
                 }
                 r = ld_ready(ready);
             }
-            load_ext(k, C);
+            load_ext(C);
         }
         // the next step's inputs, in flight while this step computes (after
         // the wait above, so the common path never waits on these reads)
@@ -336,7 +354,7 @@ __device__ __forceinline__ void pipe_wave(const PipeArgs &a, int lane, int x0, i
         // not be sunk into the branch a select may become (the empty asm
         // pins them here).
         T vx = XS == 1 ? shr1(C.ex, xl) : lane_read(xl, xaddr);
-        T vy = XS == 1 ? T(0) : shr1(T(0), xl);
+        T vy = XS == 1 ? T(0) : shr1z(xl);
         __asm__ volatile("" : "+v"(vx), "+v"(vy));
         const T pz = k == 0 ? C.ez : xl;
         const T px = XS == 1 ? vx : (fx ? C.ex : vx);
@@ -349,8 +367,7 @@ __device__ __forceinline__ void pipe_wave(const PipeArgs &a, int lane, int x0, i
         }
         s = UPPER ? fma(px, C.cx, s) : fma(C.cx, px, s);
         xl = (C.bb - s) / C.dd;
-        const int q = slot(k);
-        xv[q] = xl;
+        xv[C.q] = xl;
         // (a lane without a row this step publishes a value no consumer
         // reads: idle lanes take parents only from idle lanes, and their
         // arithmetic never yields the signalling-NaN sentinel)
