@@ -164,6 +164,20 @@ __host__ __device__ inline MrLayout mr_layout(size_t nk, size_t E, size_t ts, si
 // diagonals d = k + j (ty values each, (kz + ty - 1) ty in all), y: the y
 // face by d = k + i ((kz + tx - 1) tx; none on a 2D grid), z: its top
 // plane (tx ty).  A box stages at most PIPE_FACE_MAX values of each face.
+// Where the entries of row (x, y, z) start in the lower triangle of a full
+// 7-point (5-point: sy < 0, a line of nx rows is a plane) stencil grid in
+// natural order: one entry per earlier row, plus one per earlier row with
+// an x parent, a y parent and a z parent.  The PIPE plan builder checks it
+// against the matrix's row pointers.
+__host__ __device__ inline int pipe_lp(int nx, int ny, int sy, int sz, int x, int y, int z) {
+    const int key = x + (sy > 0 ? sy * y : 0) + sz * z;
+    const int cz = key - sz > 0 ? key - sz : 0;
+    const int w = (y - 1) * nx + x;
+    const int cy = sy > 0 ? z * (sz - nx) + (w > 0 ? w : 0) : 0;
+    const int lines = sy > 0 ? z * ny + y : z;
+    const int cx = lines * (nx - 1) + (x > 1 ? x - 1 : 0);
+    return key + cx + cy + cz;
+}
 constexpr int PIPE_FACE_MAX = 1024;
 constexpr int PIPE_DUMP = 8;  // values after a box's faces that its idle publishing lanes write
 __host__ __device__ inline int3 pipe_face_sizes(int tx, int ty, int kz, bool has_y) {

@@ -51,7 +51,7 @@ struct PipeArgs {
     // offsets of the box's own faces, of the x face before it, of the y
     // face before it and of the top plane below it (-1: none)
     const int4 *box;
-    const int *rowptr, *colidx;
+    int nnz;  // entries of L (and of U = L^T)
     int *pcount;
     Ctx ctx;
     const int *skip;
@@ -433,54 +433,43 @@ __global__ __launch_bounds__(128) void k_sweep_pipe(PipeArgs a, const T *__restr
         const bool hx = bf.y >= 0, hy = bf.z >= 0, hz = bf.w >= 0;
         if (tid == 0) *ready = hx | hy | hz ? 0 : nsteps;
         // the box's rows: eight per thread per pass, every load of a pass in
-        // flight together (row pointers and b, then the entries); a parent
-        // the row lacks gets coefficient -0
+        // flight together.  A row's entries sit where the full stencil puts
+        // them (pipe_lp): for L the z, y and x parents in that order, then
+        // the diagonal; for U = L^T (the backward sweep, whose flipped
+        // coordinates make the same count of parents) the diagonal first.
+        // A parent the row lacks reads the diagonal and gets coefficient -0.
         constexpr int PB = 8;
         for (int q0 = tid; q0 < nrow; q0 += PB * nt) {
-            int row[PB], p0[PB], p1[PB];
-            T bb[PB];
+            bool hxr[PB], hyr[PB], hzr[PB];
+            T cz[PB], cy[PB], cx[PB], cd[PB], bb[PB];
 #pragma unroll
             for (int v = 0; v < PB; v++) {
                 const int q = min(q0 + v * nt, nrow - 1);
                 const int ii = q % tx, r2 = q / tx, jj = r2 % ty, kk = r2 / ty;
-                row[v] = pipe_row<UPPER>(a, x0 + ii, y0 + jj, z0 + kk);
-                p0[v] = a.rowptr[row[v]];
-                p1[v] = a.rowptr[row[v] + 1];
-                bb[v] = b[(size_t)row[v] * a.nrhs + a.rhs];
-            }
-            int cc[PB][4];
-            T cv[PB][4];
-#pragma unroll
-            for (int v = 0; v < PB; v++)
-#pragma unroll
-                for (int e = 0; e < 4; e++) {
-                    const int idx = min(p0[v] + e, p1[v] - 1);
-                    cc[v][e] = a.colidx[idx];
-                    cv[v][e] = vals[idx];
+                const int xg = x0 + ii, yg = y0 + jj, zg = z0 + kk;
+                const int ex = xg > 0, ey = a.sy > 0 && yg > 0, ez = zg > 0;
+                const int lp = pipe_lp(a.nx, a.ny, a.sy, a.sz, xg, yg, zg);
+                int pz, py, px, pd;
+                if (!UPPER) {
+                    pz = lp, py = pz + ez, px = py + ey, pd = px + ex;
+                } else {
+                    pd = a.nnz - lp - (1 + ex + ey + ez), px = pd + 1, py = px + ex, pz = py + ey;
                 }
+                hxr[v] = ex, hyr[v] = ey, hzr[v] = ez;
+                cz[v] = vals[ez ? pz : pd];
+                cy[v] = vals[ey ? py : pd];
+                cx[v] = vals[ex ? px : pd];
+                cd[v] = vals[pd];
+                bb[v] = b[(size_t)pipe_row<UPPER>(a, xg, yg, zg) * a.nrhs + a.rhs];
+            }
 #pragma unroll
             for (int v = 0; v < PB; v++) {
                 const int q = q0 + v * nt;
                 if (q >= nrow) continue;
-                const int cnt = p1[v] - p0[v];
-                T vz = -T(0), vy = -T(0), vx = -T(0), vd = T(1);
-#pragma unroll
-                for (int e = 0; e < 4; e++) {
-                    if (e >= cnt) continue;
-                    const int d = UPPER ? cc[v][e] - row[v] : row[v] - cc[v][e];
-                    if (d == 0)
-                        vd = cv[v][e];
-                    else if (d == a.sz)
-                        vz = cv[v][e];
-                    else if (d == a.sy)
-                        vy = cv[v][e];
-                    else if (d == 1)
-                        vx = cv[v][e];
-                }
-                c0[q] = vz;
-                c1[q] = vy;
-                c2[q] = vx;
-                dg[q] = vd;
+                c0[q] = hzr[v] ? cz[v] : -T(0);
+                c1[q] = hyr[v] ? cy[v] : -T(0);
+                c2[q] = hxr[v] ? cx[v] : -T(0);
+                dg[q] = cd[v];
                 bv[q] = bb[v];
             }
         }
@@ -565,8 +554,7 @@ int launch_pipe(baas_handle *h, baas_schedule *s, const Plan &p, int uplo, const
     a.K = p.pg[7];
     a.half = p.pipe_half;
     a.box = reinterpret_cast<const int4 *>(p.pipe_box.p);
-    a.rowptr = uplo ? s->u_rowptr.p : s->l_rowptr.p;
-    a.colidx = uplo ? s->u_colidx.p : s->l_colidx.p;
+    a.nnz = s->nnz_lower;
     a.pcount = p.pcount.p;
     a.ctx = Ctx{h->counters, h->counters + 1, h->counters + 2};
     a.skip = skip;

@@ -475,6 +475,12 @@ static int pipe_units(const baas_schedule *s, const HostCsr &L, int R, std::vect
                 return BAAS_UNSUPPORTED;
         }
     }
+    // the entries of every row start where the stencil puts them (the kernel
+    // computes these positions instead of reading the row pointers)
+    for (int r = 0; r <= n; r++) {
+        const int x = r % g.nx, y = (r / g.nx) % g.ny, z = r / g.sz;
+        if (L.ptr[r] != pipe_lp(g.nx, g.ny, g.sy, g.sz, x, y, z)) return BAAS_UNSUPPORTED;
+    }
     g.TX = three ? std::min(8, g.nx) : std::min(64, g.nx);
     g.TY = three ? std::min(64 / g.TX, g.ny) : 1;
     g.K = std::max(1, std::min({R / (g.TX * g.TY), g.nz, 255, 2048 / (g.TX * g.TY)}));
