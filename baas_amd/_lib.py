@@ -58,6 +58,7 @@ _SIGS = {
     "baas_replan": (_i, [_P, _i, _i, _i]),
     "baas_replan_tiles": (_i, [_P, _i, _i, _i]),
     "baas_schedule_plan": (_i, [_P, _P]),
+    "baas_schedule_pipe": (_i, [_P, _P]),
     "baas_plan_dataflow": (_i, [_P, _i]),
     "baas_autotune": (_i, [_P, _P, _P, _i, _i, _P]),
     "baas_autotune_f32": (_i, [_P, _P, _P, _i, _i, _P]),

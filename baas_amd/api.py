@@ -187,6 +187,12 @@ class Schedule:
         L.call("baas_schedule_plan", self._s, C.cast(c, C.c_void_p))
         return {"mode": c[0], "rows_per_unit": c[1], "block": c[2], "dataflow": c[3], "band": c[4]}
 
+    def pipe_info(self):
+        """(the forward plan is a PIPE plan, IC0/ILU0 run pipelined on it)."""
+        c = (C.c_int * 2)()
+        L.call("baas_schedule_pipe", self._s, C.cast(c, C.c_void_p))
+        return bool(c[0]), bool(c[1])
+
     def apply_plan(self, plan):
         """Re-pack into a plan returned by plan() or autotune() (for example one
         saved from an earlier run, so a new process skips the search)."""

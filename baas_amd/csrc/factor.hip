@@ -21,6 +21,7 @@
 
 namespace baas {
 int build_ic0_plans(baas_schedule *s);
+int launch_pipe_factor(baas_handle *h, baas_schedule *s, Plan &p, int kind, const double *a_vals, double *lu);
 int build_factor_lane(baas_schedule *s);
 size_t lds_limit(baas_handle *h);
 
@@ -691,6 +692,14 @@ static int factor(baas_handle *h, baas_schedule *s, const double *a_vals, double
     if (s->kind != BAAS_KIND_IC0 && s->kind != BAAS_KIND_ILU0) return BAAS_INVALID_ARG;
     if (a_vals != lu)
         hipLaunchKernelGGL(k_copy_f64, dim3(nblk(s->nnz_full)), dim3(256), 0, h->stream, s->nnz_full, a_vals, lu);
+    // PIPE plans over a full stencil: the pipelined factorisation (reads A
+    // from a_vals when lu is a copy, else from lu before overwriting it row
+    // by row -- each row's A entries are read when its box starts, and only
+    // that box writes them)
+    {
+        const int rc = launch_pipe_factor(h, s, s->fwd, KIND, a_vals, lu);
+        if (rc != BAAS_UNSUPPORTED) return rc;
+    }
     const Plan &p = s->fwd;
     FactorArgs a;
     a.n = s->n;

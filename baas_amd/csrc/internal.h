@@ -122,6 +122,7 @@ struct Plan {
     int pg[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     DBuf<int> pipe_box;  // [8 nunits]
     long long pipe_half = 0;
+    int pipe_fact = 0;  // A (IC0/ILU0 schedules) is the full stencil pipe_pa describes
     std::vector<int> h_unit_ptr;  // host copy: unit boundaries in rows
     std::vector<int> h_rows;      // host copy of rows (without flag bit)
 };
@@ -177,6 +178,20 @@ __host__ __device__ inline int pipe_lp(int nx, int ny, int sy, int sz, int x, in
     const int lines = sy > 0 ? z * ny + y : z;
     const int cx = lines * (nx - 1) + (x > 1 ? x - 1 : 0);
     return key + cx + cy + cz;
+}
+// The same for the full matrix A = L + L^T pattern (IC0 and ILU0 schedules):
+// also one entry per earlier row with an x, a y and a z child.
+__host__ __device__ inline int pipe_pa(int nx, int ny, int nz, int sy, int sz, int x, int y, int z) {
+    const int key = x + (sy > 0 ? sy * y : 0) + sz * z;
+    const int lines = sy > 0 ? z * ny + y : z;
+    int up = lines * (nx - 1) + x;
+    if (sy > 0) {
+        const int w = y * nx + x, m = sz - nx;
+        up += z * m + (w < m ? w : m);
+    }
+    const int nm = sz * (nz - 1);
+    up += key < nm ? key : nm;
+    return pipe_lp(nx, ny, sy, sz, x, y, z) + up;
 }
 constexpr int PIPE_FACE_MAX = 1024;
 constexpr int PIPE_DUMP = 8;  // values after a box's faces that its idle publishing lanes write

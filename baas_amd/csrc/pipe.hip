@@ -42,6 +42,8 @@
 
 namespace baas {
 
+size_t lds_limit(baas_handle *h);
+
 struct PipeArgs {
     int n, nunits, nrhs, rhs;
     int nx, ny, nz, sy, sz;  // grid; sy < 0: a 2D grid (no y direction)
@@ -199,21 +201,29 @@ __device__ __forceinline__ int ld_ready(const int *ready) {
     return r;
 }
 
+// What the pipelined sweep computes: a triangular solve, or the stencil
+// IC0 or ILU0 factorisation (the published value is then the row's
+// diagonal of L, or of U).
+enum { PIPE_SOLVE = 0, PIPE_IC0 = 1, PIPE_ILU0 = 2 };
+
 // LDS of one box of at most RMX rows (compile-time, so every record read is
-// one address register plus an immediate offset): per row the three parent
-// coefficients (z, y, x), the diagonal, b and the solution, each in an
-// array of RMX + 1 values (index RMX is the spare row idle lanes read and
-// write); then the staged faces (EX: the x face, EY: the y face, each in
-// the order of its publication, EZ: the plane below), a zero slot and a
-// dump slot; then the progress word.
-template <class T, int RMX>
+// one address register plus an immediate offset), in arrays of RMX + 1
+// values (index RMX is the spare row idle lanes read and write): per row
+// the three parent coefficients (z, y, x) and the diagonal; for a solve b
+// and the solution; for ILU0 the parents' entries in this row's column
+// (z, y, x).  A factorisation writes its results over the coefficients and
+// the diagonal.  Then the staged faces (EX: the x face, EY: the y face,
+// each in the order of its publication, EZ: the plane below), the slot
+// parents outside the grid read (1) and a dump slot; then the progress word.
+template <class T, int RMX, int MODE = PIPE_SOLVE>
 struct PipeLds {
     static constexpr int NS = RMX + 1;
-    static constexpr int EX = 0, EY = PIPE_FACE_MAX, EZ = 2 * PIPE_FACE_MAX, ZERO = EZ + 64, DUMP = ZERO + 1,
+    static constexpr int NA = MODE == PIPE_SOLVE ? 6 : (MODE == PIPE_IC0 ? 4 : 7);
+    static constexpr int EX = 0, EY = PIPE_FACE_MAX, EZ = 2 * PIPE_FACE_MAX, NONE = EZ + 64, DUMP = NONE + 1,
                          NE = DUMP + 1;
-    static constexpr size_t C0 = 0, C1 = C0 + NS * sizeof(T), C2 = C1 + NS * sizeof(T), DG = C2 + NS * sizeof(T),
-                            BV = DG + NS * sizeof(T), XV = BV + NS * sizeof(T), EV = XV + NS * sizeof(T),
-                            RDY = EV + NE * sizeof(T), TOTAL = RDY + 16;
+    static constexpr size_t A = NS * sizeof(T);
+    static constexpr size_t C0 = 0, C1 = A, C2 = 2 * A, DG = 3 * A, BV = 4 * A, XV = 5 * A, U0 = 4 * A, U1 = 5 * A,
+                            U2 = 6 * A, EV = NA * A, RDY = EV + NE * sizeof(T), TOTAL = RDY + 16;
 };
 
 // One step's inputs of a lane, read a step ahead so that their LDS reads
@@ -223,7 +233,7 @@ struct PipeLds {
 // before them is past the step).
 template <class T>
 struct PipeRec {
-    T cz, cy, cx, dd, bb, ez, ey, ex;
+    T cz, cy, cx, dd, bb, uz, uy, ux, ez, ey, ex;
     int rdy, q, ax, ay;  // progress word; record slot; staged x- and y-face slots read
 };
 
@@ -232,23 +242,28 @@ struct PipeRec {
 // 0 then takes the x-face value as the shift's `old`; the LDS crossbar
 // otherwise).  A lane without a row this step (before its first plane,
 // after its last, outside a partial box) reads and writes the spare row,
-// whose coefficients are -0, diagonal 1 and b +0.  hx, hy, hz: the box has
+// and what it computes is read by no lane with a row.  A parent outside
+// the grid has coefficient -0 and value 1 (the NONE slot), which adds
+// nothing to a sum and divides nothing by zero.  hx, hy, hz: the box has
 // staged x, y and z faces; opub: its own faces in the publication half.
 // The x-face lanes (i = tx - 1) write diagonal d = t - (tx - 1) of the x
-// face at step t, all of it, with +0 for lanes that have no row this step;
-// the y face likewise; the top plane row by row.  A face value's offset is
-// linear in t, and whether a step writes a diagonal depends on t alone.
-template <class T, bool UPPER, int XS, int RMX>
+// face at step t, all of it; the y face likewise; the top plane row by
+// row.  A face value's offset is linear in t, and whether a step writes a
+// diagonal depends on t alone.
+template <class T, bool UPPER, int XS, int RMX, int MODE = PIPE_SOLVE>
 __device__ __forceinline__ void pipe_wave(const PipeArgs &a, int lane, int x0, int y0, int z0, int tx, int ty,
                                           int kz, bool hx, bool hy, bool hz, int3 fsz, unsigned char *smem,
                                           const int *ready, T *opub) {
-    using Lay = PipeLds<T, RMX>;
+    using Lay = PipeLds<T, RMX, MODE>;
     const T *c0 = reinterpret_cast<const T *>(smem + Lay::C0);
     const T *c1 = reinterpret_cast<const T *>(smem + Lay::C1);
     const T *c2 = reinterpret_cast<const T *>(smem + Lay::C2);
     const T *dg = reinterpret_cast<const T *>(smem + Lay::DG);
     const T *bv = reinterpret_cast<const T *>(smem + Lay::BV);
     T *xv = reinterpret_cast<T *>(smem + Lay::XV);
+    const T *u0 = reinterpret_cast<const T *>(smem + Lay::U0);
+    const T *u1 = reinterpret_cast<const T *>(smem + Lay::U1);
+    const T *u2 = reinterpret_cast<const T *>(smem + Lay::U2);
     const T *ev = reinterpret_cast<const T *>(smem + Lay::EV);
     const int TY = a.TY;
     const int i = lane / TY, j = lane - (lane / TY) * TY;
@@ -266,12 +281,12 @@ __device__ __forceinline__ void pipe_wave(const PipeArgs &a, int lane, int x0, i
     const int helper = tx == 1 || ty == 1 ? 63 : 0;
     const int xaddr = (XS != 1 && lane == helper ? corner : (lane + 64 - TY) & 63) << 2;
     // staged slots: the plane below, and per plane the x face (lanes with
-    // i = 0) or the y face (lanes with j = 0); the zero slot where a box
-    // has no such face (a parent that does not exist has value +0)
+    // i = 0) or the y face (lanes with j = 0); the NONE slot where a box has
+    // no such face (a parent outside the grid)
     const bool fx = lok && i == 0, fy = lok && j == 0;
-    const int zslot = lok && hz ? Lay::EZ + i + tx * j : Lay::ZERO;
-    const int xs0 = fx && hx ? Lay::EX + j * (ty + 1) : Lay::ZERO, xst = fx && hx ? ty : 0;
-    const int ys0 = fy && hy ? Lay::EY + i * (tx + 1) : Lay::ZERO, yst = fy && hy ? tx : 0;
+    const int zslot = lok && hz ? Lay::EZ + i + tx * j : Lay::NONE;
+    const int xs0 = fx && hx ? Lay::EX + j * (ty + 1) : Lay::NONE, xst = fx && hx ? ty : 0;
+    const int ys0 = fy && hy ? Lay::EY + i * (tx + 1) : Lay::NONE, yst = fy && hy ? tx : 0;
     const int qb = j * tx + i;
     // Publication, one store of every lane per step (lanes with nothing to
     // publish write the box's dump slot): at step t a lane writes slot
@@ -319,7 +334,12 @@ __device__ __forceinline__ void pipe_wave(const PipeArgs &a, int lane, int x0, i
         if (XS != 1) R.cy = c1[q];
         R.cx = c2[q];
         R.dd = dg[q];
-        R.bb = bv[q];
+        if (MODE == PIPE_SOLVE) R.bb = bv[q];
+        if (MODE == PIPE_ILU0) {
+            R.uz = u0[q];
+            if (XS != 1) R.uy = u1[q];
+            R.ux = u2[q];
+        }
         R.ax = ax;
         R.ay = ay;
         ax += xst;
@@ -358,16 +378,50 @@ __device__ __forceinline__ void pipe_wave(const PipeArgs &a, int lane, int x0, i
         __asm__ volatile("" : "+v"(vx), "+v"(vy));
         const T pz = k == 0 ? C.ez : xl;
         const T px = XS == 1 ? vx : (fx ? C.ex : vx);
-        // z, y, x: ascending columns for L, descending for U (a 2D grid,
-        // XS = 1, has no y term)
-        T s = UPPER ? fma(pz, C.cz, T(0)) : fma(C.cz, pz, T(0));
-        if (XS != 1) {
-            const T py = fy ? C.ey : vy;
-            s = UPPER ? fma(py, C.cy, s) : fma(C.cy, py, s);
+        const T py = XS == 1 ? T(0) : (fy ? C.ey : vy);
+        if (MODE == PIPE_SOLVE) {
+            // z, y, x: ascending columns for L, descending for U (a 2D grid,
+            // XS = 1, has no y term)
+            T s = UPPER ? fma(pz, C.cz, T(0)) : fma(C.cz, pz, T(0));
+            if (XS != 1) s = UPPER ? fma(py, C.cy, s) : fma(C.cy, py, s);
+            s = UPPER ? fma(px, C.cx, s) : fma(C.cx, px, s);
+            xl = (C.bb - s) / C.dd;
+            xv[C.q] = xl;
+        } else {
+            // the row's entries over its parents' diagonals (the parents'
+            // rows share no earlier column with it on a 7- or 5-point stencil,
+            // so the reference's dot products are zero: A_ik - 0 = A_ik)
+            const T lz = C.cz / pz, ly = XS == 1 ? T(0) : C.cy / py, lx = C.cx / px;
+            T d;
+            if (MODE == PIPE_IC0) {
+                // L_ii = sqrt(A_ii - sum L_ik^2) over the row's entries in
+                // ascending order; a non-positive pivot keeps A_ii
+                T dv = fma(lz, lz, T(0));
+                if (XS != 1) dv = fma(ly, ly, dv);
+                dv = fma(lx, lx, dv);
+                const T e = C.dd - dv;
+                d = e > T(0) ? sqrt(e) : C.dd;
+                if (!(e > T(0)) && C.q != RMX)
+                    raise_status(a.status, BAAS_NONPOSITIVE_PIVOT, pipe_row<false>(a, x0 + i, y0 + j, z0 + k));
+            } else {
+                // U_ii = A_ii - sum L_ik U_ki, the parents' U_ki being A_ki
+                d = fma(-lz, C.uz, C.dd);
+                if (XS != 1) d = fma(-ly, C.uy, d);
+                d = fma(-lx, C.ux, d);
+                // a pivot the reference asserts against (parents outside the
+                // grid divide by 1)
+                const bool small = !(fabs(pz) > 1.0e-16) | !(fabs(px) > 1.0e-16) | (XS != 1 && !(fabs(py) > 1.0e-16));
+                if (small && C.q != RMX)
+                    raise_status(a.status, BAAS_ZERO_PIVOT, pipe_row<false>(a, x0 + i, y0 + j, z0 + k));
+            }
+            T *w0 = reinterpret_cast<T *>(smem + Lay::C0), *w1 = reinterpret_cast<T *>(smem + Lay::C1);
+            T *w2 = reinterpret_cast<T *>(smem + Lay::C2), *wd = reinterpret_cast<T *>(smem + Lay::DG);
+            w0[C.q] = lz;
+            if (XS != 1) w1[C.q] = ly;
+            w2[C.q] = lx;
+            wd[C.q] = d;
+            xl = d;
         }
-        s = UPPER ? fma(px, C.cx, s) : fma(C.cx, px, s);
-        xl = (C.bb - s) / C.dd;
-        xv[C.q] = xl;
         // (a lane without a row this step publishes a value no consumer
         // reads: idle lanes take parents only from idle lanes, and their
         // arithmetic never yields the signalling-NaN sentinel)
@@ -412,12 +466,12 @@ __global__ __launch_bounds__(128) void k_sweep_pipe(PipeArgs a, const T *__restr
     T *ev = reinterpret_cast<T *>(smem_raw + Lay::EV);
     int *ready = reinterpret_cast<int *>(smem_raw + Lay::RDY);
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nt = blockDim.x;
-    // the spare row (idle lanes compute +0 on it) and the zero slot
+    // the spare row idle lanes read and the slot of parents outside the grid
     if (tid == 0) {
         c0[RMX] = c1[RMX] = c2[RMX] = -T(0);
         dg[RMX] = T(1);
         bv[RMX] = T(0);
-        ev[Lay::ZERO] = T(0);
+        ev[Lay::NONE] = T(1);
     }
     for (;;) {
         const int u = __builtin_amdgcn_readfirstlane(claim_unit(a.ctx.ticket, &s_unit));
@@ -501,6 +555,157 @@ __global__ __launch_bounds__(128) void k_sweep_pipe(PipeArgs a, const T *__restr
     }
 }
 
+// Where the entries of row (x, y, z) of the full stencil matrix A sit for
+// the factorisations: its parents' columns (z, y, x; pos of the diagonal
+// when absent) and its diagonal, and (ILU0) its parents' entries in this
+// row's column -- the last entry of the z parent's row, the entry after
+// the y parent's diagonal and its x child, the entry after the x parent's
+// diagonal.
+struct PipeApos {
+    int z, y, x, d, uz, uy, ux;
+    bool ez, ey, ex;
+};
+__device__ __forceinline__ PipeApos pipe_apos(const PipeArgs &a, int xg, int yg, int zg) {
+    PipeApos o;
+    const bool three = a.sy > 0;
+    o.ex = xg > 0, o.ey = three && yg > 0, o.ez = zg > 0;
+    const int pa = pipe_pa(a.nx, a.ny, a.nz, a.sy, a.sz, xg, yg, zg);
+    o.z = pa;
+    o.y = o.z + o.ez;
+    o.x = o.y + o.ey;
+    o.d = o.x + o.ex;
+    const int hasx1 = xg < a.nx - 1, hasy1 = three && yg < a.ny - 1;
+    // z parent (xg, yg, zg - 1): its row ends with column r
+    o.uz = o.d;
+    if (o.ez) {
+        const int low = (xg > 0) + (three && yg > 0) + (zg - 1 > 0);
+        o.uz = pipe_pa(a.nx, a.ny, a.nz, a.sy, a.sz, xg, yg, zg - 1) + low + hasx1 + hasy1 + 1;
+    }
+    o.uy = o.d;
+    if (o.ey) {
+        const int low = (xg > 0) + (yg - 1 > 0) + (zg > 0);
+        o.uy = pipe_pa(a.nx, a.ny, a.nz, a.sy, a.sz, xg, yg - 1, zg) + low + 1 + hasx1;
+    }
+    o.ux = o.d;
+    if (o.ex) {
+        const int low = (xg - 1 > 0) + (three && yg > 0) + (zg > 0);
+        o.ux = pipe_pa(a.nx, a.ny, a.nz, a.sy, a.sz, xg - 1, yg, zg) + low + 1;
+    }
+    return o;
+}
+
+// IC0 (MODE PIPE_IC0) or ILU0 (PIPE_ILU0) of a full 7- or 5-point stencil
+// matrix in natural order on PIPE plans (merge/IC.cpp:32-89 up-looking IC0,
+// merge/ILU.cpp:350-386 up-looking ILU0), in place of lu (which holds A's
+// values): the box machinery of k_sweep_pipe, with each row's value the
+// diagonal its children divide by.
+template <int MODE, int XS, int RMX>
+__global__ __launch_bounds__(128) void k_factor_pipe(PipeArgs a, const double *__restrict__ av, double *lu,
+                                                     double *xp) {
+    using T = double;
+    using Lay = PipeLds<T, RMX, MODE>;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ int s_unit;
+    const int epoch = read_epoch(a.ctx);
+    const int par = __builtin_amdgcn_readfirstlane(ld_agent(a.pcount)) & 1;
+    T *pub = xp + (size_t)par * a.half;
+    T *nxt = xp + (size_t)(par ^ 1) * a.half;
+    T *c0 = reinterpret_cast<T *>(smem_raw + Lay::C0);
+    T *c1 = reinterpret_cast<T *>(smem_raw + Lay::C1);
+    T *c2 = reinterpret_cast<T *>(smem_raw + Lay::C2);
+    T *dg = reinterpret_cast<T *>(smem_raw + Lay::DG);
+    T *u0 = reinterpret_cast<T *>(smem_raw + Lay::U0);
+    T *u1 = reinterpret_cast<T *>(smem_raw + Lay::U1);
+    T *u2 = reinterpret_cast<T *>(smem_raw + Lay::U2);
+    T *ev = reinterpret_cast<T *>(smem_raw + Lay::EV);
+    int *ready = reinterpret_cast<int *>(smem_raw + Lay::RDY);
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6, nt = blockDim.x;
+    if (tid == 0) {
+        c0[RMX] = c1[RMX] = c2[RMX] = -T(0);
+        dg[RMX] = T(1);
+        if (MODE == PIPE_ILU0) u0[RMX] = u1[RMX] = u2[RMX] = T(0);
+        ev[Lay::NONE] = T(1);
+    }
+    for (;;) {
+        const int u = __builtin_amdgcn_readfirstlane(claim_unit(a.ctx.ticket, &s_unit));
+        if (u >= a.nunits) break;
+        const int4 bx = a.box[2 * u], bf = a.box[2 * u + 1];
+        const int x0 = bx.x, y0 = bx.y, z0 = bx.z;
+        const int tx = bx.w & 255, ty = (bx.w >> 8) & 255, kz = (bx.w >> 16) & 255;
+        const int nrow = tx * ty * kz, nsteps = kz + tx + ty - 2;
+        const int3 fsz = pipe_face_sizes(tx, ty, kz, a.sy > 0);
+        const bool hx = bf.y >= 0, hy = bf.z >= 0, hz = bf.w >= 0;
+        if (tid == 0) *ready = hx | hy | hz ? 0 : nsteps;
+        // the box's rows of A, eight per thread per pass with every load of a
+        // pass in flight together; a parent the row lacks gets -0 (and, for
+        // ILU0, an entry of 0 in this row's column)
+        constexpr int PB = 8;
+        for (int q0 = tid; q0 < nrow; q0 += PB * nt) {
+            bool hxr[PB], hyr[PB], hzr[PB];
+            T cz[PB], cy[PB], cx[PB], cd[PB], vz[PB], vy[PB], vx[PB];
+#pragma unroll
+            for (int v = 0; v < PB; v++) {
+                const int q = min(q0 + v * nt, nrow - 1);
+                const int ii = q % tx, r2 = q / tx, jj = r2 % ty, kk = r2 / ty;
+                const PipeApos o = pipe_apos(a, x0 + ii, y0 + jj, z0 + kk);
+                hxr[v] = o.ex, hyr[v] = o.ey, hzr[v] = o.ez;
+                cz[v] = av[o.z];
+                cy[v] = av[o.y];
+                cx[v] = av[o.x];
+                cd[v] = av[o.d];
+                if (MODE == PIPE_ILU0) {
+                    vz[v] = av[o.uz];
+                    vy[v] = av[o.uy];
+                    vx[v] = av[o.ux];
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < PB; v++) {
+                const int q = q0 + v * nt;
+                if (q >= nrow) continue;
+                c0[q] = hzr[v] ? cz[v] : -T(0);
+                c1[q] = hyr[v] ? cy[v] : -T(0);
+                c2[q] = hxr[v] ? cx[v] : -T(0);
+                dg[q] = cd[v];
+                if (MODE == PIPE_ILU0) {
+                    u0[q] = hzr[v] ? vz[v] : T(0);
+                    u1[q] = hyr[v] ? vy[v] : T(0);
+                    u2[q] = hxr[v] ? vx[v] : T(0);
+                }
+            }
+        }
+        __syncthreads();
+        if (wv == 0)
+            pipe_stage<T>(hx ? pub + bf.y : nullptr, fsz.x, ty, hy ? pub + bf.z : nullptr, fsz.y, tx,
+                          hz ? pub + bf.w : nullptr, fsz.z, ev + Lay::EX, ev + Lay::EY, ev + Lay::EZ, ev + Lay::DUMP,
+                          pub, nsteps, ready, lane, a.spin_limit, a.status, (long long)(nxt - pub));
+        else if (wv == 1)
+            pipe_wave<T, false, XS, RMX, MODE>(a, lane, x0, y0, z0, tx, ty, kz, hx, hy, hz, fsz, smem_raw, ready,
+                                               pub + bf.x);
+        __syncthreads();
+        // the box's L (IC0) or L multipliers and U diagonal (ILU0) into lu
+        for (int q = tid; q < nrow; q += nt) {
+            const int ii = q % tx, r2 = q / tx, jj = r2 % ty, kk = r2 / ty;
+            const PipeApos o = pipe_apos(a, x0 + ii, y0 + jj, z0 + kk);
+            if (o.ez) lu[o.z] = c0[q];
+            if (o.ey) lu[o.y] = c1[q];
+            if (o.ex) lu[o.x] = c2[q];
+            lu[o.d] = dg[q];
+        }
+    }
+    if (threadIdx.x == 0) {
+        __threadfence();
+        const int d = atomicAdd(a.ctx.done, 1);
+        if (d == (int)gridDim.x - 1) {
+            atomicExch(a.ctx.ticket, 0);
+            atomicExch(a.ctx.epoch, epoch == 0x7fffffff ? 1 : epoch + 1);
+            atomicAdd(a.pcount, 1);
+            __threadfence();
+            atomicExch(a.ctx.done, 0);
+        }
+    }
+}
+
 // Rows per box the kernel instances hold (the plan keeps TX TY K within).
 static int pipe_rmx(const Plan &p) { return p.pg[5] * p.pg[6] * p.pg[7] <= 1024 ? 1024 : 2048; }
 
@@ -565,6 +770,82 @@ int launch_pipe(baas_handle *h, baas_schedule *s, const Plan &p, int uplo, const
     if (a.TX * a.TY * a.K > 2048 || !pipe_faces_fit(a.TX, a.TY, a.K, a.sy > 0) || a.half < 1) return BAAS_UNSUPPORTED;
     return pipe_rmx(p) == 1024 ? launch_pipe_r<T, 1024>(h, a, uplo, vals, b, x, xp, p.nunits, nrhs, stream)
                                : launch_pipe_r<T, 2048>(h, a, uplo, vals, b, x, xp, p.nunits, nrhs, stream);
+}
+
+__global__ void k_pipe_arm(size_t n, double *p) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = Sentinel<double>::value();
+}
+
+// LDS bytes of the factorisation on plan p.
+static size_t pipe_factor_lds(const Plan &p, int kind) {
+    if (p.pg[5] * p.pg[6] * p.pg[7] > 2048) return (size_t)1 << 40;
+    const bool r1 = pipe_rmx(p) == 1024;
+    if (kind == BAAS_KIND_IC0) return r1 ? PipeLds<double, 1024, PIPE_IC0>::TOTAL : PipeLds<double, 2048, PIPE_IC0>::TOTAL;
+    return r1 ? PipeLds<double, 1024, PIPE_ILU0>::TOTAL : PipeLds<double, 2048, PIPE_ILU0>::TOTAL;
+}
+
+template <int MODE, int RMX>
+static int launch_factor_r(baas_handle *h, const PipeArgs &a, const double *av, double *lu, double *xp, int nunits) {
+    using K = void (*)(PipeArgs, const double *, double *, double *);
+    K kern = a.TY == 1 ? k_factor_pipe<MODE, 1, RMX> : k_factor_pipe<MODE, 0, RMX>;
+    const size_t lds = PipeLds<double, RMX, MODE>::TOTAL;
+    int per_cu = 0;
+    BAAS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 128, lds));
+    if (per_cu <= 0) return BAAS_UNSUPPORTED;
+    long long grid = std::min<long long>((long long)per_cu * h->num_cus, nunits);
+    if (grid < 1) grid = 1;
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(128), lds, h->stream, a, av, lu, xp);
+    BAAS_HIP_TRY(hipGetLastError());
+    return BAAS_OK;
+}
+
+// IC0 / ILU0 on the forward plan when it is a PIPE plan over a full stencil
+// (p.pipe_fact); BAAS_UNSUPPORTED otherwise.  lu holds A's values.
+int launch_pipe_factor(baas_handle *h, baas_schedule *s, Plan &p, int kind, const double *a_vals, double *lu) {
+    if (!p.pipe || !p.pipe_fact || p.dataflow != 5) return BAAS_UNSUPPORTED;
+    if (pipe_factor_lds(p, kind) > lds_limit(h)) return BAAS_UNSUPPORTED;
+    if (p.pg[5] * p.pg[6] * p.pg[7] > 2048 || !pipe_faces_fit(p.pg[5], p.pg[6], p.pg[7], p.pg[3] > 0) ||
+        p.pipe_half < 1)
+        return BAAS_UNSUPPORTED;
+    // the publication halves, armed (shared with FP64 solves on this plan)
+    if (p.xp_type != (int)sizeof(double) || p.xp_nrhs != -1) {
+        const size_t cnt = 2 * (size_t)p.pipe_half;
+        BAAS_TRY(p.xp_raw.alloc(cnt * sizeof(double)));
+        hipLaunchKernelGGL(k_pipe_arm, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, h->stream, cnt,
+                           reinterpret_cast<double *>(p.xp_raw.p));
+        BAAS_HIP_TRY(hipGetLastError());
+        p.xp_type = (int)sizeof(double);
+        p.xp_nrhs = -1;
+    }
+    PipeArgs a;
+    a.n = s->n;
+    a.nunits = p.nunits;
+    a.nrhs = 1;
+    a.rhs = 0;
+    a.nx = p.pg[0];
+    a.ny = p.pg[1];
+    a.nz = p.pg[2];
+    a.sy = p.pg[3];
+    a.sz = p.pg[4];
+    a.TX = p.pg[5];
+    a.TY = p.pg[6];
+    a.K = p.pg[7];
+    a.half = p.pipe_half;
+    a.box = reinterpret_cast<const int4 *>(p.pipe_box.p);
+    a.nnz = s->nnz_lower;
+    a.pcount = p.pcount.p;
+    a.ctx = Ctx{h->counters, h->counters + 1, h->counters + 2};
+    a.skip = nullptr;
+    a.status = h->status;
+    a.spin_limit = h->spin_limit;
+    double *xp = reinterpret_cast<double *>(p.xp_raw.p);
+    const bool r1 = pipe_rmx(p) == 1024;
+    if (kind == BAAS_KIND_IC0)
+        return r1 ? launch_factor_r<PIPE_IC0, 1024>(h, a, a_vals, lu, xp, p.nunits)
+                  : launch_factor_r<PIPE_IC0, 2048>(h, a, a_vals, lu, xp, p.nunits);
+    return r1 ? launch_factor_r<PIPE_ILU0, 1024>(h, a, a_vals, lu, xp, p.nunits)
+              : launch_factor_r<PIPE_ILU0, 2048>(h, a, a_vals, lu, xp, p.nunits);
 }
 
 template int launch_pipe<double>(baas_handle *, baas_schedule *, const Plan &, int, const double *, const double *,

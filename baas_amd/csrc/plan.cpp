@@ -197,6 +197,7 @@ int make_plan(Plan &p, int n, const std::vector<int> &order, const std::vector<i
     p.mr_lds_key = -1;
     p.pipe = 0;  // a PIPE plan's boxes are set by its builder after this
     p.pipe_half = 0;
+    p.pipe_fact = 0;
     BAAS_TRY(upload(p.scol, scol, st));
     BAAS_TRY(upload(p.ext_ptr, ext_ptr, st));
     BAAS_TRY(upload(p.ext_cols, ext_cols, st));
@@ -434,6 +435,31 @@ static int box_units(const baas_schedule *s, const HostCsr &L, int R, std::vecto
 struct PipeGeom {
     int nx = 0, ny = 1, nz = 1, sy = -1, sz = 0, TX = 1, TY = 1, K = 1;
 };
+// A's rows are the grid's full stencil: row r holds r - sz, r - sy, r - 1,
+// r, r + 1, r + sy, r + sz (those inside the grid, ascending) from
+// position pipe_pa.
+static bool pipe_stencil_rows(const PipeGeom &g, const HostCsr &A) {
+    const int n = (int)A.ptr.size() - 1;
+    const bool three = g.sy > 0;
+    int want[7];
+    for (int r = 0; r <= n; r++) {
+        const int x = r % g.nx, y = three ? (r / g.nx) % g.ny : 0, z = r / g.sz;
+        if (A.ptr[r] != pipe_pa(g.nx, g.ny, g.nz, g.sy, g.sz, x, y, z)) return false;
+        if (r == n) break;
+        int c = 0;
+        if (z > 0) want[c++] = r - g.sz;
+        if (three && y > 0) want[c++] = r - g.sy;
+        if (x > 0) want[c++] = r - 1;
+        want[c++] = r;
+        if (x < g.nx - 1) want[c++] = r + 1;
+        if (three && y < g.ny - 1) want[c++] = r + g.sy;
+        if (z < g.nz - 1) want[c++] = r + g.sz;
+        if (A.ptr[r + 1] - A.ptr[r] != c) return false;
+        for (int e = 0; e < c; e++)
+            if (A.idx[A.ptr[r] + e] != want[e]) return false;
+    }
+    return true;
+}
 static int pipe_units(const baas_schedule *s, const HostCsr &L, int R, std::vector<int> &order, std::vector<int> &fu,
                       PipeGeom &g, std::vector<int> &fbox, std::vector<int> &bbox, long long &half) {
     const int n = s->n;
@@ -717,9 +743,18 @@ int build_plans_mode(baas_schedule *s, int mode, int rows_per_unit) {
         BAAS_TRY(upload(s->fwd.pipe_box, pbox, st));
         BAAS_TRY(upload(s->bwd.pipe_box, bbox, st));
         if (hipStreamSynchronize(st) != hipSuccess) return BAAS_HIP_ERROR;
+        // the factorisations take the pipelined path when A's rows hold
+        // exactly the stencil's entries where pipe_pa puts them
+        int fact = 0;
+        if (s->kind != BAAS_KIND_TRSV) {
+            HostCsr A;
+            BAAS_TRY(fetch(A, s->a_rowptr.p, s->a_colidx.p, n, st));
+            fact = pipe_stencil_rows(pg, A) ? 1 : 0;
+        }
         for (Plan *p : {&s->fwd, &s->bwd}) {
             std::copy(g8, g8 + 8, p->pg);
             p->pipe_half = phalf;
+            p->pipe_fact = fact;
             p->pipe = 1;
         }
     }
@@ -1040,6 +1075,13 @@ extern "C" int baas_replan_tiles(baas_schedule *s, int row_range, int band_level
     const int rc = baas_replan(s, BAAS_PLAN_TILES, row_range, block_threads);
     if (rc != BAAS_OK) s->tile_band = old;
     return rc;
+}
+
+extern "C" int baas_schedule_pipe(const baas_schedule *s, int *info) {
+    if (!s || !info) return BAAS_INVALID_ARG;
+    info[0] = s->fwd.pipe;
+    info[1] = s->fwd.pipe && s->fwd.pipe_fact;
+    return BAAS_OK;
 }
 
 extern "C" int baas_schedule_plan(const baas_schedule *s, int *plan) {

@@ -227,6 +227,9 @@ static int autotune_factor(baas_handle *h, baas_schedule *s, const double *a_val
             for (int blk : {512, 1024}) cands.push_back({BAAS_PLAN_SCHEDULE, rpu, blk, 2, 0});
         for (int R : {512, 576, 900, 1000}) cands.push_back({BAAS_PLAN_BOXES, R, 1024, 2, 0});
     }
+    // the pipelined factorisation on full stencils (other patterns fall
+    // back to the lane kernel on the same plan)
+    for (int R : {512, 1024, 2048}) cands.push_back({BAAS_PLAN_PIPE, R, 128, 5, 0});
     hipEvent_t e0, e1;
     BAAS_HIP_TRY(hipEventCreate(&e0));
     BAAS_HIP_TRY(hipEventCreate(&e1));

@@ -281,6 +281,11 @@ int baas_replan_tiles(baas_schedule *s, int row_range, int band_levels, int bloc
 /* The plan in use (host int[5]): mode, rows per unit (TILES: row range),
  * threads per workgroup, dataflow, band levels (TILES, else 0). */
 int baas_schedule_plan(const baas_schedule *s, int *plan);
+/* Where the PIPE plan in use applies (host int[2]): [0] = 1 when the
+ * forward plan is a PIPE plan (stencil boxes), [1] = 1 when IC0 / ILU0 of
+ * this schedule run pipelined on it (A is the full stencil).  No reference
+ * counterpart: a diagnostic of this library's plan kinds. */
+int baas_schedule_pipe(const baas_schedule *s, int *info);
 /* Sweeps of this schedule use a dataflow kernel when the unit's data fits
  * in LDS: enable 1 (single right-hand side) = rows in rounds of one per lane with prefetch
  * and exchange wavefronts for parents of other units; enable 2 = the unit's
