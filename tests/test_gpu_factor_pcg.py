@@ -317,7 +317,7 @@ def test_factor_autotune_keeps_bits(handle, kind, name):
     av = up(A.values)
     chosen = s.autotune_factor(av)
     print("[factor autotune]", name, chosen)
-    assert chosen["mode"] in (1, 2, 3, 4, 5) and chosen["us"] > 0
+    assert chosen["mode"] in (1, 2, 3, 4, 5, 6) and chosen["us"] > 0
     lu = s.ic0(av) if kind == _lib.KIND_IC0 else s.ilu0(av)
     _status_ok(handle)
     np.testing.assert_array_equal(down(lu), want)
