@@ -690,16 +690,16 @@ template <int KIND>
 static int factor(baas_handle *h, baas_schedule *s, const double *a_vals, double *lu) {
     if (!h || !s || !a_vals || !lu) return BAAS_INVALID_ARG;
     if (s->kind != BAAS_KIND_IC0 && s->kind != BAAS_KIND_ILU0) return BAAS_INVALID_ARG;
-    if (a_vals != lu)
-        hipLaunchKernelGGL(k_copy_f64, dim3(nblk(s->nnz_full)), dim3(256), 0, h->stream, s->nnz_full, a_vals, lu);
-    // PIPE plans over a full stencil: the pipelined factorisation (reads A
-    // from a_vals when lu is a copy, else from lu before overwriting it row
-    // by row -- each row's A entries are read when its box starts, and only
-    // that box writes them)
+    // PIPE plans over a full stencil: the pipelined factorisation, which
+    // writes every entry of lu itself (A's values where the factor leaves
+    // them).  In place it reads each row's entries when its box starts, and
+    // only that box writes them.
     {
         const int rc = launch_pipe_factor(h, s, s->fwd, KIND, a_vals, lu);
         if (rc != BAAS_UNSUPPORTED) return rc;
     }
+    if (a_vals != lu)
+        hipLaunchKernelGGL(k_copy_f64, dim3(nblk(s->nnz_full)), dim3(256), 0, h->stream, s->nnz_full, a_vals, lu);
     const Plan &p = s->fwd;
     FactorArgs a;
     a.n = s->n;

@@ -683,14 +683,20 @@ __global__ __launch_bounds__(128) void k_factor_pipe(PipeArgs a, const double *_
             pipe_wave<T, false, XS, RMX, MODE>(a, lane, x0, y0, z0, tx, ty, kz, hx, hy, hz, fsz, smem_raw, ready,
                                                pub + bf.x);
         __syncthreads();
-        // the box's L (IC0) or L multipliers and U diagonal (ILU0) into lu
+        // the box's L (IC0) or L multipliers and U diagonal (ILU0) into lu,
+        // and, when lu is not A itself, the rows' upper entries (A's values)
         for (int q = tid; q < nrow; q += nt) {
             const int ii = q % tx, r2 = q / tx, jj = r2 % ty, kk = r2 / ty;
-            const PipeApos o = pipe_apos(a, x0 + ii, y0 + jj, z0 + kk);
+            const int xg = x0 + ii, yg = y0 + jj, zg = z0 + kk;
+            const PipeApos o = pipe_apos(a, xg, yg, zg);
             if (o.ez) lu[o.z] = c0[q];
             if (o.ey) lu[o.y] = c1[q];
             if (o.ex) lu[o.x] = c2[q];
             lu[o.d] = dg[q];
+            if (av != lu) {
+                const int nup = (xg < a.nx - 1) + (a.sy > 0 && yg < a.ny - 1) + (zg < a.nz - 1);
+                for (int e = 1; e <= nup; e++) lu[o.d + e] = av[o.d + e];
+            }
         }
     }
     if (threadIdx.x == 0) {
@@ -801,7 +807,8 @@ static int launch_factor_r(baas_handle *h, const PipeArgs &a, const double *av, 
 }
 
 // IC0 / ILU0 on the forward plan when it is a PIPE plan over a full stencil
-// (p.pipe_fact); BAAS_UNSUPPORTED otherwise.  lu holds A's values.
+// (p.pipe_fact), from A's values in a_vals into lu (which may be a_vals);
+// BAAS_UNSUPPORTED otherwise, with lu not touched.
 int launch_pipe_factor(baas_handle *h, baas_schedule *s, Plan &p, int kind, const double *a_vals, double *lu) {
     if (!p.pipe || !p.pipe_fact || p.dataflow != 5) return BAAS_UNSUPPORTED;
     if (pipe_factor_lds(p, kind) > lds_limit(h)) return BAAS_UNSUPPORTED;
