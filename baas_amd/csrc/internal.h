@@ -110,7 +110,7 @@ struct Plan {
     size_t mr_lds_val = 0;
     DBuf<unsigned char> xp_raw;  // 2 x n x nrhs published values (dataflow kernels)
     int xp_type = 0;             // sizeof the value type the halves are armed for
-    int xp_nrhs = 0;             // ... and their right-hand sides per row (-1: a PIPE plan's faces)
+    int xp_nrhs = 0;             // ... and their right-hand sides per row (-k: a PIPE plan's faces for k)
     DBuf<int> pcount;            // [1] sweeps run on this plan
     // PIPE plans: the grid (nx, ny, nz, sy (-1 on a 2D grid), sz) and box
     // shape (TX, TY, K) in pg; per unit the box's origin x, y, z and

@@ -45,10 +45,11 @@ namespace baas {
 size_t lds_limit(baas_handle *h);
 
 struct PipeArgs {
-    int n, nunits, nrhs, rhs;
+    int n, nunits, nrhs;  // b and x are n x nrhs, row-major
     int nx, ny, nz, sy, sz;  // grid; sy < 0: a 2D grid (no y direction)
     int TX, TY, K;           // box shape
-    long long half;          // values per publication half
+    long long half;          // values per publication half: nrhs regions of rstride
+    long long rstride;       // one right-hand side's box faces
     // per unit two records: x0, y0, z0, tx | ty << 8 | kz << 16; then the
     // offsets of the box's own faces, of the x face before it, of the y
     // face before it and of the top plane below it (-1: none)
@@ -137,8 +138,10 @@ __device__ __forceinline__ int wave_min(int v) {
 // through).  A wait beyond the spin limit reports BAAS_TIMEOUT and stages
 // zeros.  Once every value is staged (or the wait gave up), the faces are
 // re-armed with the sentinel in the other publication half (`other`
-// values away), so the producers' steps carry no second store.
-template <class T>
+// values away), so the producers' steps carry no second store.  With NR
+// right-hand sides a face value is NR consecutive values, in the
+// publication half and in LDS alike, and it is found when all NR are.
+template <class T, int NR = 1>
 __device__ __forceinline__ void pipe_stage(const T *px, int nxf, int ty, const T *py, int nyf, int tx, const T *pz,
                                            int nzf, T *ex, T *ey, T *ez, T *dump, const T *harmless, int nsteps,
                                            int *ready, int lane, long long limit, DevStatus *status, long long other) {
@@ -153,13 +156,26 @@ __device__ __forceinline__ void pipe_stage(const T *px, int nxf, int ty, const T
     for (;;) {
         const int wx = cx & ~7, wy = cy & ~7;
         const bool qx = (cx < nxf) & (wx + lane < nxf), qy = (cy < nyf) & (wy + lane < nyf);
-        const T vx = ld_agent(qx ? px + wx + lane : harmless);
-        const T vy = ld_agent(qy ? py + wy + lane : harmless);
-        const T vz = ld_agent(zt ? pz + lane : harmless);
-        const bool fx = qx & !Sentinel<T>::is(vx), fy = qy & !Sentinel<T>::is(vy), fz = zt & !Sentinel<T>::is(vz);
-        st_lds(fx ? ex + wx + lane : dump, vx);
-        st_lds(fy ? ey + wy + lane : dump, vy);
-        st_lds(fz ? ez + lane : dump, vz);
+        T vx[NR], vy[NR], vz[NR];
+        bool fx = qx, fy = qy, fz = zt;
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            vx[r] = ld_agent(qx ? px + (wx + lane) * NR + r : harmless);
+            vy[r] = ld_agent(qy ? py + (wy + lane) * NR + r : harmless);
+            vz[r] = ld_agent(zt ? pz + lane * NR + r : harmless);
+        }
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            fx = fx & !Sentinel<T>::is(vx[r]);
+            fy = fy & !Sentinel<T>::is(vy[r]);
+            fz = fz & !Sentinel<T>::is(vz[r]);
+        }
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            st_lds(fx ? ex + (wx + lane) * NR + r : dump, vx[r]);
+            st_lds(fy ? ey + (wy + lane) * NR + r : dump, vy[r]);
+            st_lds(fz ? ez + lane * NR + r : dump, vz[r]);
+        }
         zt = zt & !fz;
         const unsigned long long mx = __ballot(qx & !fx), my = __ballot(qy & !fy);
         cx = mx ? wx + (int)__builtin_ctzll(mx) : min(wx + 64, nxf);
@@ -175,9 +191,10 @@ __device__ __forceinline__ void pipe_stage(const T *px, int nxf, int ty, const T
         if (first >= nsteps) break;
         if (++spins > limit) {
             raise_status(status, BAAS_TIMEOUT, last);
-            for (int c = cx + lane; c < nxf; c += 64) st_lds(ex + c, T(0));
-            for (int c = cy + lane; c < nyf; c += 64) st_lds(ey + c, T(0));
-            if (zt) st_lds(ez + lane, T(0));
+            for (int c = cx * NR + lane; c < nxf * NR; c += 64) st_lds(ex + c, T(0));
+            for (int c = cy * NR + lane; c < nyf * NR; c += 64) st_lds(ey + c, T(0));
+            if (zt)
+                for (int r = 0; r < NR; r++) st_lds(ez + lane * NR + r, T(0));
             if (lane == 0) __hip_atomic_store(ready, nsteps, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
             break;
         }
@@ -185,9 +202,10 @@ __device__ __forceinline__ void pipe_stage(const T *px, int nxf, int ty, const T
     }
     // every value of these faces is read by this box alone: re-arm them in
     // the other half, which the next sweep publishes into
-    for (int c = lane; c < nxf; c += 64) const_cast<T *>(px)[other + c] = Sentinel<T>::value();
-    for (int c = lane; c < nyf; c += 64) const_cast<T *>(py)[other + c] = Sentinel<T>::value();
-    if (pz && lane < nzf) const_cast<T *>(pz)[other + lane] = Sentinel<T>::value();
+    for (int c = lane; c < nxf * NR; c += 64) const_cast<T *>(px)[other + c] = Sentinel<T>::value();
+    for (int c = lane; c < nyf * NR; c += 64) const_cast<T *>(py)[other + c] = Sentinel<T>::value();
+    if (pz)
+        for (int c = lane; c < nzf * NR; c += 64) const_cast<T *>(pz)[other + c] = Sentinel<T>::value();
 }
 
 // The compute wavefront reads the progress word relaxed, then the staged
@@ -209,21 +227,25 @@ enum { PIPE_SOLVE = 0, PIPE_IC0 = 1, PIPE_ILU0 = 2 };
 // LDS of one box of at most RMX rows (compile-time, so every record read is
 // one address register plus an immediate offset), in arrays of RMX + 1
 // values (index RMX is the spare row idle lanes read and write): per row
-// the three parent coefficients (z, y, x) and the diagonal; for a solve b
-// and the solution; for ILU0 the parents' entries in this row's column
-// (z, y, x).  A factorisation writes its results over the coefficients and
-// the diagonal.  Then the staged faces (EX: the x face, EY: the y face,
-// each in the order of its publication, EZ: the plane below), the slot
-// parents outside the grid read (1) and a dump slot; then the progress word.
-template <class T, int RMX, int MODE = PIPE_SOLVE>
+// the three parent coefficients (z, y, x) and the diagonal; for a solve b,
+// whose slot then takes the solution (a row's b is read a step before its
+// value is written; idle lanes write the spare row's b, 0, back); for ILU0
+// the parents' entries in this row's column (z, y, x).  A factorisation
+// writes its results over the coefficients and the diagonal.  Then the
+// staged faces (EX: the x face, EY: the y face, each in the order of its
+// publication and of at most RMX / 2 values, EZ: the plane below), the
+// slot parents outside the grid read (1) and a dump slot; then the
+// progress word.  A solve of NR right-hand sides at once keeps NR values
+// per row in b's array and per slot in the faces (slot s at s NR .. s NR +
+// NR - 1).
+template <class T, int RMX, int MODE = PIPE_SOLVE, int NR = 1>
 struct PipeLds {
-    static constexpr int NS = RMX + 1;
-    static constexpr int NA = MODE == PIPE_SOLVE ? 6 : (MODE == PIPE_IC0 ? 4 : 7);
-    static constexpr int EX = 0, EY = PIPE_FACE_MAX, EZ = 2 * PIPE_FACE_MAX, NONE = EZ + 64, DUMP = NONE + 1,
-                         NE = DUMP + 1;
+    static constexpr int NS = RMX + 1, FMX = RMX / 2;
+    static constexpr int NA = MODE == PIPE_SOLVE ? 4 + NR : (MODE == PIPE_IC0 ? 4 : 7);
+    static constexpr int EX = 0, EY = FMX, EZ = 2 * FMX, NONE = EZ + 64, DUMP = NONE + 1, NE = DUMP + 1;
     static constexpr size_t A = NS * sizeof(T);
-    static constexpr size_t C0 = 0, C1 = A, C2 = 2 * A, DG = 3 * A, BV = 4 * A, XV = 5 * A, U0 = 4 * A, U1 = 5 * A,
-                            U2 = 6 * A, EV = NA * A, RDY = EV + NE * sizeof(T), TOTAL = RDY + 16;
+    static constexpr size_t C0 = 0, C1 = A, C2 = 2 * A, DG = 3 * A, BV = 4 * A, XV = BV, U0 = 4 * A, U1 = 5 * A,
+                            U2 = 6 * A, EV = NA * A, RDY = EV + NE * NR * sizeof(T), TOTAL = RDY + 16;
 };
 
 // One step's inputs of a lane, read a step ahead so that their LDS reads
@@ -231,10 +253,11 @@ struct PipeLds {
 // coefficients, diagonal and b, and the staged values of the
 // parents it takes from other boxes (valid when the progress word read
 // before them is past the step).
-template <class T>
+template <class T, int NR = 1>
 struct PipeRec {
-    T cz, cy, cx, dd, bb, uz, uy, ux, ez, ey, ex;
-    int rdy, q, ax, ay;  // progress word; record slot; staged x- and y-face slots read
+    T cz, cy, cx, dd, uz, uy, ux;
+    T bb[NR], ez[NR], ey[NR], ex[NR];  // per right-hand side
+    int rdy, q, ax, ay;                // progress word; record slot; staged x- and y-face slots read
 };
 
 // The compute wavefront.  Lane (i, j) = (lane / TY, lane % TY); its y parent
@@ -249,12 +272,14 @@ struct PipeRec {
 // The x-face lanes (i = tx - 1) write diagonal d = t - (tx - 1) of the x
 // face at step t, all of it; the y face likewise; the top plane row by
 // row.  A face value's offset is linear in t, and whether a step writes a
-// diagonal depends on t alone.
-template <class T, bool UPPER, int XS, int RMX, int MODE = PIPE_SOLVE>
+// diagonal depends on t alone.  With NR right-hand sides (solves only) a
+// lane carries NR values through the same steps, on the same coefficients.
+template <class T, bool UPPER, int XS, int RMX, int MODE = PIPE_SOLVE, int NR = 1>
 __device__ __forceinline__ void pipe_wave(const PipeArgs &a, int lane, int x0, int y0, int z0, int tx, int ty,
                                           int kz, bool hx, bool hy, bool hz, int3 fsz, unsigned char *smem,
                                           const int *ready, T *opub) {
-    using Lay = PipeLds<T, RMX, MODE>;
+    static_assert(NR == 1 || MODE == PIPE_SOLVE, "factorisations carry one value a row");
+    using Lay = PipeLds<T, RMX, MODE, NR>;
     const T *c0 = reinterpret_cast<const T *>(smem + Lay::C0);
     const T *c1 = reinterpret_cast<const T *>(smem + Lay::C1);
     const T *c2 = reinterpret_cast<const T *>(smem + Lay::C2);
@@ -311,9 +336,10 @@ __device__ __forceinline__ void pipe_wave(const PipeArgs &a, int lane, int x0, i
     }
     const int oz0 = oz ? fsz.x + fsz.y + i + tx * j : dmp;
     const int tz = kz - 1 + i + j;
-    auto publish_a = [&](int t, T v) {
+    auto publish_a = [&](int t, const T *v) {
         const int o = (unsigned)(t - aA) < (unsigned)nA ? t * sA + cA : dmp;
-        st_agent(opub + (unsigned)o, v);
+#pragma unroll
+        for (int r = 0; r < NR; r++) st_agent(opub + (unsigned)(o * NR + r), v[r]);
     };
     auto slot = [&](int k) { return lok & ((unsigned)k < (unsigned)kz) ? qb + k * txy : RMX; };
     // Staged face slots advance by xst (yst) a step.  A lane reads them at
@@ -321,12 +347,15 @@ __device__ __forceinline__ void pipe_wave(const PipeArgs &a, int lane, int x0, i
     // j = 0, 0 <= k < kz); at the others the slot is still inside the staged
     // faces' LDS, and the value read goes to an idle computation.
     int ax = xs0 + (-i - j) * xst, ay = ys0 + (-i - j) * yst;
-    auto load_ext = [&](PipeRec<T> &R) {
-        R.ez = ev[zslot];
-        if (XS != 1) R.ey = ev[R.ay];
-        R.ex = ev[R.ax];
+    auto load_ext = [&](PipeRec<T, NR> &R) {
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            R.ez[r] = ev[zslot * NR + r];
+            if (XS != 1) R.ey[r] = ev[R.ay * NR + r];
+            R.ex[r] = ev[R.ax * NR + r];
+        }
     };
-    auto load = [&](int k, PipeRec<T> &R) {
+    auto load = [&](int k, PipeRec<T, NR> &R) {
         R.rdy = ld_ready(ready);
         const int q = slot(k);
         R.q = q;
@@ -334,7 +363,9 @@ __device__ __forceinline__ void pipe_wave(const PipeArgs &a, int lane, int x0, i
         if (XS != 1) R.cy = c1[q];
         R.cx = c2[q];
         R.dd = dg[q];
-        if (MODE == PIPE_SOLVE) R.bb = bv[q];
+        if (MODE == PIPE_SOLVE)
+#pragma unroll
+            for (int r = 0; r < NR; r++) R.bb[r] = bv[q * NR + r];
         if (MODE == PIPE_ILU0) {
             R.uz = u0[q];
             if (XS != 1) R.uy = u1[q];
@@ -346,8 +377,10 @@ __device__ __forceinline__ void pipe_wave(const PipeArgs &a, int lane, int x0, i
         ay += yst;
         load_ext(R);
     };
-    T xl = T(0);  // the value this lane solved at the previous step
-    auto step = [&](int t, PipeRec<T> &C, PipeRec<T> &N) {
+    T xl[NR];  // the values this lane solved at the previous step
+#pragma unroll
+    for (int r = 0; r < NR; r++) xl[r] = T(0);
+    auto step = [&](int t, PipeRec<T, NR> &C, PipeRec<T, NR> &N) {
         const int k = t - i - j;
         if (__builtin_expect(t >= C.rdy, 0)) {
             // a parent from another box this step is not staged yet: wait
@@ -373,21 +406,33 @@ __device__ __forceinline__ void pipe_wave(const PipeArgs &a, int lane, int x0, i
         // read from a lane that is off in EXEC returns zero, so they must
         // not be sunk into the branch a select may become (the empty asm
         // pins them here).
-        T vx = XS == 1 ? shr1(C.ex, xl) : lane_read(xl, xaddr);
-        T vy = XS == 1 ? T(0) : shr1z(xl);
-        __asm__ volatile("" : "+v"(vx), "+v"(vy));
-        const T pz = k == 0 ? C.ez : xl;
-        const T px = XS == 1 ? vx : (fx ? C.ex : vx);
-        const T py = XS == 1 ? T(0) : (fy ? C.ey : vy);
+        T vx[NR], vy[NR], pzv[NR], pxv[NR], pyv[NR];
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            vx[r] = XS == 1 ? shr1(C.ex[r], xl[r]) : lane_read(xl[r], xaddr);
+            vy[r] = XS == 1 ? T(0) : shr1z(xl[r]);
+            __asm__ volatile("" : "+v"(vx[r]), "+v"(vy[r]));
+        }
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            pzv[r] = k == 0 ? C.ez[r] : xl[r];
+            pxv[r] = XS == 1 ? vx[r] : (fx ? C.ex[r] : vx[r]);
+            pyv[r] = XS == 1 ? T(0) : (fy ? C.ey[r] : vy[r]);
+        }
         if (MODE == PIPE_SOLVE) {
             // z, y, x: ascending columns for L, descending for U (a 2D grid,
             // XS = 1, has no y term)
-            T s = UPPER ? fma(pz, C.cz, T(0)) : fma(C.cz, pz, T(0));
-            if (XS != 1) s = UPPER ? fma(py, C.cy, s) : fma(C.cy, py, s);
-            s = UPPER ? fma(px, C.cx, s) : fma(C.cx, px, s);
-            xl = (C.bb - s) / C.dd;
-            xv[C.q] = xl;
+#pragma unroll
+            for (int r = 0; r < NR; r++) {
+                const T pz = pzv[r], px = pxv[r], py = pyv[r];
+                T s = UPPER ? fma(pz, C.cz, T(0)) : fma(C.cz, pz, T(0));
+                if (XS != 1) s = UPPER ? fma(py, C.cy, s) : fma(C.cy, py, s);
+                s = UPPER ? fma(px, C.cx, s) : fma(C.cx, px, s);
+                xl[r] = (C.bb[r] - s) / C.dd;
+                xv[C.q * NR + r] = xl[r];
+            }
         } else {
+            const T pz = pzv[0], px = pxv[0], py = pyv[0];
             // the row's entries over its parents' diagonals (the parents'
             // rows share no earlier column with it on a 7- or 5-point stencil,
             // so the reference's dot products are zero: A_ik - 0 = A_ik)
@@ -420,16 +465,21 @@ __device__ __forceinline__ void pipe_wave(const PipeArgs &a, int lane, int x0, i
             if (XS != 1) w1[C.q] = ly;
             w2[C.q] = lx;
             wd[C.q] = d;
-            xl = d;
+            xl[0] = d;
         }
         // (a lane without a row this step publishes a value no consumer
         // reads: idle lanes take parents only from idle lanes, and their
         // arithmetic never yields the signalling-NaN sentinel)
-        publish_a(t, cy_help ? vx : xl);
-        if (t >= kz - 1) st_agent(opub + (unsigned)(t == tz ? oz0 : dmp), xl);
+        T pv[NR];
+#pragma unroll
+        for (int r = 0; r < NR; r++) pv[r] = cy_help ? vx[r] : xl[r];
+        publish_a(t, pv);
+        if (t >= kz - 1)
+#pragma unroll
+            for (int r = 0; r < NR; r++) st_agent(opub + (unsigned)((t == tz ? oz0 : dmp) * NR + r), xl[r]);
     };
     // two register sets used in turn, so no step copies one into the other
-    PipeRec<T> A, B;
+    PipeRec<T, NR> A, B;
     load(-i - j, A);
     for (int t = 0; t < nsteps; t += 2) {
         step(t, A, B);
@@ -437,16 +487,20 @@ __device__ __forceinline__ void pipe_wave(const PipeArgs &a, int lane, int x0, i
     }
     if (XS != 1) {
         // the corner's last y-face value (its final step's row)
-        T vc = lane_read(xl, xaddr);
-        __asm__ volatile("" : "+v"(vc));
+        T vc[NR];
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+            vc[r] = lane_read(xl[r], xaddr);
+            __asm__ volatile("" : "+v"(vc[r]));
+        }
         if (cy_help) publish_a(nsteps, vc);
     }
 }
 
-template <class T, bool UPPER, int XS, int RMX>
+template <class T, bool UPPER, int XS, int RMX, int NR>
 __global__ __launch_bounds__(128) void k_sweep_pipe(PipeArgs a, const T *__restrict__ vals, const T *__restrict__ b,
                                                     T *x, T *xp) {
-    using Lay = PipeLds<T, RMX>;
+    using Lay = PipeLds<T, RMX, PIPE_SOLVE, NR>;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     __shared__ int s_unit;
     if (a.skip && ld_agent(a.skip)) return;
@@ -470,12 +524,20 @@ __global__ __launch_bounds__(128) void k_sweep_pipe(PipeArgs a, const T *__restr
     if (tid == 0) {
         c0[RMX] = c1[RMX] = c2[RMX] = -T(0);
         dg[RMX] = T(1);
-        bv[RMX] = T(0);
-        ev[Lay::NONE] = T(1);
+        for (int r = 0; r < NR; r++) {
+            bv[RMX * NR + r] = T(0);
+            ev[Lay::NONE * NR + r] = T(1);
+        }
     }
     for (;;) {
-        const int u = __builtin_amdgcn_readfirstlane(claim_unit(a.ctx.ticket, &s_unit));
-        if (u >= a.nunits) break;
+        // tickets name (box, group of NR right-hand sides) pairs, box-major:
+        // the boxes a box waits for (same group, earlier in wavefront order)
+        // hold earlier tickets
+        const int ng = a.nrhs / NR;
+        const int tk = __builtin_amdgcn_readfirstlane(claim_unit(a.ctx.ticket, &s_unit));
+        if (tk >= a.nunits * ng) break;
+        const int u = tk / ng, grp = tk - u * ng, rhs = grp * NR;
+        T *rpub = pub + (long long)grp * a.rstride * NR;  // this group's faces
         const int4 bx = a.box[2 * u], bf = a.box[2 * u + 1];
         const int x0 = bx.x, y0 = bx.y, z0 = bx.z;
         const int tx = bx.w & 255, ty = (bx.w >> 8) & 255, kz = (bx.w >> 16) & 255;
@@ -492,10 +554,10 @@ __global__ __launch_bounds__(128) void k_sweep_pipe(PipeArgs a, const T *__restr
         // the diagonal; for U = L^T (the backward sweep, whose flipped
         // coordinates make the same count of parents) the diagonal first.
         // A parent the row lacks reads the diagonal and gets coefficient -0.
-        constexpr int PB = 8;
+        constexpr int PB = NR == 1 ? 8 : 4;
         for (int q0 = tid; q0 < nrow; q0 += PB * nt) {
             bool hxr[PB], hyr[PB], hzr[PB];
-            T cz[PB], cy[PB], cx[PB], cd[PB], bb[PB];
+            T cz[PB], cy[PB], cx[PB], cd[PB], bb[PB][NR];
 #pragma unroll
             for (int v = 0; v < PB; v++) {
                 const int q = min(q0 + v * nt, nrow - 1);
@@ -514,7 +576,9 @@ __global__ __launch_bounds__(128) void k_sweep_pipe(PipeArgs a, const T *__restr
                 cy[v] = vals[ey ? py : pd];
                 cx[v] = vals[ex ? px : pd];
                 cd[v] = vals[pd];
-                bb[v] = b[(size_t)pipe_row<UPPER>(a, xg, yg, zg) * a.nrhs + a.rhs];
+                const size_t rb = (size_t)pipe_row<UPPER>(a, xg, yg, zg) * a.nrhs + rhs;
+#pragma unroll
+                for (int r = 0; r < NR; r++) bb[v][r] = b[rb + r];
             }
 #pragma unroll
             for (int v = 0; v < PB; v++) {
@@ -524,22 +588,25 @@ __global__ __launch_bounds__(128) void k_sweep_pipe(PipeArgs a, const T *__restr
                 c1[q] = hyr[v] ? cy[v] : -T(0);
                 c2[q] = hxr[v] ? cx[v] : -T(0);
                 dg[q] = cd[v];
-                bv[q] = bb[v];
+#pragma unroll
+                for (int r = 0; r < NR; r++) bv[q * NR + r] = bb[v][r];
             }
         }
         __syncthreads();
         if (wv == 0)
-            pipe_stage<T>(hx ? pub + bf.y : nullptr, fsz.x, ty, hy ? pub + bf.z : nullptr, fsz.y, tx,
-                          hz ? pub + bf.w : nullptr, fsz.z, ev + Lay::EX, ev + Lay::EY, ev + Lay::EZ, ev + Lay::DUMP,
-                          pub, nsteps, ready, lane, a.spin_limit, a.status, (long long)(nxt - pub));
+            pipe_stage<T, NR>(hx ? rpub + bf.y * NR : nullptr, fsz.x, ty, hy ? rpub + bf.z * NR : nullptr, fsz.y, tx,
+                              hz ? rpub + bf.w * NR : nullptr, fsz.z, ev + Lay::EX * NR, ev + Lay::EY * NR,
+                              ev + Lay::EZ * NR, ev + Lay::DUMP * NR, rpub, nsteps, ready, lane, a.spin_limit,
+                              a.status, (long long)(nxt - pub));
         else if (wv == 1)
-            pipe_wave<T, UPPER, XS, RMX>(a, lane, x0, y0, z0, tx, ty, kz, hx, hy, hz, fsz, smem_raw, ready,
-                                         pub + bf.x);
+            pipe_wave<T, UPPER, XS, RMX, PIPE_SOLVE, NR>(a, lane, x0, y0, z0, tx, ty, kz, hx, hy, hz, fsz, smem_raw,
+                                                         ready, rpub + bf.x * NR);
         __syncthreads();
         for (int q = tid; q < nrow; q += nt) {
             const int ii = q % tx, r2 = q / tx, jj = r2 % ty, kk = r2 / ty;
             const int row = pipe_row<UPPER>(a, x0 + ii, y0 + jj, z0 + kk);
-            x[(size_t)row * a.nrhs + a.rhs] = xv[q];
+#pragma unroll
+            for (int r = 0; r < NR; r++) x[(size_t)row * a.nrhs + rhs + r] = xv[q * NR + r];
         }
     }
     if (threadIdx.x == 0) {
@@ -712,41 +779,66 @@ __global__ __launch_bounds__(128) void k_factor_pipe(PipeArgs a, const double *_
     }
 }
 
-// Rows per box the kernel instances hold (the plan keeps TX TY K within).
-static int pipe_rmx(const Plan &p) { return p.pg[5] * p.pg[6] * p.pg[7] <= 1024 ? 1024 : 2048; }
+// The kernel instance for plan p: the smallest of 512, 1024 and 2048 rows
+// that holds the box's rows and, in RMX / 2 slots each, its x and y faces
+// (0: none does; the plan builder keeps boxes within 2048 rows and faces
+// within PIPE_FACE_MAX = 1024 values, so one always does).  Smaller
+// instances take less LDS, and more boxes share a CU.
+static int pipe_rmx(const Plan &p) {
+    const int rows = p.pg[5] * p.pg[6] * p.pg[7];
+    const int3 f = pipe_face_sizes(p.pg[5], p.pg[6], p.pg[7], p.pg[3] > 0);
+    if (f.z > 64) return 0;
+    for (int r : {512, 1024, 2048})
+        if (rows <= r && f.x <= r / 2 && f.y <= r / 2) return r;
+    return 0;
+}
 
 // LDS bytes the pipelined sweep needs per workgroup on plan p.
 size_t pipe_lds(const Plan &p, int ts) {
-    if (p.pg[5] * p.pg[6] * p.pg[7] > 2048) return (size_t)1 << 40;  // no instance holds such boxes
-    const bool r1 = pipe_rmx(p) == 1024;
-    return ts == 8 ? (r1 ? PipeLds<double, 1024>::TOTAL : PipeLds<double, 2048>::TOTAL)
-                   : (r1 ? PipeLds<float, 1024>::TOTAL : PipeLds<float, 2048>::TOTAL);
+    switch (pipe_rmx(p)) {
+    case 512: return ts == 8 ? PipeLds<double, 512>::TOTAL : PipeLds<float, 512>::TOTAL;
+    case 1024: return ts == 8 ? PipeLds<double, 1024>::TOTAL : PipeLds<float, 1024>::TOTAL;
+    case 2048: return ts == 8 ? PipeLds<double, 2048>::TOTAL : PipeLds<float, 2048>::TOTAL;
+    }
+    return (size_t)1 << 40;  // no instance holds such boxes
 }
 
-template <class T, int RMX>
+template <class T, int RMX, int NR>
 static int launch_pipe_r(baas_handle *h, const PipeArgs &a0, int uplo, const T *vals, const T *b, T *x, T *xp,
                          int nunits, int nrhs, hipStream_t stream) {
     // the x parent sits TY lanes down: a DPP shift for TY = 1 (2D grids),
     // the LDS crossbar otherwise
     using K = void (*)(PipeArgs, const T *, const T *, T *, T *);
-    K kern = a0.TY == 1 ? (uplo ? k_sweep_pipe<T, true, 1, RMX> : k_sweep_pipe<T, false, 1, RMX>)
-                        : (uplo ? k_sweep_pipe<T, true, 0, RMX> : k_sweep_pipe<T, false, 0, RMX>);
-    const size_t lds = PipeLds<T, RMX>::TOTAL;
+    K kern = a0.TY == 1 ? (uplo ? k_sweep_pipe<T, true, 1, RMX, NR> : k_sweep_pipe<T, false, 1, RMX, NR>)
+                        : (uplo ? k_sweep_pipe<T, true, 0, RMX, NR> : k_sweep_pipe<T, false, 0, RMX, NR>);
+    const size_t lds = PipeLds<T, RMX, PIPE_SOLVE, NR>::TOTAL;
     int per_cu = 0;
     BAAS_HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, 128, lds));
     if (per_cu <= 0) return BAAS_UNSUPPORTED;
-    long long grid = std::min<long long>((long long)per_cu * h->num_cus, nunits);
+    long long grid = std::min<long long>((long long)per_cu * h->num_cus, (long long)nunits * (nrhs / NR));
     if (grid < 1) grid = 1;
-    PipeArgs a = a0;
-    for (int r = 0; r < nrhs; r++) {
-        a.rhs = r;
-        hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(128), lds, stream, a, vals, b, x, xp);
-        BAAS_HIP_TRY(hipGetLastError());
-    }
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(128), lds, stream, a0, vals, b, x, xp);
+    BAAS_HIP_TRY(hipGetLastError());
     return BAAS_OK;
 }
 
-// One launch per right-hand side (b and x are n x nrhs, row-major).
+// Right-hand sides per lane: two where the count is even (each lane then
+// solves two columns on the same coefficients, half the tickets, and b and
+// x move in 16-byte pairs), else one.  On the 64^3 Laplacian with 32
+// right-hand sides, two a lane took 0.19 ms per launch against 0.24 ms for
+// one and 0.21 ms for four (boxes of 1024 rows; other box sizes ranked the
+// same or tied).
+template <class T, int RMX>
+static int launch_pipe_nr(baas_handle *h, const PipeArgs &a, int uplo, const T *vals, const T *b, T *x, T *xp,
+                          int nunits, int nrhs, hipStream_t stream) {
+    if (nrhs % 2 == 0 && PipeLds<T, RMX, PIPE_SOLVE, 2>::TOTAL <= lds_limit(h))
+        return launch_pipe_r<T, RMX, 2>(h, a, uplo, vals, b, x, xp, nunits, nrhs, stream);
+    return launch_pipe_r<T, RMX, 1>(h, a, uplo, vals, b, x, xp, nunits, nrhs, stream);
+}
+
+// All right-hand sides in one launch (b and x are n x nrhs, row-major):
+// each has its own copy of the box faces in each publication half, and the
+// workgroups take (box, right-hand side) pairs from one ticket.
 template <class T>
 int launch_pipe(baas_handle *h, baas_schedule *s, const Plan &p, int uplo, const T *vals, const T *b, T *x, T *xp,
                 int nrhs, hipStream_t stream, const int *skip) {
@@ -754,7 +846,6 @@ int launch_pipe(baas_handle *h, baas_schedule *s, const Plan &p, int uplo, const
     a.n = s->n;
     a.nunits = p.nunits;
     a.nrhs = nrhs;
-    a.rhs = 0;
     a.nx = p.pg[0];
     a.ny = p.pg[1];
     a.nz = p.pg[2];
@@ -763,7 +854,8 @@ int launch_pipe(baas_handle *h, baas_schedule *s, const Plan &p, int uplo, const
     a.TX = p.pg[5];
     a.TY = p.pg[6];
     a.K = p.pg[7];
-    a.half = p.pipe_half;
+    a.half = p.pipe_half * nrhs;
+    a.rstride = p.pipe_half;
     a.box = reinterpret_cast<const int4 *>(p.pipe_box.p);
     a.nnz = s->nnz_lower;
     a.pcount = p.pcount.p;
@@ -774,8 +866,13 @@ int launch_pipe(baas_handle *h, baas_schedule *s, const Plan &p, int uplo, const
     // the kernel's LDS holds boxes of up to 2048 rows and faces of up to
     // PIPE_FACE_MAX values (the plan builder keeps within both)
     if (a.TX * a.TY * a.K > 2048 || !pipe_faces_fit(a.TX, a.TY, a.K, a.sy > 0) || a.half < 1) return BAAS_UNSUPPORTED;
-    return pipe_rmx(p) == 1024 ? launch_pipe_r<T, 1024>(h, a, uplo, vals, b, x, xp, p.nunits, nrhs, stream)
-                               : launch_pipe_r<T, 2048>(h, a, uplo, vals, b, x, xp, p.nunits, nrhs, stream);
+    if ((long long)p.nunits * nrhs > 0x7fffffff) return BAAS_TOO_LARGE;  // tickets are 32-bit
+    switch (pipe_rmx(p)) {
+    case 512: return launch_pipe_nr<T, 512>(h, a, uplo, vals, b, x, xp, p.nunits, nrhs, stream);
+    case 1024: return launch_pipe_nr<T, 1024>(h, a, uplo, vals, b, x, xp, p.nunits, nrhs, stream);
+    case 2048: return launch_pipe_nr<T, 2048>(h, a, uplo, vals, b, x, xp, p.nunits, nrhs, stream);
+    }
+    return BAAS_UNSUPPORTED;
 }
 
 __global__ void k_pipe_arm(size_t n, double *p) {
@@ -785,10 +882,13 @@ __global__ void k_pipe_arm(size_t n, double *p) {
 
 // LDS bytes of the factorisation on plan p.
 static size_t pipe_factor_lds(const Plan &p, int kind) {
-    if (p.pg[5] * p.pg[6] * p.pg[7] > 2048) return (size_t)1 << 40;
-    const bool r1 = pipe_rmx(p) == 1024;
-    if (kind == BAAS_KIND_IC0) return r1 ? PipeLds<double, 1024, PIPE_IC0>::TOTAL : PipeLds<double, 2048, PIPE_IC0>::TOTAL;
-    return r1 ? PipeLds<double, 1024, PIPE_ILU0>::TOTAL : PipeLds<double, 2048, PIPE_ILU0>::TOTAL;
+    const bool ic = kind == BAAS_KIND_IC0;
+    switch (pipe_rmx(p)) {
+    case 512: return ic ? PipeLds<double, 512, PIPE_IC0>::TOTAL : PipeLds<double, 512, PIPE_ILU0>::TOTAL;
+    case 1024: return ic ? PipeLds<double, 1024, PIPE_IC0>::TOTAL : PipeLds<double, 1024, PIPE_ILU0>::TOTAL;
+    case 2048: return ic ? PipeLds<double, 2048, PIPE_IC0>::TOTAL : PipeLds<double, 2048, PIPE_ILU0>::TOTAL;
+    }
+    return (size_t)1 << 40;
 }
 
 template <int MODE, int RMX>
@@ -829,7 +929,6 @@ int launch_pipe_factor(baas_handle *h, baas_schedule *s, Plan &p, int kind, cons
     a.n = s->n;
     a.nunits = p.nunits;
     a.nrhs = 1;
-    a.rhs = 0;
     a.nx = p.pg[0];
     a.ny = p.pg[1];
     a.nz = p.pg[2];
@@ -839,6 +938,7 @@ int launch_pipe_factor(baas_handle *h, baas_schedule *s, Plan &p, int kind, cons
     a.TY = p.pg[6];
     a.K = p.pg[7];
     a.half = p.pipe_half;
+    a.rstride = p.pipe_half;
     a.box = reinterpret_cast<const int4 *>(p.pipe_box.p);
     a.nnz = s->nnz_lower;
     a.pcount = p.pcount.p;
@@ -847,12 +947,19 @@ int launch_pipe_factor(baas_handle *h, baas_schedule *s, Plan &p, int kind, cons
     a.status = h->status;
     a.spin_limit = h->spin_limit;
     double *xp = reinterpret_cast<double *>(p.xp_raw.p);
-    const bool r1 = pipe_rmx(p) == 1024;
-    if (kind == BAAS_KIND_IC0)
-        return r1 ? launch_factor_r<PIPE_IC0, 1024>(h, a, a_vals, lu, xp, p.nunits)
-                  : launch_factor_r<PIPE_IC0, 2048>(h, a, a_vals, lu, xp, p.nunits);
-    return r1 ? launch_factor_r<PIPE_ILU0, 1024>(h, a, a_vals, lu, xp, p.nunits)
-              : launch_factor_r<PIPE_ILU0, 2048>(h, a, a_vals, lu, xp, p.nunits);
+    const bool ic = kind == BAAS_KIND_IC0;
+    switch (pipe_rmx(p)) {
+    case 512:
+        return ic ? launch_factor_r<PIPE_IC0, 512>(h, a, a_vals, lu, xp, p.nunits)
+                  : launch_factor_r<PIPE_ILU0, 512>(h, a, a_vals, lu, xp, p.nunits);
+    case 1024:
+        return ic ? launch_factor_r<PIPE_IC0, 1024>(h, a, a_vals, lu, xp, p.nunits)
+                  : launch_factor_r<PIPE_ILU0, 1024>(h, a, a_vals, lu, xp, p.nunits);
+    case 2048:
+        return ic ? launch_factor_r<PIPE_IC0, 2048>(h, a, a_vals, lu, xp, p.nunits)
+                  : launch_factor_r<PIPE_ILU0, 2048>(h, a, a_vals, lu, xp, p.nunits);
+    }
+    return BAAS_UNSUPPORTED;
 }
 
 template int launch_pipe<double>(baas_handle *, baas_schedule *, const Plan &, int, const double *, const double *,

@@ -1185,11 +1185,11 @@ int sweep(baas_handle *h, baas_schedule *s, int uplo, const T *vals, const T *b,
         (nrhs == 1 && p.dataflow == 1 && p.block >= 192 && p.block <= 512 &&
          p.df_lds[sizeof(T) == 8] <= lds_limit(h))) {
         // publication halves (one value per row and right-hand side, or a
-        // PIPE plan's box faces), armed with this value type's sentinel; any
-        // change of layout re-arms them
+        // PIPE plan's box faces per right-hand side), armed with this value
+        // type's sentinel; any change of layout re-arms them
         const int xr = lane ? nrhs : 1;
-        const size_t half = pipe ? (size_t)p.pipe_half : (size_t)s->n * xr;
-        const int xkey = pipe ? -1 : xr;
+        const size_t half = pipe ? (size_t)p.pipe_half * nrhs : (size_t)s->n * xr;
+        const int xkey = pipe ? -nrhs : xr;
         T *xp = reinterpret_cast<T *>(p.xp_raw.p);
         if (fixed_xp) {
             // shared by every call on this GPU and left holding the last

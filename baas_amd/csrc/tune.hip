@@ -113,9 +113,9 @@ int autotune(baas_handle *h, baas_schedule *s, const T *vals, int uplo, int nrhs
         cands.push_back({BAAS_PLAN_BOXES, R, 256, 0, 0});
     }
     // the wave-pipelined grid sweep: boxes of 8 x 8 (64 x 1 on a 2D grid)
-    // rows across by R / 64 planes
-    if (nrhs == 1)
-        for (int R : {512, 1024, 2048}) cands.push_back({BAAS_PLAN_PIPE, R, 128, 5, 0});
+    // rows across by R / 64 planes; several right-hand sides run their
+    // pipelines side by side in one launch
+    for (int R : {512, 1024, 2048}) cands.push_back({BAAS_PLAN_PIPE, R, 128, 5, 0});
     if (nrhs > 1) {
         for (int R : {64, 125, 216}) cands.push_back({BAAS_PLAN_BOXES, R, 1024, 2, 0});
         // the multi-RHS lane kernel (parents of other units read directly,

@@ -221,7 +221,9 @@ def _pipe_matrix(name, shape):
 def test_pipe_plans_bit_exact(handle, name, shape):
     """PIPE plans (boxes swept by one wavefront as a diagonal pipeline) on
     stencil grids, partial boxes and narrow grids included: forward and
-    backward bit for bit, every box depth."""
+    backward bit for bit, every box depth; then four and two right-hand
+    sides (two a lane: two pairs, one pair), each column against the
+    oracle."""
     from baas_amd.api import Schedule
     L, tag = _pipe_matrix(name, shape)
     s = Schedule(handle, L.rowptr, L.colidx, kind=0)
@@ -231,6 +233,9 @@ def test_pipe_plans_bit_exact(handle, name, shape):
     y_ref = O.sptrsv_upper(U, b)
     lv, bd = up(L.values), up(b)
     uv = s.transpose_values(lv)
+    B = g.rhs_uniform(L.n, 17, 4)
+    X_ref = np.stack([O.sptrsv_lower(L, np.ascontiguousarray(B[:, c])) for c in range(4)], 1)
+    Y_ref = np.stack([O.sptrsv_upper(U, np.ascontiguousarray(B[:, c])) for c in range(4)], 1)
     for rpu in (64, 256, 512, 1024, 2048):
         announce("pipe", tag, rpu)
         s.replan(6, rpu, 128, dataflow=5)
@@ -240,12 +245,21 @@ def test_pipe_plans_bit_exact(handle, name, shape):
             handle.check()
             np.testing.assert_array_equal(down(x), x_ref, err_msg=f"fwd {tag} {rpu}")
             np.testing.assert_array_equal(down(y), y_ref, err_msg=f"bwd {tag} {rpu}")
+        for k in (4, 2):
+            for _ in range(2):
+                X = down(s.solve(lv, up(np.ascontiguousarray(B[:, :k]))))
+                Y = down(s.solve(uv, up(np.ascontiguousarray(B[:, :k])), uplo=1))
+                handle.check()
+                np.testing.assert_array_equal(X, X_ref[:, :k], err_msg=f"fwd {tag} {rpu} {k} rhs")
+                np.testing.assert_array_equal(Y, Y_ref[:, :k], err_msg=f"bwd {tag} {rpu} {k} rhs")
 
 
 @pytest.mark.parametrize("name,shape", [("C2_lap3d_64", None), ("grid", (70, 5))])
 def test_pipe_plans_f32_and_multi_rhs(handle, name, shape):
-    """FP32 sweeps on PIPE plans against the oracle's FP32 solves, and three
-    right-hand sides (one launch per column) against the FP64 oracle."""
+    """FP32 sweeps on PIPE plans against the oracle's FP32 solves, and
+    several right-hand sides (all in one launch, each with its own box
+    faces) against the FP64 oracle: two calls per count, so both
+    publication halves carry them, and a change of count in between."""
     import torch
     from baas_amd.api import Schedule
     L, tag = _pipe_matrix(name, shape)
@@ -261,15 +275,28 @@ def test_pipe_plans_f32_and_multi_rhs(handle, name, shape):
     handle.check()
     np.testing.assert_array_equal(down(x), O.sptrsv_lower_f32(L, b))
     np.testing.assert_array_equal(down(y), O.sptrsv_upper_f32(U, b))
-    announce("pipe 3 rhs", tag)
-    B = g.rhs_uniform(L.n, 9, 3)
-    lv = up(L.values)
-    X = down(s.solve(lv, up(B)))
-    Y = down(s.solve(s.transpose_values(lv), up(B), uplo=1))
+    announce("pipe f32 2 rhs", tag)  # two a lane
+    B2 = g.rhs_uniform(L.n, 21, 2)
+    X = down(s.solve(lv32, up(B2, torch.float32)))
+    Y = down(s.solve(uv32, up(B2, torch.float32), uplo=1))
     handle.check()
-    for c in range(3):
-        np.testing.assert_array_equal(X[:, c], O.sptrsv_lower(L, np.ascontiguousarray(B[:, c])))
-        np.testing.assert_array_equal(Y[:, c], O.sptrsv_upper(U, np.ascontiguousarray(B[:, c])))
+    for c in range(2):
+        np.testing.assert_array_equal(X[:, c], O.sptrsv_lower_f32(L, np.ascontiguousarray(B2[:, c])))
+        np.testing.assert_array_equal(Y[:, c], O.sptrsv_upper_f32(U, np.ascontiguousarray(B2[:, c])))
+    lv = up(L.values)
+    uv = s.transpose_values(lv)
+    for k in ((3, 32) if L.n > 100000 else (5, 2)):
+        announce("pipe rhs", tag, k)
+        B = g.rhs_uniform(L.n, 9 + k, k)
+        want_x = [O.sptrsv_lower(L, np.ascontiguousarray(B[:, c])) for c in range(k)]
+        want_y = [O.sptrsv_upper(U, np.ascontiguousarray(B[:, c])) for c in range(k)]
+        for rep in range(2):
+            X = down(s.solve(lv, up(B)))
+            Y = down(s.solve(uv, up(B), uplo=1))
+            handle.check()
+            for c in range(k):
+                np.testing.assert_array_equal(X[:, c], want_x[c], err_msg=f"fwd {tag} {k} {c} {rep}")
+                np.testing.assert_array_equal(Y[:, c], want_y[c], err_msg=f"bwd {tag} {k} {c} {rep}")
 
 
 @pytest.mark.parametrize("case", ["C2_lap3d_64_perm1", "C3_knn", "wrap", "hole"])
