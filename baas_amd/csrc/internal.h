@@ -218,7 +218,9 @@ __host__ __device__ inline LaneLayout lane_layout(size_t nk, size_t ne, size_t E
 }  // namespace baas
 
 constexpr int BAAS_BARRIER_BASE = 64;
-constexpr int BAAS_COUNTER_INTS = BAAS_BARRIER_BASE + 8 * 32;
+// per-XCD ticket queues of multi-RHS PIPE sweeps, one 128-byte line each
+constexpr int BAAS_QTICKET_BASE = BAAS_BARRIER_BASE + 8 * 32;
+constexpr int BAAS_COUNTER_INTS = BAAS_QTICKET_BASE + 8 * 32;
 // TILES plans: largest work unit (rows); its values fit the dataflow kernel's LDS
 constexpr int BAAS_TILE_MAX_ROWS = 6144;
 

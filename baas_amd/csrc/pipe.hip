@@ -50,6 +50,8 @@ struct PipeArgs {
     int TX, TY, K;           // box shape
     long long half;          // values per publication half: nrhs regions of rstride
     long long rstride;       // one right-hand side's box faces
+    int nq;                  // ticket queues: 1 (the context's ticket) or one per XCD
+    int *qticket;            // queue q's ticket at qticket[32 q]
     // per unit two records: x0, y0, z0, tx | ty << 8 | kz << 16; then the
     // offsets of the box's own faces, of the x face before it, of the y
     // face before it and of the top plane below it (-1: none)
@@ -529,14 +531,23 @@ __global__ __launch_bounds__(128) void k_sweep_pipe(PipeArgs a, const T *__restr
             ev[Lay::NONE * NR + r] = T(1);
         }
     }
+    const int qid = (int)(blockIdx.x % (unsigned)a.nq);
+    int *tkt = a.nq > 1 ? a.qticket + 32 * qid : a.ctx.ticket;
+    const int qboxes = (a.nunits - qid + a.nq - 1) / a.nq;
     for (;;) {
         // tickets name (box, group of NR right-hand sides) pairs, box-major:
         // the boxes a box waits for (same group, earlier in wavefront order)
-        // hold earlier tickets
+        // hold earlier tickets.  With several groups, workgroup w takes
+        // tickets from queue w mod nq, which holds the boxes u = q mod nq:
+        // every group of a box then runs on one XCD (workgroups are
+        // dealt to the XCDs in turn) and reads the box's coefficients
+        // through one L2.  Every wait still ends: each queue is in box
+        // order and has a workgroup, so the smallest box any workgroup waits
+        // for has its producers finished or next in their queues.
         const int ng = a.nrhs / NR;
-        const int tk = __builtin_amdgcn_readfirstlane(claim_unit(a.ctx.ticket, &s_unit));
-        if (tk >= a.nunits * ng) break;
-        const int u = tk / ng, grp = tk - u * ng, rhs = grp * NR;
+        const int tk = __builtin_amdgcn_readfirstlane(claim_unit(tkt, &s_unit));
+        if (tk >= qboxes * ng) break;
+        const int u = qid + a.nq * (tk / ng), grp = tk - (tk / ng) * ng, rhs = grp * NR;
         T *rpub = pub + (long long)grp * a.rstride * NR;  // this group's faces
         const int4 bx = a.box[2 * u], bf = a.box[2 * u + 1];
         const int x0 = bx.x, y0 = bx.y, z0 = bx.z;
@@ -614,6 +625,7 @@ __global__ __launch_bounds__(128) void k_sweep_pipe(PipeArgs a, const T *__restr
         const int d = atomicAdd(a.ctx.done, 1);
         if (d == (int)gridDim.x - 1) {
             atomicExch(a.ctx.ticket, 0);
+            for (int q = 0; q < a.nq && a.nq > 1; q++) atomicExch(a.qticket + 32 * q, 0);
             atomicExch(a.ctx.epoch, epoch == 0x7fffffff ? 1 : epoch + 1);
             atomicAdd(a.pcount, 1);
             __threadfence();
@@ -817,7 +829,11 @@ static int launch_pipe_r(baas_handle *h, const PipeArgs &a0, int uplo, const T *
     if (per_cu <= 0) return BAAS_UNSUPPORTED;
     long long grid = std::min<long long>((long long)per_cu * h->num_cus, (long long)nunits * (nrhs / NR));
     if (grid < 1) grid = 1;
-    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(128), lds, stream, a0, vals, b, x, xp);
+    PipeArgs a = a0;
+    // several groups: one ticket queue per XCD (fewer when the grid or the
+    // boxes are fewer, so that every queue has a workgroup and a box)
+    a.nq = nrhs / NR > 1 ? (int)std::min<long long>({8, grid, (long long)nunits}) : 1;
+    hipLaunchKernelGGL(kern, dim3((unsigned)grid), dim3(128), lds, stream, a, vals, b, x, xp);
     BAAS_HIP_TRY(hipGetLastError());
     return BAAS_OK;
 }
@@ -856,6 +872,8 @@ int launch_pipe(baas_handle *h, baas_schedule *s, const Plan &p, int uplo, const
     a.K = p.pg[7];
     a.half = p.pipe_half * nrhs;
     a.rstride = p.pipe_half;
+    a.nq = 1;
+    a.qticket = h->counters + BAAS_QTICKET_BASE;
     a.box = reinterpret_cast<const int4 *>(p.pipe_box.p);
     a.nnz = s->nnz_lower;
     a.pcount = p.pcount.p;
@@ -939,6 +957,8 @@ int launch_pipe_factor(baas_handle *h, baas_schedule *s, Plan &p, int kind, cons
     a.K = p.pg[7];
     a.half = p.pipe_half;
     a.rstride = p.pipe_half;
+    a.nq = 1;
+    a.qticket = nullptr;
     a.box = reinterpret_cast<const int4 *>(p.pipe_box.p);
     a.nnz = s->nnz_lower;
     a.pcount = p.pcount.p;

@@ -59,7 +59,7 @@ def _case(h, L, plan, nrhs, seed):
 def test_noalloc_equals_sptrsv_two_calls_smaller_second(handle, plan):
     big = g.laplace3d(64).lower()
     small = g.laplace2d_lower(128)
-    for tag, L, nrhs in (("64^3", big, 1), ("128^2", small, 1), ("128^2 x3", small, 3)):
+    for tag, L, nrhs in (("64^3", big, 1), ("128^2", small, 1), ("128^2 x3", small, 3), ("128^2 x4", small, 4)):
         announce("noalloc", plan, tag)
         for got, want in _case(handle, L, plan, nrhs, seed=len(tag)):
             np.testing.assert_array_equal(got, want, err_msg=f"{plan} {tag}")
