@@ -841,9 +841,9 @@ static int launch_pipe_r(baas_handle *h, const PipeArgs &a0, int uplo, const T *
 // Right-hand sides per lane: two where the count is even (each lane then
 // solves two columns on the same coefficients, half the tickets, and b and
 // x move in 16-byte pairs), else one.  On the 64^3 Laplacian with 32
-// right-hand sides, two a lane took 0.19 ms per launch against 0.24 ms for
-// one and 0.21 ms for four (boxes of 1024 rows; other box sizes ranked the
-// same or tied).
+// right-hand sides, two a lane took 0.17 ms per launch against 0.20 ms for
+// one and 0.20 ms for four (boxes of 1024 rows, per-XCD ticket queues;
+// 512- and 2048-row boxes ranked the same).
 template <class T, int RMX>
 static int launch_pipe_nr(baas_handle *h, const PipeArgs &a, int uplo, const T *vals, const T *b, T *x, T *xp,
                           int nunits, int nrhs, hipStream_t stream) {
