@@ -143,6 +143,57 @@ __global__ void k_spmv_pap(int k, const Chunk *__restrict__ ch, const int *__res
     chunk_reduce(k, acc, red, part);
 }
 
+// The same with two columns a thread (k even): a row's indices and values
+// are read once for both, and p and Ap move in 16-byte pairs.  The chunk's
+// p . Ap partials sum each column's per-thread partials in thread order.
+template <int CH>
+__global__ void k_spmv_pap2(int k, const Chunk *__restrict__ ch, const int *__restrict__ rp,
+                            const int *__restrict__ ci, const double *__restrict__ va, const double *__restrict__ p,
+                            double *__restrict__ Ap, double *part, const int *alldone, const int *__restrict__ coldone) {
+    extern __shared__ double red[];  // 2 blockDim.x values
+    if (stopped(alldone)) return;
+    const Chunk c = ch[blockIdx.x];
+    const int kp = k >> 1, sh = __builtin_ctz(kp), cp = (int)(threadIdx.x & (unsigned)(kp - 1));
+    const bool run0 = !coldone[c.sys * k + 2 * cp], run1 = !coldone[c.sys * k + 2 * cp + 1];
+    double acc0 = 0.0, acc1 = 0.0;
+    if (run0 | run1) {
+        for (size_t e = (size_t)c.r0 * kp + threadIdx.x; e < (size_t)c.r1 * kp; e += blockDim.x) {
+            const int i = (int)(e >> sh);
+            const int b = rp[i], en = rp[i + 1];
+            double s0 = 0.0, s1 = 0.0;
+            for (int j0 = b; j0 < en; j0 += CH) {
+                double2 xv[CH];
+                double av[CH];
+#pragma unroll
+                for (int q = 0; q < CH; q++) {
+                    const int j = j0 + q < en ? j0 + q : en - 1;
+                    av[q] = va[j];
+                    xv[q] = *reinterpret_cast<const double2 *>(p + (size_t)ci[j] * k + 2 * cp);
+                }
+#pragma unroll
+                for (int q = 0; q < CH; q++)
+                    if (j0 + q < en) {
+                        s0 = fma(av[q], xv[q].x, s0);
+                        s1 = fma(av[q], xv[q].y, s1);
+                    }
+            }
+            reinterpret_cast<double2 *>(Ap)[e] = make_double2(s0, s1);
+            const double2 pe = reinterpret_cast<const double2 *>(p)[e];
+            acc0 = fma(pe.x, s0, acc0);
+            acc1 = fma(pe.y, s1, acc1);
+        }
+    }
+    red[2 * threadIdx.x] = run0 ? acc0 : 0.0;
+    red[2 * threadIdx.x + 1] = run1 ? acc1 : 0.0;
+    __syncthreads();
+    if (threadIdx.x < (unsigned)k) {
+        const int pc = (int)threadIdx.x >> 1, hf = (int)threadIdx.x & 1;
+        double s = 0.0;
+        for (int q = pc; q < (int)blockDim.x; q += kp) s += red[2 * q + hf];
+        part[(size_t)blockIdx.x * k + threadIdx.x] = s;
+    }
+}
+
 // x += alpha p, r -= alpha Ap, r32 = (float) r and the chunk's r . r partials
 __global__ void k_update_xr_rr(int k, const Chunk *__restrict__ ch, double *x, double *r,
                                const double *__restrict__ p, const double *__restrict__ Ap,
@@ -489,7 +540,13 @@ static int batch_iteration(baas_batch *B, int cur) {
     const dim3 grid(B->nchunks), block(256);
     const size_t red = 256 * sizeof(double);
     const int w = 2 * B->s->fwd.max_deg + 1;  // the symmetric pattern's longest row
-    if (w <= 5)
+    if (k >= 2 && w <= 5)  // two columns a thread (k is a power of two)
+        hipLaunchKernelGGL(k_spmv_pap2<5>, grid, block, 2 * red, B->st, k, B->chunks.p, B->rowptr, B->colidx,
+                           B->a_vals, B->p.p, B->Ap.p, B->part.p, ctl, cd);
+    else if (k >= 2 && w <= 7)
+        hipLaunchKernelGGL(k_spmv_pap2<7>, grid, block, 2 * red, B->st, k, B->chunks.p, B->rowptr, B->colidx,
+                           B->a_vals, B->p.p, B->Ap.p, B->part.p, ctl, cd);
+    else if (w <= 5)
         hipLaunchKernelGGL(k_spmv_pap<5>, grid, block, red, B->st, k, B->chunks.p, B->rowptr, B->colidx, B->a_vals,
                            B->p.p, B->Ap.p, B->part.p, ctl, cd);
     else if (w <= 7)
