@@ -338,12 +338,33 @@ __device__ __forceinline__ void pipe_wave(const PipeArgs &a, int lane, int x0, i
     }
     const int oz0 = oz ? fsz.x + fsz.y + i + tx * j : dmp;
     const int tz = kz - 1 + i + j;
-    auto publish_a = [&](int t, const T *v) {
-        const int o = (unsigned)(t - aA) < (unsigned)nA ? t * sA + cA : dmp;
-#pragma unroll
-        for (int r = 0; r < NR; r++) st_agent(opub + (unsigned)(o * NR + r), v[r]);
+    // Offsets in bytes from opub (a 32-bit offset from the box's uniform base
+    // address), advanced a step at a time rather than multiplied out: pob is
+    // (t sA + cA) face values at step t, taken modulo 2^32, exact at the
+    // steps that publish; pt is t - aA.
+    constexpr unsigned VB = NR * sizeof(T);  // one face value
+    const unsigned dmpb = (unsigned)dmp * VB, oz0b = (unsigned)oz0 * VB, psb = (unsigned)sA * VB;
+    unsigned pob = (unsigned)cA * VB;
+    int pt = -aA;
+    auto st_pub = [&](unsigned boff, T v) {
+        st_agent(reinterpret_cast<T *>(reinterpret_cast<char *>(opub) + boff), v);
     };
-    auto slot = [&](int k) { return lok & ((unsigned)k < (unsigned)kz) ? qb + k * txy : RMX; };
+    auto publish_a = [&](const T *v) {  // this step's store, then on to the next step
+        const unsigned ob = (unsigned)pt < (unsigned)nA ? pob : dmpb;
+#pragma unroll
+        for (int r = 0; r < NR; r++) st_pub(ob + r * (unsigned)sizeof(T), v[r]);
+        pob += psb;
+        pt++;
+    };
+    // the record slot of the row the next load reads (plane kk, slot qk),
+    // advanced a plane at a time
+    int kk = -i - j, qk = qb + (-i - j) * txy;
+    auto slot_next = [&]() {
+        const int q = lok & ((unsigned)kk < (unsigned)kz) ? qk : RMX;
+        kk++;
+        qk += txy;
+        return q;
+    };
     // Staged face slots advance by xst (yst) a step.  A lane reads them at
     // every step but uses them only at the steps it has a row (i = 0 or
     // j = 0, 0 <= k < kz); at the others the slot is still inside the staged
@@ -357,9 +378,9 @@ __device__ __forceinline__ void pipe_wave(const PipeArgs &a, int lane, int x0, i
             R.ex[r] = ev[R.ax * NR + r];
         }
     };
-    auto load = [&](int k, PipeRec<T, NR> &R) {
+    auto load = [&](PipeRec<T, NR> &R) {  // the rows of planes -i-j, -i-j+1, ... in turn
         R.rdy = ld_ready(ready);
-        const int q = slot(k);
+        const int q = slot_next();
         R.q = q;
         R.cz = c0[q];
         if (XS != 1) R.cy = c1[q];
@@ -401,7 +422,7 @@ __device__ __forceinline__ void pipe_wave(const PipeArgs &a, int lane, int x0, i
         }
         // the next step's inputs, in flight while this step computes (after
         // the wait above, so the common path never waits on these reads)
-        load(k + 1, N);
+        load(N);
         // the parents: z from this lane, y from lane - 1, x from lane - TY
         // (staged values on the box's faces)
         // The lane moves run on every lane before any select: a cross-lane
@@ -475,14 +496,14 @@ __device__ __forceinline__ void pipe_wave(const PipeArgs &a, int lane, int x0, i
         T pv[NR];
 #pragma unroll
         for (int r = 0; r < NR; r++) pv[r] = cy_help ? vx[r] : xl[r];
-        publish_a(t, pv);
+        publish_a(pv);
         if (t >= kz - 1)
 #pragma unroll
-            for (int r = 0; r < NR; r++) st_agent(opub + (unsigned)((t == tz ? oz0 : dmp) * NR + r), xl[r]);
+            for (int r = 0; r < NR; r++) st_pub((t == tz ? oz0b : dmpb) + r * (unsigned)sizeof(T), xl[r]);
     };
     // two register sets used in turn, so no step copies one into the other
     PipeRec<T, NR> A, B;
-    load(-i - j, A);
+    load(A);
     for (int t = 0; t < nsteps; t += 2) {
         step(t, A, B);
         if (t + 1 < nsteps) step(t + 1, B, A);
@@ -495,7 +516,7 @@ __device__ __forceinline__ void pipe_wave(const PipeArgs &a, int lane, int x0, i
             vc[r] = lane_read(xl[r], xaddr);
             __asm__ volatile("" : "+v"(vc[r]));
         }
-        if (cy_help) publish_a(nsteps, vc);
+        if (cy_help) publish_a(vc);  // step nsteps
     }
 }
 
