@@ -529,8 +529,8 @@ def main():
         cpu_baseline_worker(args.cpu_baseline_worker)
         return
     if args.gpus != 1:
-        print("bench.py measures one GPU; this build has no multi-GPU path (DESIGN.md explains how the "
-              "solve would shard)", file=sys.stderr)
+        print("bench.py measures one GPU: the library runs a solve on one GPU and has no multi-GPU path "
+              "(no collectives or cross-GPU publication); run with --gpus 1", file=sys.stderr)
         sys.exit(2)
     import torch
 
